@@ -1,0 +1,93 @@
+"""ctypes binding of ``libhiten_hip.so`` (the C-ABI declared in include/hiten_hip.h).
+
+torch is imported before the library is opened so that the process has a
+single HIP runtime: torch ships ``libamdhip64.so`` with the same SONAME the
+library links against, and the dynamic loader reuses the one already mapped.
+
+There is no fallback: if the library is missing, or no GPU is visible, every
+entry point raises :class:`HipUnavailable`.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import torch  # noqa: F401  (must precede the library load, see module docstring)
+
+from .build import LIB
+
+HH_OK = 0
+HH_VARIANT_PLAIN = 0
+HH_VARIANT_PERSISTENT = 1
+HH_STATUS_OK, HH_STATUS_MAXSTEPS, HH_STATUS_NONFINITE = 0, 1, 2
+ABI_VERSION = 1
+
+
+class HipUnavailable(RuntimeError):
+    """The HIP library or a GPU is not available."""
+
+
+class HipLibraryError(RuntimeError):
+    """An entry point of libhiten_hip.so returned an error code."""
+
+
+_P = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+_LL = C.c_longlong
+_ULL = C.c_ulonglong
+
+_SIGNATURES = {
+    "hh_last_error": ([], C.c_char_p),
+    "hh_abi_version": ([], _I),
+    "hh_rk8_fixed": ([_I, _P, _P, _P, _I, _I, _ULL, _P, _P, _P, _I, _P], _I),
+    "hh_dop853_final": ([_I, _I, _P, _P, _P, _D, _D, _D, _D, _LL, _I, _ULL, _P, _P, _P, _P, _P,
+                         _I, _I, _I, _P, _P], _I),
+    "hh_dop853_event": ([_I, _P, _P, _P, _D, _D, _D, _D, _LL, _I, _D, _I, _D, _D, _I, _ULL, _P,
+                         _P, _P, _P, _P, _P, _P, _P, _I, _P], _I),
+    "hh_eval_rhs": ([_I, _I, _P, _P, _P, _P], _I),
+    "hh_eval_math": ([_I, _P, _P, _P, _P], _I),
+}
+
+EXPORTED = tuple(_SIGNATURES)
+
+_lock = threading.Lock()
+_lib = None
+
+
+def library_path() -> str:
+    return LIB
+
+
+def load(require_gpu: bool = True):
+    """Open the library (once).  Raises HipUnavailable if it cannot be used."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB):
+                raise HipUnavailable(
+                    f"{LIB} is missing: build it with `python -m hiten_amd.build`")
+            lib = C.CDLL(LIB)
+            for name, (argt, rest) in _SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.argtypes = argt
+                fn.restype = rest
+            if lib.hh_abi_version() != ABI_VERSION:
+                raise HipUnavailable("libhiten_hip.so ABI version mismatch; rebuild it")
+            _lib = lib
+    if require_gpu and not torch.cuda.is_available():
+        raise HipUnavailable("no HIP device is visible to this process")
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != HH_OK:
+        msg = _lib.hh_last_error().decode(errors="replace") if _lib is not None else ""
+        raise HipLibraryError(f"{what} failed (code {rc}): {msg}")
+
+
+def stream_handle(stream=None) -> int:
+    s = torch.cuda.current_stream() if stream is None else stream
+    return int(s.cuda_stream)

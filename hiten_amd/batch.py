@@ -1,0 +1,205 @@
+"""Batched GPU propagation: thin Python wrappers over the C-ABI.
+
+Inputs may be NumPy arrays or torch tensors; trajectories are rows
+(``y0.shape == (n, dim)``, the reference's per-trajectory vector layout).
+Device buffers are structure-of-arrays (``[dim][n]``); results are returned as
+torch tensors on the GPU (rows = trajectories) unless ``to_numpy=True``.
+
+Every function here runs the HIP kernels of ``libhiten_hip.so``; there is no
+CPU path.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+
+EPS = float(np.finfo(float).eps)
+MIN_STEP = 10.0 * EPS                  # integrators/rk.py:833-834
+DEFAULT_MAX_STEPS = 1_000_000
+ALL_COMPONENTS = (1 << 64) - 1
+
+
+def flip_mask(dim: int, flip_indices=None) -> int:
+    """Bit mask of the components a backward-time _DirectedSystem negates.
+
+    ``None`` flips every component (dynamics/base.py:298-300).
+    """
+    if flip_indices is None:
+        return ALL_COMPONENTS
+    if isinstance(flip_indices, slice):
+        idx = range(dim)[flip_indices]
+    else:
+        idx = [int(i) for i in np.atleast_1d(flip_indices)]
+    m = 0
+    for i in idx:
+        if not 0 <= i < dim:
+            raise ValueError(f"flip index {i} out of range for dim {dim}")
+        m |= 1 << i
+    return m
+
+
+def _device() -> torch.device:
+    _lib.load()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _vec(x, n: int, name: str) -> torch.Tensor:
+    """(n,) float64 device tensor from a scalar or array."""
+    dev = _device()
+    if isinstance(x, torch.Tensor):
+        t = x.to(device=dev, dtype=torch.float64)
+    else:
+        t = torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev)
+    if t.ndim == 0:
+        t = t.expand(n)
+    if t.shape != (n,):
+        raise ValueError(f"{name} must be a scalar or have shape ({n},), got {tuple(t.shape)}")
+    return t.contiguous()
+
+
+def _soa(y0, dim: int) -> torch.Tensor:
+    """(n, dim) rows -> contiguous (dim, n) device tensor."""
+    dev = _device()
+    if isinstance(y0, torch.Tensor):
+        y = y0.to(device=dev, dtype=torch.float64)
+    else:
+        y = torch.as_tensor(np.asarray(y0, dtype=np.float64), device=dev)
+    if y.ndim == 1:
+        y = y[None, :]
+    if y.ndim != 2 or y.shape[1] != dim:
+        raise ValueError(f"states must have shape (n, {dim}), got {tuple(y.shape)}")
+    return y.t().contiguous()
+
+
+def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
+    return None if t is None else t.data_ptr()
+
+
+def _out(t: torch.Tensor, to_numpy: bool):
+    return t.cpu().numpy() if to_numpy else t
+
+
+@dataclass
+class FinalResult:
+    """DOP853 final-value propagation of a batch."""
+    y: torch.Tensor              # (n, dim) state at tf (dense-output value)
+    status: torch.Tensor         # (n,) int32, 0 ok / 1 max_steps / 2 non-finite
+    n_accepted: torch.Tensor     # (n,) int32
+    n_rejected: torch.Tensor     # (n,) int32
+
+
+@dataclass
+class EventResult:
+    hit: torch.Tensor            # (n,) bool
+    t_hit: torch.Tensor          # (n,)
+    y_hit: torch.Tensor          # (n, 6)
+    y_last: torch.Tensor         # (n, 6)
+    status: torch.Tensor
+    n_accepted: torch.Tensor
+    n_rejected: torch.Tensor
+
+
+def rk8_fixed(mu, y0, t0, tf, steps: int, forward: int = 1, flip_indices=None,
+              return_trajectory: bool = False, block: int = 0, stream=None):
+    """Fixed-step RK8 over numpy.linspace(t0, tf, steps) (rk.py:531-588).
+
+    Returns the final states (n, 6); with ``return_trajectory`` also the
+    grid states as an (n, steps, 6) view.
+    """
+    lib = _lib.load()
+    y = _soa(y0, 6)
+    n = y.shape[1]
+    mu_t, t0_t, tf_t = _vec(mu, n, "mu"), _vec(t0, n, "t0"), _vec(tf, n, "tf")
+    out = torch.empty_like(y)
+    traj = torch.empty((steps, 6, n), dtype=torch.float64, device=y.device) if return_trajectory else None
+    rc = lib.hh_rk8_fixed(n, _ptr(mu_t), _ptr(t0_t), _ptr(tf_t), int(steps), int(forward),
+                          flip_mask(6, flip_indices), _ptr(y), _ptr(out), _ptr(traj), int(block),
+                          _lib.stream_handle(stream))
+    _lib.check(rc, "hh_rk8_fixed")
+    if return_trajectory:
+        return out.t(), traj.permute(2, 0, 1)
+    return out.t()
+
+
+def dop853_final(mu, y0, t0, tf, *, dim: int = 6, rtol: float = 1e-12, atol: float = 1e-12,
+                 max_step: float = 1e4, min_step: float = MIN_STEP,
+                 max_steps: int = DEFAULT_MAX_STEPS, forward: int = 1, flip_indices=None,
+                 variant: str = "plain", block: int = 0, grid: int = 0,
+                 stream=None) -> FinalResult:
+    """Adaptive DOP853 to tf; the state returned is the dense-output value at tf
+    (the last row of _propagate_dynsys(..., steps=2)).
+    """
+    lib = _lib.load()
+    y = _soa(y0, dim)
+    n = y.shape[1]
+    mu_t, t0_t, tf_t = _vec(mu, n, "mu"), _vec(t0, n, "t0"), _vec(tf, n, "tf")
+    out = torch.empty_like(y)
+    i32 = dict(dtype=torch.int32, device=y.device)
+    status, nacc, nrej = torch.empty(n, **i32), torch.empty(n, **i32), torch.empty(n, **i32)
+    counter = torch.zeros(1, **i32)
+    var = {"plain": _lib.HH_VARIANT_PLAIN, "persistent": _lib.HH_VARIANT_PERSISTENT}[variant]
+    rc = lib.hh_dop853_final(n, dim, _ptr(mu_t), _ptr(t0_t), _ptr(tf_t), float(rtol), float(atol),
+                             float(max_step), float(min_step), int(max_steps), int(forward),
+                             flip_mask(dim, flip_indices), _ptr(y), _ptr(out), _ptr(status),
+                             _ptr(nacc), _ptr(nrej), var, int(block), int(grid), _ptr(counter),
+                             _lib.stream_handle(stream))
+    _lib.check(rc, "hh_dop853_final")
+    return FinalResult(out.t(), status, nacc, nrej)
+
+
+def dop853_event(mu, y0, t0, tmax, *, ev_index: int = 1, ev_offset: float = 0.0,
+                 direction: int = 0, rtol: float = 1e-12, atol: float = 1e-12,
+                 max_step: float = math.inf, min_step: float = MIN_STEP,
+                 max_steps: int = DEFAULT_MAX_STEPS, xtol: float = 1e-12, gtol: float = 1e-12,
+                 forward: int = 1, flip_indices=None, block: int = 0,
+                 stream=None) -> EventResult:
+    """DOP853 until the first crossing of y[ev_index] = ev_offset (rk.py:2678-2803)."""
+    lib = _lib.load()
+    y = _soa(y0, 6)
+    n = y.shape[1]
+    mu_t, t0_t, tm_t = _vec(mu, n, "mu"), _vec(t0, n, "t0"), _vec(tmax, n, "tmax")
+    i32 = dict(dtype=torch.int32, device=y.device)
+    hit = torch.empty(n, **i32)
+    th = torch.empty(n, dtype=torch.float64, device=y.device)
+    yh = torch.empty_like(y)
+    yl = torch.empty_like(y)
+    status, nacc, nrej = torch.empty(n, **i32), torch.empty(n, **i32), torch.empty(n, **i32)
+    rc = lib.hh_dop853_event(n, _ptr(mu_t), _ptr(t0_t), _ptr(tm_t), float(rtol), float(atol),
+                             float(max_step), float(min_step), int(max_steps), int(ev_index),
+                             float(ev_offset), int(direction), float(xtol), float(gtol),
+                             int(forward), flip_mask(6, flip_indices), _ptr(y), _ptr(hit),
+                             _ptr(th), _ptr(yh), _ptr(yl), _ptr(status), _ptr(nacc), _ptr(nrej),
+                             int(block), _lib.stream_handle(stream))
+    _lib.check(rc, "hh_dop853_event")
+    return EventResult(hit.bool(), th, yh.t(), yl.t(), status, nacc, nrej)
+
+
+def eval_rhs(mu, y, dim: int = 6, stream=None) -> torch.Tensor:
+    """Right-hand side at each row of y (rtbp.py:30-74 for dim 6, 167-255 for 42)."""
+    lib = _lib.load()
+    ys = _soa(y, dim)
+    n = ys.shape[1]
+    mu_t = _vec(mu, n, "mu")
+    out = torch.empty_like(ys)
+    _lib.check(lib.hh_eval_rhs(n, dim, _ptr(mu_t), _ptr(ys), _ptr(out), _lib.stream_handle(stream)),
+               "hh_eval_rhs")
+    return out.t()
+
+
+def eval_math(x, y, stream=None) -> torch.Tensor:
+    """(5, n): sqrt(x), x/y, x**1.5, x**2.5, x**y as the kernels compute them."""
+    lib = _lib.load()
+    xt = _vec(x, len(x), "x")
+    yt = _vec(y, len(x), "y")
+    n = xt.shape[0]
+    out = torch.empty((5, n), dtype=torch.float64, device=xt.device)
+    _lib.check(lib.hh_eval_math(n, _ptr(xt), _ptr(yt), _ptr(out), _lib.stream_handle(stream)),
+               "hh_eval_math")
+    return out

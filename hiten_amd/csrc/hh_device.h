@@ -1,0 +1,496 @@
+/* hh_device.h -- device-side building blocks of the CR3BP propagation
+ * kernels for gfx950 (CDNA4, wave64).
+ *
+ * Every arithmetic expression here is written in the operation order of
+ * hiten's numba kernels (fastmath=False), and the library is compiled with
+ * -ffp-contract=off, so that a kernel's FP64 results are bit-identical to a
+ * scalar CPU evaluation of the reference formulas:
+ *   rhs6            dynamics/rtbp.py:65-74   (_crtbp_accel)
+ *   Geom42          dynamics/rtbp.py:115-141, 227-242 (Jacobian entries and
+ *                   the state part of _var_equations)
+ *   column_rhs      dynamics/rtbp.py:219-225 (F @ Phi; the zero entries of F
+ *                   contribute exactly +-0 to the reference's running sum, so
+ *                   only the non-zero products are formed, starting the sum
+ *                   from +0.0 exactly as the reference does)
+ *   DOP853 stepping integrators/rk.py:1636-1708, 1790-2003, 2005-2102,
+ *                   2443-2484; integrators/utils.py:13-296
+ *
+ * Two lane layouts share the stepping code through a policy class:
+ *   Lane6   one trajectory per lane, the whole 6-d state in registers.
+ *   Lane42  one trajectory per 8-lane group; lane b of the group owns one
+ *           6-double block of the 42-d variational state (b = 0: the state
+ *           x, b = 1..6: STM column b-1, b = 7: idle padding).  The state
+ *           block is broadcast to the group with ds_swizzle at every stage,
+ *           error norms are reduced across the group with an xor butterfly.
+ */
+#ifndef HH_DEVICE_H
+#define HH_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HH_TABLE static __device__ const
+#define HH_FN static __device__ __forceinline__
+#include "hh_pow.h"
+
+namespace hh {
+
+constexpr int DOP_S = 12;     /* stages of one DOP853 step                 */
+constexpr int DOP_K = 13;     /* stored stage derivatives (k[12] = f(y_high)) */
+constexpr int DOP_SX = 16;    /* stages of the dense-output extension       */
+constexpr int DOP_P = 7;      /* interpolant rows                           */
+constexpr int RK8_S = 13;
+
+constexpr int ST_OK = 0;
+constexpr int ST_MAXSTEPS = 1;
+constexpr int ST_NONFINITE = 2;
+
+__device__ __forceinline__ double np_max(double a, double b) {
+    /* numpy.maximum: NaN-propagating */
+    if (a != a || b != b) return a + b;
+    return a >= b ? a : b;
+}
+
+__device__ __forceinline__ bool isclose(double a, double b) {
+    /* numpy.isclose defaults (rtol 1e-5, atol 1e-8), dynamics/base.py:421 */
+    return __builtin_fabs(a - b) <= 1e-8 + 1e-5 * __builtin_fabs(b);
+}
+
+/* ds_swizzle in bitmask mode: lane reads ((lane & AND) | OR) ^ XOR within
+ * its 32-lane half.  Used on the two halves of a double.                  */
+template <int PATTERN>
+__device__ __forceinline__ double swz(double v) {
+    int lo = __double2loint(v);
+    int hi = __double2hiint(v);
+    lo = __builtin_amdgcn_ds_swizzle(lo, PATTERN);
+    hi = __builtin_amdgcn_ds_swizzle(hi, PATTERN);
+    return __hiloint2double(hi, lo);
+}
+
+/* value held by lane K of this lane's aligned 8-lane group */
+template <int K>
+__device__ __forceinline__ double group_get(double v) { return swz<0x18 | (K << 5)>(v); }
+
+template <int X>
+__device__ __forceinline__ double group_xor(double v) { return swz<0x1f | (X << 10)>(v); }
+
+/* ---------------------------------------------------------------------- */
+/* CR3BP vector fields                                                      */
+/* ---------------------------------------------------------------------- */
+
+/* dynamics/rtbp.py:65-74 */
+__device__ __forceinline__ void rhs6(const double (&s)[6], double mu, double (&o)[6]) {
+    const double x = s[0], y = s[1], z = s[2], vx = s[3], vy = s[4], vz = s[5];
+    const double xm = x + mu;
+    const double r1 = __builtin_sqrt((xm * xm + y * y) + z * z);
+    const double one_mu = 1.0 - mu;
+    const double xr = x - one_mu;
+    const double r2 = __builtin_sqrt((xr * xr + y * y) + z * z);
+    const double r1c = r1 * (r1 * r1);
+    const double r2c = r2 * (r2 * r2);
+    const double xm1 = (x - 1.0) + mu;
+    o[0] = vx;
+    o[1] = vy;
+    o[2] = vz;
+    o[3] = ((2.0 * vy + x) - (one_mu * xm) / r1c) - (mu * xm1) / r2c;
+    o[4] = ((-2.0 * vx + y) - (one_mu * y) / r1c) - (mu * y) / r2c;
+    o[5] = ((-one_mu) * z) / r1c - (mu * z) / r2c;
+}
+
+/* Quantities of _var_equations that depend only on the state x:
+ * the Hessian entries of the Jacobian (rtbp.py:115-141) and the state
+ * acceleration (rtbp.py:227-242).                                          */
+struct Geom42 {
+    double uxx, uyy, uzz, uxy, uxz, uyz;
+    double ax, ay, az;
+};
+
+__device__ __forceinline__ Geom42 geom42(double x, double y, double z, double vx, double vy,
+                                         double mu, double mu2) {
+    Geom42 g;
+    const double xm = x + mu;
+    const double xr = x - mu2;
+    const double r2 = (xm * xm + y * y) + z * z;
+    const double R2 = (xr * xr + y * y) + z * z;
+    const double r3 = hh_pow15(r2);
+    const double r5 = hh_pow25(r2);
+    const double R3 = hh_pow15(R2);
+    const double R5 = hh_pow25(R2);
+    const double a5 = mu2 / r5;
+    const double b5 = mu / R5;
+    const double c3 = mu2 / r3 + mu / R3;
+    const double a53 = a5 * 3.0;
+    const double b53 = b5 * 3.0;
+    g.uxx = ((1.0 + a53 * (xm * xm)) + b53 * (xr * xr)) - c3;
+    g.uyy = ((1.0 + a53 * (y * y)) + b53 * (y * y)) - c3;
+    g.uzz = ((0.0 + a53 * (z * z)) + b53 * (z * z)) - c3;
+    const double gx = (mu2 * xm) / r5 + (mu * xr) / R5;
+    g.uxy = (3.0 * y) * gx;
+    g.uxz = (3.0 * z) * gx;
+    g.uyz = ((3.0 * y) * z) * (a5 + b5);
+    g.ax = ((x - mu2 * (xm / r3)) - mu * (xr / R3)) + 2.0 * vy;
+    g.ay = ((y - mu2 * (y / r3)) - mu * (y / R3)) - 2.0 * vx;
+    g.az = (-mu2 * (z / r3)) - mu * (z / R3);
+    return g;
+}
+
+/* One STM column p = Phi[:, j] -> (F @ Phi)[:, j]. */
+__device__ __forceinline__ void column_rhs(const Geom42& g, const double (&p)[6], double (&o)[6]) {
+    o[0] = 0.0 + p[3];
+    o[1] = 0.0 + p[4];
+    o[2] = 0.0 + p[5];
+    o[3] = (((0.0 + g.uxx * p[0]) + g.uxy * p[1]) + g.uxz * p[2]) + 2.0 * p[4];
+    o[4] = (((0.0 + g.uxy * p[0]) + g.uyy * p[1]) + g.uyz * p[2]) + (-2.0) * p[3];
+    o[5] = ((0.0 + g.uxz * p[0]) + g.uyz * p[1]) + g.uzz * p[2];
+}
+
+/* ---------------------------------------------------------------------- */
+/* lane layout policies                                                     */
+/* ---------------------------------------------------------------------- */
+
+/* One trajectory per lane, 6-d state. */
+struct Lane6 {
+    static constexpr int N = 6;
+    double mu;
+    double sgn[6];      /* _DirectedSystem flip signs: +1 or -1 per component */
+
+    __device__ __forceinline__ void rhs(const double (&s)[6], double (&o)[6]) const {
+        rhs6(s, mu, o);
+#pragma unroll
+        for (int d = 0; d < 6; ++d) o[d] = o[d] * sgn[d];
+    }
+    __device__ __forceinline__ double reduce(double v) const { return v; }
+    __device__ __forceinline__ bool group_any(bool v) const { return v; }
+};
+
+/* One trajectory per aligned group of 8 lanes, 42-d variational state. */
+struct Lane42 {
+    static constexpr int N = 42;
+    double mu, mu2;
+    double sgn[6];
+    int blk;            /* lane & 7 */
+
+    __device__ __forceinline__ void rhs(const double (&s)[6], double (&o)[6]) const {
+        const double x = group_get<0>(s[0]);
+        const double y = group_get<0>(s[1]);
+        const double z = group_get<0>(s[2]);
+        const double vx = group_get<0>(s[3]);
+        const double vy = group_get<0>(s[4]);
+        const double vz = group_get<0>(s[5]);
+        const Geom42 g = geom42(x, y, z, vx, vy, mu, mu2);
+        double c[6];
+        column_rhs(g, s, c);
+        const bool st = (blk == 0);
+        o[0] = st ? vx : c[0];
+        o[1] = st ? vy : c[1];
+        o[2] = st ? vz : c[2];
+        o[3] = st ? g.ax : c[3];
+        o[4] = st ? g.ay : c[4];
+        o[5] = st ? g.az : c[5];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) o[d] = o[d] * sgn[d];
+    }
+    /* canonical block tree: ((b0+b1)+(b2+b3)) + ((b4+b5)+(b6+b7)), b7 = 0 */
+    __device__ __forceinline__ double reduce(double v) const {
+        v = (blk == 7) ? 0.0 : v;
+        v = v + group_xor<1>(v);
+        v = v + group_xor<2>(v);
+        v = v + group_xor<4>(v);
+        return v;
+    }
+};
+
+/* component index of lane-block b, entry d, in the reference's 42-vector
+ * layout [Phi row-major (36) | x (6)]                                       */
+__device__ __forceinline__ int comp42(int b, int d) { return b == 0 ? 36 + d : 6 * d + (b - 1); }
+
+/* ---------------------------------------------------------------------- */
+/* step control: integrators/utils.py                                       */
+/* ---------------------------------------------------------------------- */
+
+__device__ __forceinline__ double select_initial_step(double d0, double d1, double min_step,
+                                                      double max_step) {
+    double h = (d0 < 1.0e-5 || d1 < 1.0e-5) ? 1.0e-6 : (0.01 * d0) / d1;
+    if (h > max_step) h = max_step;
+    if (h < min_step) h = min_step;
+    return h;
+}
+
+__device__ __forceinline__ double clamp_step(double h, double max_step, double min_step) {
+    if (h > max_step) h = max_step;
+    if (h < min_step) h = min_step;
+    return h;
+}
+
+__device__ __forceinline__ double adjust_step_to_endpoint(double t, double h, double t_end) {
+    return (t + h > t_end) ? __builtin_fabs(t_end - t) : h;
+}
+
+/* beta = 1/9, alpha = 0.4/9 as the reference computes them for order 8 */
+__device__ __forceinline__ double pi_accept_factor(double err, double err_prev) {
+    const double beta = 1.0 / (8.0 + 1.0);
+    const double alpha = 0.4 * beta;
+    double f;
+    if (err == 0.0) f = 10.0;
+    else if (err_prev < 0.0) f = 0.9 * hh_pow(err, -beta);
+    else f = (0.9 * hh_pow(err, -beta)) * hh_pow(err_prev, alpha);
+    if (!(f == f)) f = 10.0;
+    if (f < 0.2) f = 0.2;
+    if (f > 10.0) f = 10.0;
+    return f;
+}
+
+__device__ __forceinline__ double pi_reject_factor(double err) {
+    const double err_exp = 1.0 / 8.0;
+    double f = (err <= 0.0) ? 0.2 : 0.9 * hh_pow(err, -err_exp);
+    if (!(f == f)) f = 0.2;
+    if (f < 0.2) f = 0.2;
+    if (f > 10.0) f = 10.0;
+    return f;
+}
+
+/* rk.py:2443-2448 */
+template <class P>
+__device__ __forceinline__ double initial_step(const P& p, const double (&y)[6],
+                                               const double (&f0)[6], double rtol, double atol,
+                                               double min_step, double max_step) {
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const double sc = atol + rtol * __builtin_fabs(y[d]);
+        const double a = y[d] / sc;
+        const double b = f0[d] / sc;
+        s0 += a * a;
+        s1 += b * b;
+    }
+    s0 = p.reduce(s0);
+    s1 = p.reduce(s1);
+    const double sn = __builtin_sqrt((double)P::N);
+    const double d0 = __builtin_sqrt(s0) / sn;
+    const double d1 = __builtin_sqrt(s1) / sn;
+    return select_initial_step(d0, d1, min_step, max_step);
+}
+
+/* ---------------------------------------------------------------------- */
+/* DOP853 step attempt (rk.py:1636-1699) + scaled error norm (2457-2467)   */
+/* k[0] must hold f(y) on entry; k[1..12] and yh are written.               */
+/* ---------------------------------------------------------------------- */
+template <class P>
+__device__ __forceinline__ double dop853_attempt(const P& p, const double (&y)[6],
+                                                 double (&k)[DOP_K][6], double h, double rtol,
+                                                 double atol, double (&yh)[6]) {
+#pragma unroll
+    for (int i = 1; i < DOP_S; ++i) {
+        double ys[6];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) ys[d] = y[d];
+#pragma unroll
+        for (int j = 0; j < i; ++j) {
+            const double a = HH_DOP_A[i][j];
+            if (a != 0.0) {
+                const double ha = h * a;
+#pragma unroll
+                for (int d = 0; d < 6; ++d) ys[d] = ys[d] + ha * k[j][d];
+            }
+        }
+        p.rhs(ys, k[i]);
+    }
+#pragma unroll
+    for (int d = 0; d < 6; ++d) yh[d] = y[d];
+#pragma unroll
+    for (int j = 0; j < DOP_S; ++j) {
+        const double b = HH_DOP_B[j];
+        if (b != 0.0) {
+            const double hb = h * b;
+#pragma unroll
+            for (int d = 0; d < 6; ++d) yh[d] = yh[d] + hb * k[j][d];
+        }
+    }
+    p.rhs(yh, k[DOP_S]);
+    double e5[6], e3[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        e5[d] = 0.0;
+        e3[d] = 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < DOP_K; ++j) {
+        const double c5 = HH_DOP_E5[j];
+        const double c3 = HH_DOP_E3[j];
+        if (c5 != 0.0) {
+#pragma unroll
+            for (int d = 0; d < 6; ++d) e5[d] = e5[d] + c5 * k[j][d];
+        }
+        if (c3 != 0.0) {
+#pragma unroll
+            for (int d = 0; d < 6; ++d) e3[d] = e3[d] + c3 * k[j][d];
+        }
+    }
+    double s5 = 0.0, s3 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const double sc = atol + rtol * np_max(__builtin_fabs(y[d]), __builtin_fabs(yh[d]));
+        const double q5 = (e5[d] * h) / sc;
+        const double q3 = (e3[d] * h) / sc;
+        s5 += q5 * q5;
+        s3 += q3 * q3;
+    }
+    s5 = p.reduce(s5);
+    s3 = p.reduce(s3);
+    if (s5 == 0.0 && s3 == 0.0) return 0.0;
+    const double denom = s5 + 0.01 * s3;
+    return (__builtin_fabs(h) * s5) / __builtin_sqrt(denom * (double)P::N);
+}
+
+/* Dense-output interpolant of one step (rk.py:1790-1875); Fc[7][6]. */
+template <class P>
+__device__ __forceinline__ void dop853_dense(const P& p, const double (&y_old)[6],
+                                             const double (&y_new)[6], double (&k)[DOP_K][6],
+                                             double hs, double (&Fc)[DOP_P][6]) {
+    double kx[DOP_SX - DOP_K][6];
+#pragma unroll
+    for (int sr = DOP_K; sr < DOP_SX; ++sr) {
+        double ys[6];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            double acc = 0.0;
+#pragma unroll
+            for (int r = 0; r < sr; ++r) {
+                const double a = HH_DOP_A[sr][r];
+                if (a != 0.0) acc = acc + a * (r < DOP_K ? k[r][d] : kx[r - DOP_K][d]);
+            }
+            ys[d] = y_old[d] + hs * acc;
+        }
+        p.rhs(ys, kx[sr - DOP_K]);
+    }
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const double dy = y_new[d] - y_old[d];
+        Fc[0][d] = dy;
+        Fc[1][d] = hs * k[0][d] - dy;
+        Fc[2][d] = 2.0 * dy - hs * (k[DOP_S][d] + k[0][d]);
+    }
+#pragma unroll
+    for (int i = 0; i < DOP_P - 3; ++i) {
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            double acc = 0.0;
+#pragma unroll
+            for (int r = 0; r < DOP_SX; ++r) {
+                const double c = HH_DOP_D[i][r];
+                if (c != 0.0) acc = acc + c * (r < DOP_K ? k[r][d] : kx[r - DOP_K][d]);
+            }
+            Fc[3 + i][d] = hs * acc;
+        }
+    }
+}
+
+/* rk.py:1989-2003 for one component */
+__device__ __forceinline__ double dense_eval1(double y_old, const double (&Fc)[DOP_P][6], int d,
+                                              double x) {
+    double v = 0.0;
+    const double omx = 1.0 - x;
+#pragma unroll
+    for (int i = DOP_P - 1; i >= 0; --i) {
+        v = v + Fc[i][d];
+        v = v * (((DOP_P - 1 - i) % 2 == 0) ? x : omx);
+    }
+    return v + y_old;
+}
+
+/* rk.py:2005-2102 with the event g = y[ev_index] - ev_offset.  Only the event
+ * component is interpolated inside the bisection; the full state is formed
+ * once at the returned abscissa (each component of the reference's y_mid is
+ * computed independently, so the result is the same).                     */
+__device__ __forceinline__ int crossed_direction(double gl, double gm, int dir) {
+    if (dir == 0) return (gl < 0.0 && gm > 0.0) || (gl > 0.0 && gm < 0.0);
+    if (dir > 0) return gl < 0.0 && gm > 0.0;
+    return gl > 0.0 && gm < 0.0;
+}
+
+__device__ __forceinline__ int event_crossed(double gp, double gn, int dir) {
+    if (dir == 0) return (gp < 0.0 && gn > 0.0) || (gp > 0.0 && gn < 0.0) || (gn == 0.0);
+    if (dir > 0) return (gp < 0.0 && gn > 0.0) || (gn == 0.0);
+    return (gp > 0.0 && gn < 0.0) || (gn == 0.0);
+}
+
+__device__ __forceinline__ double pick6(const double (&v)[6], int i) {
+    double r = v[0];
+#pragma unroll
+    for (int d = 1; d < 6; ++d) r = (i == d) ? v[d] : r;
+    return r;
+}
+
+__device__ __forceinline__ double refine_bisect(double y_ev0, const double (&Fc)[DOP_P][6], int ev,
+                                                double ev_off, int dir, double h, double xtol,
+                                                double gtol) {
+    /* returns the abscissa x in [0, 1] of the hit */
+    double a = 0.0, b = 1.0;
+    double gl = y_ev0 - ev_off;
+    double fc[DOP_P];
+#pragma unroll
+    for (int i = 0; i < DOP_P; ++i) {
+        double c = Fc[i][0];
+#pragma unroll
+        for (int d = 1; d < 6; ++d) c = (ev == d) ? Fc[i][d] : c;
+        fc[i] = c;
+    }
+    for (int it = 0; it < 128; ++it) {
+        const double mid = 0.5 * (a + b);
+        double v = 0.0;
+        const double omx = 1.0 - mid;
+#pragma unroll
+        for (int i = DOP_P - 1; i >= 0; --i) {
+            v = v + fc[i];
+            v = v * (((DOP_P - 1 - i) % 2 == 0) ? mid : omx);
+        }
+        const double gm = (v + y_ev0) - ev_off;
+        if (__builtin_fabs(gm) <= gtol) return mid;
+        if (crossed_direction(gl, gm, dir)) {
+            b = mid;
+        } else {
+            a = mid;
+            gl = gm;
+        }
+        if ((b - a) * __builtin_fabs(h) <= xtol) break;
+    }
+    return b;
+}
+
+/* ---------------------------------------------------------------------- */
+/* fixed-step RK8 (rk.py:154-215)                                           */
+/* ---------------------------------------------------------------------- */
+template <class P>
+__device__ __forceinline__ void rk8_step(const P& p, double (&y)[6], double h) {
+    double k[RK8_S][6];
+    p.rhs(y, k[0]);
+#pragma unroll
+    for (int i = 1; i < RK8_S; ++i) {
+        double ys[6];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) ys[d] = y[d];
+#pragma unroll
+        for (int j = 0; j < i; ++j) {
+            const double a = HH_RK8_A[i][j];
+            if (a != 0.0) {
+                const double ha = h * a;
+#pragma unroll
+                for (int d = 0; d < 6; ++d) ys[d] = ys[d] + ha * k[j][d];
+            }
+        }
+        p.rhs(ys, k[i]);
+    }
+#pragma unroll
+    for (int j = 0; j < RK8_S; ++j) {
+        const double b = HH_RK8_B[j];
+        if (b != 0.0) {
+            const double hb = h * b;
+#pragma unroll
+            for (int d = 0; d < 6; ++d) y[d] = y[d] + hb * k[j][d];
+        }
+    }
+}
+
+}  // namespace hh
+
+#endif /* HH_DEVICE_H */

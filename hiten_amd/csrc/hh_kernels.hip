@@ -1,0 +1,628 @@
+/* hh_kernels.hip -- HIP kernels and the C-ABI of the CR3BP propagation path.
+ *
+ * Kernels (gfx950, wave64):
+ *   k_rk8_fixed6        fixed-step RK8, 6-d, one trajectory per lane
+ *                       (integrators/rk.py:531-588 over numpy.linspace grid)
+ *   k_dop853_final6     DOP853 to t_f, 6-d, one trajectory per lane; the
+ *                       returned state is the dense-output value at t_f
+ *                       (rk.py:2375-2549 with t_eval = [t0, tf])
+ *   k_dop853_event6     DOP853 until the plane event y[i] = c, 6-d
+ *                       (rk.py:2678-2803, 2005-2102)
+ *   k_dop853_final42    DOP853 to t_f on the 42-d variational system, one
+ *                       trajectory per 8-lane group; `persistent` template
+ *                       flag selects the work-stealing form that refills a
+ *                       group from a global counter as soon as its
+ *                       trajectory ends
+ * State arrays are structure-of-arrays: component c of trajectory i lives at
+ * [c * n + i], c in the reference's order ([Phi row-major | x] for 42-d).
+ */
+#include "hh_device.h"
+#include "../../include/hiten_hip.h"
+
+#include <stdio.h>
+#include <string.h>
+
+using namespace hh;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* msg) {
+    snprintf(g_err, sizeof g_err, "%s", msg);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    snprintf(g_err, sizeof g_err, "%s: %s", what, hipGetErrorString(e));
+    return HH_ERR_HIP;
+}
+
+__device__ __forceinline__ double flip_sign(int fwd, unsigned long long flip, int c) {
+    return (fwd == -1 && ((flip >> c) & 1ULL)) ? -1.0 : 1.0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* per-trajectory DOP853 state machine, final-value mode                     */
+/* ------------------------------------------------------------------------ */
+struct StepParams {
+    double rtol, atol, max_step, min_step;
+    long long max_steps;
+};
+
+template <class P>
+struct Traj {
+    double y[6];
+    double k[DOP_K][6];
+    double t, tf, h, err_prev;
+    int nacc, nrej, status;
+    bool done;
+};
+
+/* Start a trajectory: k0 = f(y0), initial step (rk.py:2441-2448).  A span
+ * that numpy.isclose calls zero (dynamics/base.py:421) or a reversed span
+ * returns y0 unchanged, as the reference does.                              */
+template <class P>
+__device__ __forceinline__ void traj_init(const P& p, Traj<P>& s, double t0, double tf,
+                                          const StepParams& sp) {
+    s.t = t0;
+    s.tf = tf;
+    s.err_prev = -1.0;
+    s.nacc = 0;
+    s.nrej = 0;
+    s.status = ST_OK;
+    s.done = isclose(t0, tf) || !((t0 - tf) * 1.0 < 0.0);
+    p.rhs(s.y, s.k[0]);
+    s.h = initial_step(p, s.y, s.k[0], sp.rtol, sp.atol, sp.min_step, sp.max_step);
+}
+
+/* One pass of the `while (t - tf) < 0` loop of rk.py:2452-2484.  On the
+ * accepted step that reaches t_f the dense-output value at t_f is written to
+ * yout (rk.py:2503-2543 for t_eval[-1]) and the trajectory is marked done.  */
+template <class P>
+__device__ __forceinline__ void traj_iterate(const P& p, Traj<P>& s, const StepParams& sp,
+                                             double (&yout)[6]) {
+    if (s.nacc + s.nrej >= sp.max_steps) {
+        s.status = ST_MAXSTEPS;
+        s.done = true;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) yout[d] = s.y[d];
+        return;
+    }
+    double h = clamp_step(s.h, sp.max_step, sp.min_step);
+    h = adjust_step_to_endpoint(s.t, h, s.tf);
+    double yh[6];
+    const double err = dop853_attempt(p, s.y, s.k, h, sp.rtol, sp.atol, yh);
+    if (err != err) {
+        s.status = ST_NONFINITE;
+        s.done = true;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) yout[d] = s.y[d];
+        return;
+    }
+    if (err <= 1.0) {
+        const double t_new = s.t + h;
+        if (!((t_new - s.tf) * 1.0 < 0.0)) {
+            const double hs = t_new - s.t;
+            if (hs == 0.0) {
+#pragma unroll
+                for (int d = 0; d < 6; ++d) yout[d] = s.y[d];
+            } else {
+                double Fc[DOP_P][6];
+                dop853_dense(p, s.y, yh, s.k, hs, Fc);
+                const double x = (s.tf - s.t) / hs;
+#pragma unroll
+                for (int d = 0; d < 6; ++d) yout[d] = dense_eval1(s.y[d], Fc, d, x);
+            }
+            s.done = true;
+        }
+        s.t = t_new;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            s.y[d] = yh[d];
+            s.k[0][d] = s.k[DOP_S][d];
+        }
+        s.h = h * pi_accept_factor(err, s.err_prev);
+        s.err_prev = err;
+        s.nacc += 1;
+    } else {
+        s.h = clamp_step(h * pi_reject_factor(err), sp.max_step, sp.min_step);
+        s.nrej += 1;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* kernels                                                                   */
+/* ------------------------------------------------------------------------ */
+
+__global__ __launch_bounds__(256) void k_rk8_fixed6(int n, const double* __restrict__ mu,
+                                                    const double* __restrict__ t0a,
+                                                    const double* __restrict__ tfa, int steps,
+                                                    int fwd, unsigned long long flip,
+                                                    const double* __restrict__ y0,
+                                                    double* __restrict__ yout,
+                                                    double* __restrict__ traj) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Lane6 p;
+    p.mu = mu[i];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) p.sgn[d] = flip_sign(fwd, flip, d);
+    double y[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) y[d] = y0[(size_t)d * n + i];
+    const double t0 = t0a[i], tf = tfa[i];
+    /* numpy.linspace(t0, tf, steps) */
+    const double delta = tf - t0;
+    const double step = delta / (double)(steps - 1);
+    const bool zstep = (step == 0.0);
+    const bool constant = isclose(t0, tf);
+    if (traj) {
+#pragma unroll
+        for (int d = 0; d < 6; ++d) traj[((size_t)0 * 6 + d) * n + i] = y[d];
+    }
+    double t_prev = t0;
+    for (int s = 1; s < steps; ++s) {
+        double t_s;
+        if (s == steps - 1) t_s = tf;
+        else if (zstep) t_s = ((double)s / (double)(steps - 1)) * delta + t0;
+        else t_s = (double)s * step + t0;
+        if (!constant) rk8_step(p, y, t_s - t_prev);
+        t_prev = t_s;
+        if (traj) {
+#pragma unroll
+            for (int d = 0; d < 6; ++d) traj[((size_t)s * 6 + d) * n + i] = y[d];
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 6; ++d) yout[(size_t)d * n + i] = y[d];
+}
+
+__global__ __launch_bounds__(256) void k_dop853_final6(
+    int n, const double* __restrict__ mu, const double* __restrict__ t0a,
+    const double* __restrict__ tfa, StepParams sp, int fwd, unsigned long long flip,
+    const double* __restrict__ y0, double* __restrict__ yout, int* __restrict__ status,
+    int* __restrict__ nacc, int* __restrict__ nrej) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Lane6 p;
+    p.mu = mu[i];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) p.sgn[d] = flip_sign(fwd, flip, d);
+    Traj<Lane6> s;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) s.y[d] = y0[(size_t)d * n + i];
+    double yo[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) yo[d] = s.y[d];
+    traj_init(p, s, t0a[i], tfa[i], sp);
+    while (!s.done) traj_iterate(p, s, sp, yo);
+#pragma unroll
+    for (int d = 0; d < 6; ++d) yout[(size_t)d * n + i] = yo[d];
+    if (status) status[i] = s.status;
+    if (nacc) nacc[i] = s.nacc;
+    if (nrej) nrej[i] = s.nrej;
+}
+
+struct EventParams {
+    int index;
+    double offset;
+    int direction;
+    double xtol, gtol;
+};
+
+__global__ __launch_bounds__(256) void k_dop853_event6(
+    int n, const double* __restrict__ mu, const double* __restrict__ t0a,
+    const double* __restrict__ tmaxa, StepParams sp, EventParams ev, int fwd,
+    unsigned long long flip, const double* __restrict__ y0, int* __restrict__ hit,
+    double* __restrict__ t_hit, double* __restrict__ y_hit, double* __restrict__ y_last,
+    int* __restrict__ status, int* __restrict__ nacc, int* __restrict__ nrej) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Lane6 p;
+    p.mu = mu[i];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) p.sgn[d] = flip_sign(fwd, flip, d);
+    double y[6], k[DOP_K][6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) y[d] = y0[(size_t)d * n + i];
+    double t = t0a[i];
+    const double tmax = tmaxa[i];
+    p.rhs(y, k[0]);
+    double gp = pick6(y, ev.index) - ev.offset;
+    double h = initial_step(p, y, k[0], sp.rtol, sp.atol, sp.min_step, sp.max_step);
+    double err_prev = -1.0;
+    int na = 0, nr = 0, st = ST_OK, found = 0;
+    double th = 0.0, yh_out[6], ylast[6];
+    while ((t - tmax) * 1.0 < 0.0) {
+        if (na + nr >= sp.max_steps) {
+            st = ST_MAXSTEPS;
+            break;
+        }
+        h = clamp_step(h, sp.max_step, sp.min_step);
+        h = adjust_step_to_endpoint(t, h, tmax);
+        double yh[6];
+        const double err = dop853_attempt(p, y, k, h, sp.rtol, sp.atol, yh);
+        if (err != err) {
+            st = ST_NONFINITE;
+            break;
+        }
+        if (err <= 1.0) {
+            const double t_new = t + h;
+            const double gn = pick6(yh, ev.index) - ev.offset;
+            na += 1;
+            if (event_crossed(gp, gn, ev.direction)) {
+                double Fc[DOP_P][6];
+                dop853_dense(p, y, yh, k, h, Fc);
+                const double x = refine_bisect(pick6(y, ev.index), Fc, ev.index, ev.offset,
+                                               ev.direction, h, ev.xtol, ev.gtol);
+                th = t + x * h;
+#pragma unroll
+                for (int d = 0; d < 6; ++d) {
+                    yh_out[d] = dense_eval1(y[d], Fc, d, x);
+                    ylast[d] = yh[d];
+                }
+                found = 1;
+                break;
+            }
+            t = t_new;
+#pragma unroll
+            for (int d = 0; d < 6; ++d) {
+                y[d] = yh[d];
+                k[0][d] = k[DOP_S][d];
+            }
+            gp = gn;
+            h = h * pi_accept_factor(err, err_prev);
+            err_prev = err;
+        } else {
+            h = clamp_step(h * pi_reject_factor(err), sp.max_step, sp.min_step);
+            nr += 1;
+        }
+    }
+    if (!found) {
+        th = t;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            yh_out[d] = y[d];
+            ylast[d] = y[d];
+        }
+    }
+    hit[i] = found;
+    t_hit[i] = th;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        y_hit[(size_t)d * n + i] = yh_out[d];
+        if (y_last) y_last[(size_t)d * n + i] = ylast[d];
+    }
+    if (status) status[i] = st;
+    if (nacc) nacc[i] = na;
+    if (nrej) nrej[i] = nr;
+}
+
+/* ---- 42-d, one trajectory per 8-lane group ------------------------------ */
+
+struct Final42Args {
+    int n;
+    const double* mu;
+    const double* t0;
+    const double* tf;
+    StepParams sp;
+    int fwd;
+    unsigned long long flip;
+    const double* y0;
+    double* yout;
+    int* status;
+    int* nacc;
+    int* nrej;
+    int* counter;
+};
+
+__device__ __forceinline__ void load42(const Final42Args& a, int traj, int b, Lane42& p,
+                                       Traj<Lane42>& s) {
+    p.mu = a.mu[traj];
+    p.mu2 = 1.0 - p.mu;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const int c = comp42(b < 7 ? b : 1, d);
+        p.sgn[d] = flip_sign(a.fwd, a.flip, c);
+        s.y[d] = (b < 7) ? a.y0[(size_t)c * a.n + traj] : 0.0;
+    }
+    traj_init(p, s, a.t0[traj], a.tf[traj], a.sp);
+}
+
+__device__ __forceinline__ void store42(const Final42Args& a, int traj, int b,
+                                        const Traj<Lane42>& s, const double (&yo)[6]) {
+    if (b < 7) {
+#pragma unroll
+        for (int d = 0; d < 6; ++d) a.yout[(size_t)comp42(b, d) * a.n + traj] = yo[d];
+    }
+    if (b == 0) {
+        if (a.status) a.status[traj] = s.status;
+        if (a.nacc) a.nacc[traj] = s.nacc;
+        if (a.nrej) a.nrej[traj] = s.nrej;
+    }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_dop853_final42_plain(Final42Args a) {
+    const int gtid = blockIdx.x * BLOCK + threadIdx.x;
+    const int traj = gtid >> 3;
+    const int b = gtid & 7;
+    if (traj >= a.n) return;
+    Lane42 p;
+    p.blk = b;
+    Traj<Lane42> s;
+    load42(a, traj, b, p, s);
+    double yo[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) yo[d] = s.y[d];
+    while (!s.done) traj_iterate(p, s, a.sp, yo);
+    store42(a, traj, b, s, yo);
+}
+
+/* Work-stealing form.  Each wave keeps its 8 groups busy: after every step
+ * attempt, groups whose trajectory ended take the next indices from a global
+ * counter (one atomic per wave, ranks by ballot popcount).  A wave exits when
+ * the counter has passed n and all of its groups are idle, so every wave
+ * terminates once every trajectory has (each trajectory is bounded by
+ * max_steps).                                                               */
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_dop853_final42_persist(Final42Args a) {
+    const int lane = threadIdx.x & 63;
+    const int b = lane & 7;
+    const int grp = lane >> 3;
+    Lane42 p;
+    p.blk = b;
+    Traj<Lane42> s;
+    double yo[6];
+    int traj = -1;
+    bool exhausted = false;
+    s.done = true;
+    for (;;) {
+        const bool need = (traj < 0) && !exhausted;
+        const unsigned long long m = __ballot(need && b == 0);
+        if (m) {
+            const int cnt = __popcll(m);
+            const int leader = __ffsll((long long)m) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(a.counter, cnt);
+            base = __shfl(base, leader);
+            if (need) {
+                const unsigned long long below = m & ((1ULL << (grp * 8)) - 1ULL);
+                const int idx = base + __popcll(below);
+                if (idx < a.n) {
+                    traj = idx;
+                    load42(a, traj, b, p, s);
+#pragma unroll
+                    for (int d = 0; d < 6; ++d) yo[d] = s.y[d];
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (traj >= 0 && s.done) {
+            /* zero-span trajectories finish at load time */
+            store42(a, traj, b, s, yo);
+            traj = -1;
+            continue;
+        }
+        if (!__any(traj >= 0)) {
+            if (__all(exhausted || traj >= 0)) break;
+            continue;
+        }
+        if (traj >= 0) {
+            traj_iterate(p, s, a.sp, yo);
+            if (s.done) {
+                store42(a, traj, b, s, yo);
+                traj = -1;
+            }
+        }
+    }
+}
+
+__global__ void k_fill_int(int* p, int v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+/* ---- unit probes: RHS and pow on given inputs --------------------------- */
+__global__ void k_eval_rhs(int n, int dim, const double* __restrict__ mu,
+                           const double* __restrict__ y, double* __restrict__ out) {
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (dim == 6) {
+        const int i = gtid;
+        if (i >= n) return;
+        Lane6 p;
+        p.mu = mu[i];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) p.sgn[d] = 1.0;
+        double s[6], o[6];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) s[d] = y[(size_t)d * n + i];
+        p.rhs(s, o);
+#pragma unroll
+        for (int d = 0; d < 6; ++d) out[(size_t)d * n + i] = o[d];
+    } else {
+        const int i = gtid >> 3;
+        const int b = gtid & 7;
+        if (i >= n) return;
+        Lane42 p;
+        p.blk = b;
+        p.mu = mu[i];
+        p.mu2 = 1.0 - p.mu;
+        double s[6], o[6];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            p.sgn[d] = 1.0;
+            s[d] = (b < 7) ? y[(size_t)comp42(b, d) * n + i] : 0.0;
+        }
+        p.rhs(s, o);
+        if (b < 7) {
+#pragma unroll
+            for (int d = 0; d < 6; ++d) out[(size_t)comp42(b, d) * n + i] = o[d];
+        }
+    }
+}
+
+__global__ void k_eval_math(int n, const double* __restrict__ x, const double* __restrict__ y,
+                            double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = x[i], b = y[i];
+    out[(size_t)0 * n + i] = __builtin_sqrt(a);
+    out[(size_t)1 * n + i] = a / b;
+    out[(size_t)2 * n + i] = hh_pow15(a);
+    out[(size_t)3 * n + i] = hh_pow25(a);
+    out[(size_t)4 * n + i] = hh_pow(a, b);
+}
+
+int check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, what);
+    return HH_OK;
+}
+
+int blocks_for(long long threads, int block) { return (int)((threads + block - 1) / block); }
+
+StepParams make_sp(double rtol, double atol, double max_step, double min_step,
+                   long long max_steps) {
+    StepParams sp;
+    sp.rtol = rtol;
+    sp.atol = atol;
+    sp.max_step = max_step;
+    sp.min_step = min_step;
+    sp.max_steps = max_steps;
+    return sp;
+}
+
+}  // namespace
+
+/* ======================================================================== */
+/* C-ABI                                                                     */
+/* ======================================================================== */
+extern "C" {
+
+const char* hh_last_error(void) { return g_err; }
+
+int hh_abi_version(void) { return HH_ABI_VERSION; }
+
+int hh_rk8_fixed(int n, const double* mu, const double* t0, const double* tf, int steps, int fwd,
+                 unsigned long long flip_mask, const double* y0, double* y_out, double* traj_out,
+                 int block, void* stream) {
+    if (n < 0) return fail(HH_ERR_ARGS, "hh_rk8_fixed: n < 0");
+    if (n == 0) return HH_OK;
+    if (steps < 2) return fail(HH_ERR_ARGS, "hh_rk8_fixed: steps must be >= 2");
+    if (!mu || !t0 || !tf || !y0 || !y_out) return fail(HH_ERR_ARGS, "hh_rk8_fixed: null pointer");
+    if (block <= 0) block = 128;
+    if (block > 256 || block % 64) return fail(HH_ERR_ARGS, "hh_rk8_fixed: block must be 64..256, multiple of 64");
+    hipLaunchKernelGGL(k_rk8_fixed6, dim3(blocks_for(n, block)), dim3(block), 0,
+                       (hipStream_t)stream, n, mu, t0, tf, steps, fwd, flip_mask, y0, y_out,
+                       traj_out);
+    return check_launch("hh_rk8_fixed launch");
+}
+
+int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const double* tf,
+                    double rtol, double atol, double max_step, double min_step,
+                    long long max_steps, int fwd, unsigned long long flip_mask, const double* y0,
+                    double* y_out, int* status, int* n_acc, int* n_rej, int variant, int block,
+                    int grid, int* work_counter, void* stream) {
+    if (n < 0) return fail(HH_ERR_ARGS, "hh_dop853_final: n < 0");
+    if (n == 0) return HH_OK;
+    if (dim != 6 && dim != 42) return fail(HH_ERR_ARGS, "hh_dop853_final: dim must be 6 or 42");
+    if (!mu || !t0 || !tf || !y0 || !y_out) return fail(HH_ERR_ARGS, "hh_dop853_final: null pointer");
+    if (!(rtol >= 0.0) || !(atol > 0.0) || !(min_step > 0.0) || !(max_step >= min_step) || max_steps < 1)
+        return fail(HH_ERR_ARGS, "hh_dop853_final: invalid tolerances/step bounds");
+    StepParams sp = make_sp(rtol, atol, max_step, min_step, max_steps);
+    hipStream_t st = (hipStream_t)stream;
+    if (dim == 6) {
+        if (variant != HH_VARIANT_PLAIN) return fail(HH_ERR_ARGS, "hh_dop853_final: dim 6 supports the plain variant only");
+        if (block <= 0) block = 128;
+        if (block > 256 || block % 64) return fail(HH_ERR_ARGS, "hh_dop853_final: block must be 64..256, multiple of 64");
+        hipLaunchKernelGGL(k_dop853_final6, dim3(blocks_for(n, block)), dim3(block), 0, st, n, mu,
+                           t0, tf, sp, fwd, flip_mask, y0, y_out, status, n_acc, n_rej);
+        return check_launch("hh_dop853_final(6) launch");
+    }
+    Final42Args a;
+    a.n = n; a.mu = mu; a.t0 = t0; a.tf = tf; a.sp = sp; a.fwd = fwd; a.flip = flip_mask;
+    a.y0 = y0; a.yout = y_out; a.status = status; a.nacc = n_acc; a.nrej = n_rej;
+    a.counter = work_counter;
+    if (block <= 0) block = 256;
+    if (block != 64 && block != 128 && block != 256 && block != 512)
+        return fail(HH_ERR_ARGS, "hh_dop853_final: block must be 64, 128, 256 or 512");
+    if (variant == HH_VARIANT_PLAIN) {
+        const int g = blocks_for((long long)n * 8, block);
+        switch (block) {
+            case 64: hipLaunchKernelGGL(k_dop853_final42_plain<64>, dim3(g), dim3(64), 0, st, a); break;
+            case 128: hipLaunchKernelGGL(k_dop853_final42_plain<128>, dim3(g), dim3(128), 0, st, a); break;
+            case 256: hipLaunchKernelGGL(k_dop853_final42_plain<256>, dim3(g), dim3(256), 0, st, a); break;
+            default: hipLaunchKernelGGL(k_dop853_final42_plain<512>, dim3(g), dim3(512), 0, st, a); break;
+        }
+        return check_launch("hh_dop853_final(42, plain) launch");
+    }
+    if (variant != HH_VARIANT_PERSISTENT) return fail(HH_ERR_ARGS, "hh_dop853_final: unknown variant");
+    if (!work_counter) return fail(HH_ERR_ARGS, "hh_dop853_final: persistent variant needs work_counter");
+    if (grid <= 0) {
+        int dev = 0, cus = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return hip_fail(e, "hipDeviceGetAttribute");
+        grid = cus * (1024 / block);
+    }
+    const long long max_grid = ((long long)n * 8 + block - 1) / block;
+    if (grid > max_grid) grid = (int)max_grid;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, work_counter, 0, 1);
+    switch (block) {
+        case 64: hipLaunchKernelGGL(k_dop853_final42_persist<64>, dim3(grid), dim3(64), 0, st, a); break;
+        case 128: hipLaunchKernelGGL(k_dop853_final42_persist<128>, dim3(grid), dim3(128), 0, st, a); break;
+        case 256: hipLaunchKernelGGL(k_dop853_final42_persist<256>, dim3(grid), dim3(256), 0, st, a); break;
+        default: hipLaunchKernelGGL(k_dop853_final42_persist<512>, dim3(grid), dim3(512), 0, st, a); break;
+    }
+    return check_launch("hh_dop853_final(42, persistent) launch");
+}
+
+int hh_dop853_event(int n, const double* mu, const double* t0, const double* tmax, double rtol,
+                    double atol, double max_step, double min_step, long long max_steps,
+                    int ev_index, double ev_offset, int direction, double xtol, double gtol,
+                    int fwd, unsigned long long flip_mask, const double* y0, int* hit,
+                    double* t_hit, double* y_hit, double* y_last, int* status, int* n_acc,
+                    int* n_rej, int block, void* stream) {
+    if (n < 0) return fail(HH_ERR_ARGS, "hh_dop853_event: n < 0");
+    if (n == 0) return HH_OK;
+    if (!mu || !t0 || !tmax || !y0 || !hit || !t_hit || !y_hit)
+        return fail(HH_ERR_ARGS, "hh_dop853_event: null pointer");
+    if (ev_index < 0 || ev_index > 5) return fail(HH_ERR_ARGS, "hh_dop853_event: ev_index must be 0..5");
+    if (direction < -1 || direction > 1) return fail(HH_ERR_ARGS, "hh_dop853_event: direction must be -1, 0 or 1");
+    if (!(rtol >= 0.0) || !(atol > 0.0) || !(min_step > 0.0) || !(max_step >= min_step) || max_steps < 1)
+        return fail(HH_ERR_ARGS, "hh_dop853_event: invalid tolerances/step bounds");
+    if (block <= 0) block = 128;
+    if (block > 256 || block % 64) return fail(HH_ERR_ARGS, "hh_dop853_event: block must be 64..256, multiple of 64");
+    StepParams sp = make_sp(rtol, atol, max_step, min_step, max_steps);
+    EventParams ev;
+    ev.index = ev_index; ev.offset = ev_offset; ev.direction = direction; ev.xtol = xtol; ev.gtol = gtol;
+    hipLaunchKernelGGL(k_dop853_event6, dim3(blocks_for(n, block)), dim3(block), 0,
+                       (hipStream_t)stream, n, mu, t0, tmax, sp, ev, fwd, flip_mask, y0, hit,
+                       t_hit, y_hit, y_last, status, n_acc, n_rej);
+    return check_launch("hh_dop853_event launch");
+}
+
+int hh_eval_rhs(int n, int dim, const double* mu, const double* y, double* out, void* stream) {
+    if (n < 0 || (dim != 6 && dim != 42)) return fail(HH_ERR_ARGS, "hh_eval_rhs: bad n/dim");
+    if (n == 0) return HH_OK;
+    if (!mu || !y || !out) return fail(HH_ERR_ARGS, "hh_eval_rhs: null pointer");
+    const long long threads = dim == 6 ? n : (long long)n * 8;
+    hipLaunchKernelGGL(k_eval_rhs, dim3(blocks_for(threads, 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, dim, mu, y, out);
+    return check_launch("hh_eval_rhs launch");
+}
+
+int hh_eval_math(int n, const double* x, const double* y, double* out, void* stream) {
+    if (n < 0) return fail(HH_ERR_ARGS, "hh_eval_math: n < 0");
+    if (n == 0) return HH_OK;
+    if (!x || !y || !out) return fail(HH_ERR_ARGS, "hh_eval_math: null pointer");
+    hipLaunchKernelGGL(k_eval_math, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                       n, x, y, out);
+    return check_launch("hh_eval_math launch");
+}
+
+}  /* extern "C" */

@@ -1,0 +1,114 @@
+/* hiten_hip.h -- C-ABI of the MI355X (gfx950) CR3BP propagation library.
+ *
+ * Every entry point takes plain device pointers (FP64 unless stated) that the
+ * caller owns, a trajectory count, scalars, and a hipStream_t passed as
+ * void*.  Calls are asynchronous on that stream; the library never frees or
+ * retains caller memory.  State arrays are structure-of-arrays: component c
+ * of trajectory i is element [c * n + i], with components in the reference's
+ * order ([x, y, z, vx, vy, vz] for 6-d; [Phi row-major (36) | x (6)] for
+ * 42-d).  Return value: HH_OK or an HH_ERR_* code, with a message from
+ * hh_last_error() (thread-local).
+ *
+ * Direction handling restates hiten's _DirectedSystem
+ * (src/hiten/algorithms/dynamics/base.py:186-310): with fwd == -1 the
+ * derivative of every component whose bit is set in flip_mask is negated.
+ *
+ * Step control restates integrators/utils.py:126-296 and rk.py:2443-2484
+ * (initial step, PI controller, endpoint clamp); max_steps bounds the
+ * attempts per trajectory (the reference has no bound, rk.py:2452) and is
+ * reported through status = HH_STATUS_MAXSTEPS.
+ */
+#ifndef HITEN_HIP_H
+#define HITEN_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HH_ABI_VERSION 1
+
+#define HH_OK 0
+#define HH_ERR_ARGS 1
+#define HH_ERR_HIP 2
+
+#define HH_STATUS_OK 0
+#define HH_STATUS_MAXSTEPS 1
+#define HH_STATUS_NONFINITE 2
+
+#define HH_VARIANT_PLAIN 0
+#define HH_VARIANT_PERSISTENT 1
+
+/* Message describing the last error returned on this thread. */
+const char* hh_last_error(void);
+
+/* HH_ABI_VERSION of the built library. */
+int hh_abi_version(void);
+
+/* Fixed-step 13-stage RK8 on the 6-d CR3BP field over the grid
+ * numpy.linspace(t0[i], tf[i], steps).
+ * Replaces: _FixedStepRK._integrate_fixed_rk (src/hiten/algorithms/
+ * integrators/rk.py:531-588) reached through _propagate_dynsys(...,
+ * method="fixed", order=8) (dynamics/base.py:430-434).
+ *   mu, t0, tf : [n]          y0, y_out : [6][n]
+ *   traj_out   : [steps][6][n] every grid state, or NULL
+ *   block      : threads per block (64..256, 0 = default)              */
+int hh_rk8_fixed(int n, const double* mu, const double* t0, const double* tf, int steps,
+                 int fwd, unsigned long long flip_mask, const double* y0, double* y_out,
+                 double* traj_out, int block, void* stream);
+
+/* Adaptive DOP853 from t0[i] to tf[i]; y_out is the dense-output value at
+ * tf[i], i.e. the last row of _propagate_dynsys(..., steps=2,
+ * method="adaptive") and the final Phi of _compute_stm.
+ * Replaces: _DOP853._integrate_dop853 (integrators/rk.py:2375-2549) with
+ * the step kernel dop853_step_jit_kernel (rk.py:1636-1708) on
+ * _crtbp_accel (dim 6, dynamics/rtbp.py:30-74) or _var_equations (dim 42,
+ * dynamics/rtbp.py:167-255).
+ *   dim        : 6 or 42      mu, t0, tf : [n]   y0, y_out : [dim][n]
+ *   status, n_acc, n_rej : [n] int, each may be NULL
+ *   variant    : HH_VARIANT_PLAIN (one launch, one trajectory per lane or
+ *                8-lane group) or HH_VARIANT_PERSISTENT (dim 42: work-
+ *                stealing groups refilled from *work_counter)
+ *   block      : threads per block (dim 6: 64..256; dim 42: 64/128/256/512)
+ *   grid       : persistent blocks (0 = 4 waves per CU)
+ *   work_counter : one device int, required for the persistent variant;
+ *                reset by the library on the stream before the launch     */
+int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const double* tf,
+                    double rtol, double atol, double max_step, double min_step,
+                    long long max_steps, int fwd, unsigned long long flip_mask,
+                    const double* y0, double* y_out, int* status, int* n_acc, int* n_rej,
+                    int variant, int block, int grid, int* work_counter, void* stream);
+
+/* Adaptive DOP853 (6-d) from t0[i] until the first crossing of the plane
+ * y[ev_index] = ev_offset in `direction` (-1, 0, +1), refined by bisection
+ * on the dense interpolant, or until tmax[i].
+ * Replaces: _DOP853._integrate_dop853_until_event (integrators/rk.py:
+ * 2678-2803) and _dop853_refine_in_step (rk.py:2005-2102), the engine of
+ * _SingleHitBackend._cross_event_driven (poincare/singlehit/backend.py:
+ * 164-235) with the event functions _g_x0/_g_y0/_g_z0 (backend.py:30-42).
+ *   hit : [n] int            t_hit : [n]
+ *   y_hit : [6][n] state at the hit (or at the end when no hit)
+ *   y_last : [6][n] end state of the crossing step, or NULL              */
+int hh_dop853_event(int n, const double* mu, const double* t0, const double* tmax, double rtol,
+                    double atol, double max_step, double min_step, long long max_steps,
+                    int ev_index, double ev_offset, int direction, double xtol, double gtol,
+                    int fwd, unsigned long long flip_mask, const double* y0, int* hit,
+                    double* t_hit, double* y_hit, double* y_last, int* status, int* n_acc,
+                    int* n_rej, int block, void* stream);
+
+/* Evaluate the CR3BP right-hand side at n states (dim 6: _crtbp_accel,
+ * dynamics/rtbp.py:30-74; dim 42: _var_equations, rtbp.py:167-255).
+ *   y, out : [dim][n]                                                     */
+int hh_eval_rhs(int n, int dim, const double* mu, const double* y, double* out, void* stream);
+
+/* Arithmetic probe used by the parity tests: out[0][i] = sqrt(x[i]),
+ * out[1][i] = x[i] / y[i], out[2][i] = x[i]^1.5, out[3][i] = x[i]^2.5,
+ * out[4][i] = x[i]^y[i] as the kernels compute them.  out : [5][n].
+ * No counterpart in the reference (it checks the arithmetic the reference
+ * obtains from libm: np.sqrt, '/', and '**' in rtbp.py / utils.py).       */
+int hh_eval_math(int n, const double* x, const double* y, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HITEN_HIP_H */

@@ -1,0 +1,395 @@
+"""ctypes wrapper of the CPU oracle (test infrastructure only).
+
+The oracle restates hiten's CR3BP propagation path in C (``hh_oracle.c``).
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg
+may import this module, and only to check or time the reference algorithm --
+never to produce a product result.
+
+Host logic that the reference writes in plain Python/NumPy is restated here
+in Python on top of the C integrators:
+
+* ``propagate``      dynamics/base.py:346-458 (_propagate_dynsys)
+* ``compute_stm``    dynamics/rtbp.py:258-340 (_compute_stm)
+* ``plane_crossing`` poincare/singlehit/backend.py:164-280
+                     (_SingleHitBackend._cross / _cross_event_driven)
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+KIND_CR3BP6, KIND_CR3BP42, KIND_LINEAR = 0, 1, 2
+EPS = float(np.finfo(float).eps)
+MIN_STEP = 10.0 * EPS
+ALL_FLIP = (1 << 64) - 1
+
+
+class OrSys(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("dim", C.c_int),
+        ("mu", C.c_double),
+        ("fwd", C.c_int),
+        ("flip", C.c_uint64),
+        ("libm_pow", C.c_int),
+        ("lin_a", C.POINTER(C.c_double)),
+        ("lin_b", C.POINTER(C.c_double)),
+        ("nfev", C.c_longlong),
+    ]
+
+
+class OrStats(C.Structure):
+    _fields_ = [("n_accepted", C.c_longlong), ("n_rejected", C.c_longlong), ("status", C.c_int)]
+
+
+def build() -> None:
+    """Compile liboracle.so with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+_L = C.POINTER(C.c_longlong)
+
+
+def _declare(L):
+    L.or_rhs.argtypes = [C.POINTER(OrSys), C.c_double, _D, _D]
+    L.or_crtbp_accel.argtypes = [_D, C.c_double, _D]
+    L.or_jacobian_crtbp.argtypes = [C.c_double] * 4 + [C.c_int, _D]
+    L.or_var_equations.argtypes = [_D, C.c_double, C.c_int, _D]
+    L.or_sumsq.argtypes = [_D, C.c_int]
+    L.or_sumsq.restype = C.c_double
+    for f in (L.or_pow15, L.or_pow25):
+        f.argtypes = [C.c_double]
+        f.restype = C.c_double
+    L.or_pow.argtypes = [C.c_double, C.c_double]
+    L.or_pow.restype = C.c_double
+    L.or_linspace.argtypes = [C.c_double, C.c_double, C.c_int, _D]
+    L.or_rk_fixed.argtypes = [C.POINTER(OrSys), _D, _D, C.c_int, _D, _D]
+    L.or_rk_fixed_event.argtypes = [C.POINTER(OrSys), _D, _D, C.c_int, C.c_int, C.c_double,
+                                    C.c_int, C.c_double, C.c_double, _I, _D, _D, _D]
+    L.or_dop853.argtypes = [C.POINTER(OrSys), _D, _D, C.c_int, C.c_double, C.c_double,
+                            C.c_double, C.c_double, C.c_longlong, _D, _D, C.POINTER(OrStats)]
+    L.or_dop853_event.argtypes = [C.POINTER(OrSys), _D, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, C.c_double, C.c_double, C.c_longlong, C.c_int,
+                                  C.c_double, C.c_int, C.c_double, C.c_double, _I, _D, _D, _D,
+                                  C.POINTER(OrStats)]
+    L.or_batch_rk8_final.argtypes = [C.c_int, _D, _D, C.c_double, C.c_double, C.c_int, _D, C.c_int]
+    L.or_batch_dop853_final.argtypes = [C.c_int, C.c_int, _D, _D, _D, _D, C.c_double, C.c_double,
+                                        C.c_double, C.c_longlong, _D, _L, _L, _I, C.c_int]
+    L.or_batch_dop853_event.argtypes = [C.c_int, _D, _D, _D, C.c_double, C.c_double, C.c_double,
+                                        C.c_longlong, C.c_int, C.c_double, C.c_int, C.c_double,
+                                        C.c_double, _I, _D, _D, _L, _L, C.c_int]
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class System:
+    """A right-hand side for the oracle integrators.
+
+    ``kind``: "cr3bp6" (rtbp.py:30-74), "cr3bp42" (rtbp.py:167-255) or
+    "linear" (dy = A y + b, for the closed-form known-answer tests).
+    ``fwd``/``flip`` restate _DirectedSystem (dynamics/base.py:186-310):
+    ``flip=None`` flips every component when ``fwd == -1``.
+    """
+
+    def __init__(self, kind="cr3bp6", mu=0.0, dim=None, fwd=1, flip=None, libm_pow=False,
+                 A=None, b=None):
+        self.kind = {"cr3bp6": KIND_CR3BP6, "cr3bp42": KIND_CR3BP42, "linear": KIND_LINEAR}[kind]
+        if dim is None:
+            dim = {KIND_CR3BP6: 6, KIND_CR3BP42: 42}.get(self.kind)
+        self.dim = int(dim)
+        self._A = _f64(A) if A is not None else None
+        self._b = _f64(b) if b is not None else None
+        mask = ALL_FLIP
+        if flip is not None:
+            idx = range(self.dim)[flip] if isinstance(flip, slice) else flip
+            mask = 0
+            for i in idx:
+                mask |= 1 << int(i)
+        self.s = OrSys(self.kind, self.dim, float(mu), 1 if fwd >= 0 else -1, mask,
+                       1 if libm_pow else 0,
+                       _p(self._A) if self._A is not None else None,
+                       _p(self._b) if self._b is not None else None, 0)
+
+    def rhs(self, y, t=0.0):
+        y = _f64(y)
+        out = np.empty(self.dim)
+        lib().or_rhs(C.byref(self.s), float(t), _p(y), _p(out))
+        return out
+
+
+def crtbp_accel(state, mu):
+    s = _f64(state)
+    out = np.empty(6)
+    lib().or_crtbp_accel(_p(s), float(mu), _p(out))
+    return out
+
+
+def jacobian_crtbp(x, y, z, mu, libm_pow=False):
+    F = np.empty(36)
+    lib().or_jacobian_crtbp(float(x), float(y), float(z), float(mu), int(libm_pow), _p(F))
+    return F.reshape(6, 6)
+
+
+def var_equations(phi_vec, mu, libm_pow=False):
+    p = _f64(phi_vec)
+    out = np.empty(42)
+    lib().or_var_equations(_p(p), float(mu), int(libm_pow), _p(out))
+    return out
+
+
+def sumsq(v):
+    v = _f64(v)
+    return lib().or_sumsq(_p(v), v.size)
+
+
+def pow15(x):
+    return lib().or_pow15(float(x))
+
+
+def pow25(x):
+    return lib().or_pow25(float(x))
+
+
+def powg(x, y):
+    return lib().or_pow(float(x), float(y))
+
+
+def linspace(t0, tf, n):
+    out = np.empty(n)
+    lib().or_linspace(float(t0), float(tf), int(n), _p(out))
+    return out
+
+
+def rk_fixed(system: System, y0, t_vals, derivs=False):
+    t_vals = _f64(t_vals)
+    y0 = _f64(y0)
+    st = np.empty((t_vals.size, system.dim))
+    dv = np.empty_like(st) if derivs else None
+    lib().or_rk_fixed(C.byref(system.s), _p(y0), _p(t_vals), t_vals.size, _p(st),
+                      _p(dv) if derivs else None)
+    return (st, dv) if derivs else st
+
+
+def rk_fixed_event(system, y0, t_vals, ev_index, ev_offset=0.0, direction=0, xtol=1e-12,
+                   gtol=1e-12):
+    t_vals = _f64(t_vals)
+    y0 = _f64(y0)
+    hit = C.c_int(0)
+    th = C.c_double(0)
+    yh = np.empty(system.dim)
+    yl = np.empty(system.dim)
+    lib().or_rk_fixed_event(C.byref(system.s), _p(y0), _p(t_vals), t_vals.size, ev_index,
+                            ev_offset, direction, xtol, gtol, C.byref(hit), C.byref(th), _p(yh),
+                            _p(yl))
+    return bool(hit.value), th.value, yh, yl
+
+
+def dop853(system: System, y0, t_eval, rtol=1e-13, atol=1e-13, max_step=np.inf,
+           min_step=MIN_STEP, max_steps=10_000_000, derivs=False):
+    """_DOP853._integrate_dop853 (rk.py:2375-2549): states (and derivatives) at t_eval."""
+    t_eval = _f64(t_eval)
+    y0 = _f64(y0)
+    out = np.empty((t_eval.size, system.dim))
+    dv = np.empty_like(out) if derivs else None
+    st = OrStats()
+    lib().or_dop853(C.byref(system.s), _p(y0), _p(t_eval), t_eval.size, rtol, atol, max_step,
+                    min_step, max_steps, _p(out), _p(dv) if derivs else None, C.byref(st))
+    info = dict(n_accepted=st.n_accepted, n_rejected=st.n_rejected, status=st.status)
+    return (out, dv, info) if derivs else (out, info)
+
+
+def dop853_event(system, y0, t0, tmax, ev_index, ev_offset=0.0, direction=0, rtol=1e-13,
+                 atol=1e-13, max_step=np.inf, min_step=MIN_STEP, xtol=1e-12, gtol=1e-12,
+                 max_steps=10_000_000):
+    """_DOP853._integrate_dop853_until_event (rk.py:2678-2803)."""
+    y0 = _f64(y0)
+    hit = C.c_int(0)
+    th = C.c_double(0)
+    yh = np.empty(system.dim)
+    yl = np.empty(system.dim)
+    st = OrStats()
+    lib().or_dop853_event(C.byref(system.s), _p(y0), float(t0), float(tmax), rtol, atol, max_step,
+                          min_step, max_steps, ev_index, ev_offset, direction, xtol, gtol,
+                          C.byref(hit), C.byref(th), _p(yh), _p(yl), C.byref(st))
+    return bool(hit.value), th.value, yh, yl, dict(n_accepted=st.n_accepted,
+                                                   n_rejected=st.n_rejected, status=st.status)
+
+
+# ---------------------------------------------------------------------------
+# host-level restatements
+# ---------------------------------------------------------------------------
+
+def propagate(mu, state0, t0, tf, forward=1, steps=1000, method="adaptive", order=8,
+              flip_indices=None, rtol=1e-12, atol=1e-12, max_step=1e4, kind="cr3bp6",
+              libm_pow=False):
+    """dynamics/base.py:346-458 (_propagate_dynsys) for the CR3BP systems.
+
+    Returns (times, states) with times signed by ``forward`` (base.py:456).
+    """
+    dim = 6 if kind == "cr3bp6" else 42
+    y0 = _f64(state0)
+    if y0.shape != (dim,):
+        raise ValueError(f"Initial state vector must have {dim} elements, but got shape {y0.shape}")
+    t_eval = np.linspace(t0, tf, steps)
+    if steps >= 2 and np.isclose(t_eval[0], t_eval[-1]):
+        return forward * t_eval, np.repeat(y0[None, :], len(t_eval), axis=0)
+    sysm = System(kind, mu=mu, fwd=forward, flip=flip_indices, libm_pow=libm_pow)
+    if method == "fixed":
+        if order != 8:
+            raise ValueError("only RK8 is restated")
+        states = rk_fixed(sysm, y0, t_eval)
+    elif method == "adaptive":
+        states, _ = dop853(sysm, y0, t_eval, rtol=rtol, atol=atol, max_step=max_step)
+    else:
+        raise ValueError(method)
+    return forward * t_eval, states
+
+
+def compute_stm(mu, x0, tf, steps=2000, forward=1, method="adaptive", order=8, **kw):
+    """dynamics/rtbp.py:258-340 (_compute_stm): returns (x, times, phi_T, PHI)."""
+    PHI0 = np.zeros(42)
+    PHI0[:36] = np.eye(6).ravel()
+    PHI0[36:] = x0
+    times, PHI = propagate(mu, PHI0, 0.0, tf, forward=forward, steps=steps, method=method,
+                           order=order, flip_indices=slice(36, 42), kind="cr3bp42", **kw)
+    return PHI[:, 36:42], times, PHI[-1, :36].reshape(6, 6), PHI
+
+
+def _cross_event_driven(mu, state0, t0, tmax, forward, index=1, offset=0.0, direction=0):
+    """poincare/singlehit/backend.py:164-235."""
+    t_start = float(t0)
+    if t_start <= 0.0:
+        t_start = 1e-12
+    _, st = propagate(mu, state0, 0.0, t_start, forward=forward, steps=2, method="adaptive",
+                      order=8)
+    y_start = st[-1].copy()
+    span = float(max(0.0, tmax - t_start))
+    sysm = System("cr3bp6", mu=mu)
+    times = np.array([0.0, span])
+    if np.isclose(times[0], times[-1]):
+        t_rel, y_hit = span, y_start
+    else:
+        hit, t_ev, y_ev, y_last, _ = dop853_event(sysm, y_start, 0.0, span, index, offset,
+                                                 direction, rtol=1e-12, atol=1e-12,
+                                                 max_step=np.inf)
+        t_rel, y_hit = (t_ev, y_ev) if hit else (span, y_last)
+    if 0.0 <= t_rel < span:
+        return t_start + t_rel, y_hit
+    return None
+
+
+def plane_crossing(mu, x0, forward=1, t_guess=None, t_window=None, index=1, offset=0.0,
+                   direction=0, t0_offset=0.15):
+    """poincare/singlehit/backend.py:237-280 (_cross) as used by _y_plane_crossing.
+
+    Returns (t_hit, state) or (None, None).
+    """
+    x0 = _f64(x0)
+    if t_guess is not None and t_window is not None and t_window > 0.0:
+        t0 = float(max(t_guess - 0.5 * t_window, 0.0))
+        tmax = float(t0 + t_window)
+    else:
+        t_start = float(t_guess) if t_guess is not None else float(np.pi / 2.0 - t0_offset)
+        if t_start < 0.0:
+            t_start = 0.0
+        half = float(np.pi) * 0.5
+        t0 = float(max(t_start - half, 0.0))
+        tmax = float(t0 + 2.0 * half)
+    hit = _cross_event_driven(mu, x0, t0, tmax, forward, index, offset, direction)
+    if hit is None and t_guess is not None:
+        span = float(t_window if (t_window is not None and t_window > 0.0) else 0.1)
+        while span < float(np.pi):
+            span *= 2.0
+            t0_try = float(max(t_guess - 0.5 * span, 0.0))
+            hit = _cross_event_driven(mu, x0, t0_try, t0_try + span, forward, index, offset,
+                                      direction)
+            if hit is not None:
+                break
+    if hit is None:
+        if t_guess is not None:
+            t_start = float(max(t_guess, 0.0))
+        else:
+            t_start = float(np.pi / 2.0 - t0_offset)
+            if t_start < 0.0:
+                t_start = 0.0
+        hit = _cross_event_driven(mu, x0, t_start, t_start + float(np.pi), forward, index,
+                                  offset, direction)
+    return hit if hit is not None else (None, None)
+
+
+# ---------------------------------------------------------------------------
+# batched (CPU baseline)
+# ---------------------------------------------------------------------------
+
+def batch_rk8_final(mu, y0, t0, tf, steps, n_threads=0):
+    y0 = _f64(y0)
+    n = y0.shape[0]
+    mu = _f64(np.broadcast_to(mu, (n,)))
+    out = np.empty((n, 6))
+    lib().or_batch_rk8_final(n, _p(mu), _p(y0), float(t0), float(tf), int(steps), _p(out),
+                             int(n_threads))
+    return out
+
+
+def batch_dop853_final(mu, y0, t0, tf, rtol=1e-12, atol=1e-12, max_step=1e4,
+                       max_steps=10_000_000, n_threads=0):
+    y0 = _f64(y0)
+    n, dim = y0.shape
+    mu = _f64(np.broadcast_to(mu, (n,)))
+    t0 = _f64(np.broadcast_to(t0, (n,)))
+    tf = _f64(np.broadcast_to(tf, (n,)))
+    out = np.empty((n, dim))
+    na = np.empty(n, dtype=np.int64)
+    nr = np.empty(n, dtype=np.int64)
+    stt = np.empty(n, dtype=np.int32)
+    lib().or_batch_dop853_final(n, dim, _p(mu), _p(y0), _p(t0), _p(tf), rtol, atol, max_step,
+                                max_steps, _p(out), na.ctypes.data_as(_L), nr.ctypes.data_as(_L),
+                                stt.ctypes.data_as(_I), int(n_threads))
+    return out, na, nr, stt
+
+
+def batch_dop853_event(mu, y0, tmax, ev_index=1, ev_offset=0.0, direction=-1, rtol=1e-12,
+                       atol=1e-12, max_step=np.inf, xtol=1e-13, gtol=1e-13,
+                       max_steps=10_000_000, n_threads=0):
+    y0 = _f64(y0)
+    n = y0.shape[0]
+    mu = _f64(np.broadcast_to(mu, (n,)))
+    tmax = _f64(np.broadcast_to(tmax, (n,)))
+    hit = np.empty(n, dtype=np.int32)
+    th = np.empty(n)
+    yh = np.empty((n, 6))
+    na = np.empty(n, dtype=np.int64)
+    nr = np.empty(n, dtype=np.int64)
+    lib().or_batch_dop853_event(n, _p(mu), _p(y0), _p(tmax), rtol, atol, max_step, max_steps,
+                                ev_index, ev_offset, direction, xtol, gtol,
+                                hit.ctypes.data_as(_I), _p(th), _p(yh), na.ctypes.data_as(_L),
+                                nr.ctypes.data_as(_L), int(n_threads))
+    return hit.astype(bool), th, yh, na, nr

@@ -1,0 +1,144 @@
+"""GPU parity tests: HIP kernels (through the C-ABI) against the CPU oracle.
+
+The bar is bit-for-bit equality (``np.array_equal``; +0.0 == -0.0) on the
+same seeded inputs: the kernels evaluate the reference's expressions in the
+same order with IEEE FP64 and no contraction.  The tolerances stated by the
+task (1e-12 relative at fixed step, 1e-10 for adaptive runs) are checked as
+well so that a failure report says how far apart the results are.
+"""
+
+import math
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from hiten_amd import batch, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))) if a.size else 0.0
+
+
+def assert_bitwise(got, want, what, rtol=0.0):
+    got = np.asarray(got)
+    want = np.asarray(want)
+    assert got.shape == want.shape, (what, got.shape, want.shape)
+    if not np.array_equal(got, want):
+        bad = np.argwhere(got != want)
+        msg = f"{what}: {len(bad)} of {got.size} differ, max rel {rel_err(got, want):.3e}, first {bad[:3].tolist()}"
+        if rtol > 0 and rel_err(got, want) <= rtol:
+            pytest.fail("within tolerance but not bitwise -- " + msg)
+        pytest.fail(msg)
+
+
+def test_math_probe_matches_cpu(gpu, oracle):
+    say("hh_eval_math: sqrt, div, pow15, pow25, pow vs host")
+    rng = np.random.default_rng(7)
+    n = 1 << 16
+    x = np.exp(rng.uniform(math.log(1e-8), math.log(1e4), n))
+    y = rng.uniform(-0.5, 0.5, n)
+    y[: n // 4] = -1.0 / 9.0
+    y[n // 4: n // 2] = 0.4 / 9.0
+    out = batch.eval_math(x, y).cpu().numpy()
+    torch.cuda.synchronize()
+    assert_bitwise(out[0], np.sqrt(x), "sqrt")
+    assert_bitwise(out[1], x / y, "div")
+    assert_bitwise(out[2], np.array([oracle.pow15(v) for v in x]), "pow15")
+    assert_bitwise(out[3], np.array([oracle.pow25(v) for v in x]), "pow25")
+    assert_bitwise(out[4], np.array([oracle.powg(a, b) for a, b in zip(x, y)]), "pow")
+
+
+@pytest.mark.parametrize("dim", [6, 42])
+def test_rhs_matches_oracle(gpu, oracle, dim):
+    say(f"hh_eval_rhs dim={dim}")
+    rng = np.random.default_rng(11 + dim)
+    n = 2048
+    st = W.planar_seeds(rng, n)
+    st[:, 1] = rng.uniform(-0.1, 0.1, n)
+    st[:, 3] = rng.uniform(-0.1, 0.1, n)
+    st[:, 5] = rng.uniform(-0.1, 0.1, n)
+    mu = np.asarray(W.MU_SET)[rng.integers(0, 3, n)]
+    if dim == 42:
+        y = np.concatenate([rng.normal(size=(n, 36)), st], axis=1)
+        want = np.stack([oracle.var_equations(y[i], mu[i]) for i in range(n)])
+    else:
+        y = st
+        want = np.stack([oracle.crtbp_accel(y[i], mu[i]) for i in range(n)])
+    got = batch.eval_rhs(mu, y, dim=dim).cpu().numpy()
+    assert_bitwise(got, want, f"rhs{dim}")
+
+
+def test_rk8_fixed_matches_oracle(gpu, oracle):
+    say("hh_rk8_fixed: 128 trajectories x 2000 grid points to 2*pi")
+    w = W.rk8_grid(n=128)
+    got_final, got_traj = batch.rk8_fixed(w["mu"], w["y0"], w["t0"], w["tf"], w["steps"],
+                                          return_trajectory=True)
+    got_final = got_final.cpu().numpy()
+    got_traj = got_traj.cpu().numpy()
+    want = oracle.batch_rk8_final(w["mu"], w["y0"], w["t0"], w["tf"], w["steps"])
+    assert_bitwise(got_final, want, "rk8 final", rtol=1e-12)
+    # full grid of two trajectories against the scalar driver
+    tv = np.linspace(w["t0"], w["tf"], w["steps"])
+    for i in (0, 77):
+        st = oracle.rk_fixed(oracle.System("cr3bp6", mu=w["mu"]), w["y0"][i], tv)
+        assert_bitwise(got_traj[i], st, f"rk8 grid traj {i}")
+
+
+def test_rk8_backward_flip(gpu, oracle):
+    say("hh_rk8_fixed: backward time with all components flipped")
+    w = W.rk8_grid(n=64)
+    got = batch.rk8_fixed(w["mu"], w["y0"], 0.0, 1.5, 300, forward=-1).cpu().numpy()
+    tv = np.linspace(0.0, 1.5, 300)
+    want = np.stack([oracle.rk_fixed(oracle.System("cr3bp6", mu=w["mu"], fwd=-1), w["y0"][i], tv)[-1]
+                     for i in range(64)])
+    assert_bitwise(got, want, "rk8 backward")
+
+
+def test_dop853_final_6d(gpu, oracle):
+    say("hh_dop853_final dim=6: 512 trajectories")
+    w = W.stm_uneven(n=512)
+    y0 = w["y0"][:, 36:]
+    r = batch.dop853_final(w["mu"], y0, w["t0"], w["tf"], dim=6, max_step=1e4)
+    got = r.y.cpu().numpy()
+    want, na, nr, st = oracle.batch_dop853_final(w["mu"], y0, w["t0"], w["tf"], max_step=1e4)
+    assert_bitwise(r.n_accepted.cpu().numpy(), na, "accepted steps")
+    assert_bitwise(r.n_rejected.cpu().numpy(), nr, "rejected steps")
+    assert_bitwise(got, want, "dop853 6-d final", rtol=1e-10)
+
+
+@pytest.mark.parametrize("variant", ["plain", "persistent"])
+def test_dop853_final_42d(gpu, oracle, variant):
+    say(f"hh_dop853_final dim=42 {variant}: 256 trajectories")
+    w = W.stm_uneven(n=256)
+    r = batch.dop853_final(w["mu"], w["y0"], w["t0"], w["tf"], dim=42, max_step=1e4,
+                           variant=variant)
+    got = r.y.cpu().numpy()
+    want, na, nr, st = oracle.batch_dop853_final(w["mu"], w["y0"], w["t0"], w["tf"], max_step=1e4)
+    assert (r.status.cpu().numpy() == 0).all()
+    assert_bitwise(r.n_accepted.cpu().numpy(), na, "accepted steps")
+    assert_bitwise(r.n_rejected.cpu().numpy(), nr, "rejected steps")
+    assert_bitwise(got, want, f"dop853 42-d final ({variant})", rtol=1e-10)
+
+
+def test_dop853_event_fan(gpu, oracle):
+    say("hh_dop853_event: 4 seeds x 64 directions, y=0 crossing, direction -1")
+    w = W.manifold_fan(n_seeds=4, n_dirs=64)
+    r = batch.dop853_event(w["mu"], w["y0"], w["t0"], w["tmax"], ev_index=1, ev_offset=0.0,
+                           direction=-1, xtol=w["xtol"], gtol=w["gtol"])
+    hit, th, yh, na, nr = oracle.batch_dop853_event(w["mu"], w["y0"], w["tmax"], ev_index=1,
+                                                    direction=-1, xtol=w["xtol"], gtol=w["gtol"])
+    assert_bitwise(r.hit.cpu().numpy(), hit, "hit flags")
+    assert_bitwise(r.t_hit.cpu().numpy(), th, "t_hit", rtol=1e-10)
+    assert_bitwise(r.y_hit.cpu().numpy(), yh, "y_hit", rtol=1e-10)
+    assert_bitwise(r.n_accepted.cpu().numpy(), na, "accepted")
