@@ -1,0 +1,170 @@
+#!/usr/bin/env python
+"""Benchmark of the CR3BP propagation hot path on one MI355X.
+
+Metric: FP64 STM trajectory-steps/s (accepted DOP853 steps of the 42-d
+state+STM system, summed over trajectories, per second of wall time) of the
+persistent work-stealing kernel on the uneven 262,144-trajectory workload,
+with the plain one-launch kernel measured beside it.
+
+A "step" of this script is one propagation of the whole batch to each
+trajectory's final time.  Inputs are uploaded once and stay resident in HBM;
+the timed region contains only kernel launches on the current stream.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+
+Prints one JSON line (see README of the metric fields in DESIGN.md).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64 STM trajectory-steps/s + p50/p99 solve latency, batched vs plain"
+FLOPS_PER_STEP_42 = 8622          # algorithmic FLOPs of one accepted 42-d DOP853 step
+FP64_PEAK_TFLOPS = 78.6           # MI355X FP64 vector/matrix peak
+
+
+def cpu_baseline(w, n_threads: int, target_s: float = 12.0):
+    """Oracle (C restatement, OpenMP) on a strided sample of the same workload."""
+    import numpy as np
+    from oracle import oracle as O
+
+    O.lib()
+    n = w["y0"].shape[0]
+    m = 64
+    spent = 0.0
+    steps = 0
+    used = 0
+    while True:
+        stride = max(1, n // m)
+        idx = np.arange(0, n, stride)[:m]
+        t = time.perf_counter()
+        _, na, nr, st = O.batch_dop853_final(w["mu"][idx], w["y0"][idx], w["t0"][idx],
+                                             w["tf"][idx], max_step=1e4, max_steps=w["max_steps"],
+                                             n_threads=n_threads)
+        dt = time.perf_counter() - t
+        spent, steps, used = dt, int(na.sum()), len(idx)
+        if dt >= target_s / 4 or m >= n:
+            break
+        m = min(n, int(m * max(2.0, min(16.0, target_s / max(dt, 1e-3)))))
+    return {"value": steps / spent, "unit": "trajectory-steps/s", "cores": n_threads,
+            "kind": "port",
+            "sample": f"{used} of {n} trajectories (every {max(1, n // used)}th), "
+                      f"{steps} accepted steps in {spent:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=262144)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--max-steps", type=int, default=200_000)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "trajectories are independent, see DESIGN.md for how the batch would shard.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from hiten_amd import batch, workloads as W
+
+    w = W.persistent_262k(args.n)
+    w["max_steps"] = args.max_steps
+    b = batch.prepare_final(w["mu"], w["y0"], w["t0"], w["tf"], dim=42)
+    kw = dict(rtol=w["rtol"], atol=w["atol"], max_step=1e4, max_steps=args.max_steps,
+              block=args.block, grid=args.grid)
+
+    def run(variant, reps, warm):
+        for _ in range(warm):
+            batch.launch_final(b, variant=variant, **kw)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(reps)]
+        t = time.perf_counter()
+        for e0, e1 in evs:
+            e0.record()
+            batch.launch_final(b, variant=variant, **kw)
+            e1.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
+        return wall, [e0.elapsed_time(e1) / 1e3 for e0, e1 in evs]
+
+    wall, launches = run("persistent", args.steps, args.warmup)
+    y_persist = b.y.clone()
+    nacc = b.n_accepted.to(torch.int64)
+    steps_per_pass = int(nacc.sum().item())
+    rejected = int(b.n_rejected.to(torch.int64).sum().item())
+    status = b.status.cpu().numpy()
+    value = steps_per_pass * args.steps / wall
+    launch_s = sum(launches) / len(launches)
+
+    pwall, plaunch = run("plain", args.plain_steps, 1 if args.plain_steps else 0)
+    same = bool(torch.equal(b.y, y_persist))
+    plain_value = steps_per_pass * args.plain_steps / pwall if args.plain_steps else None
+
+    achieved = steps_per_pass * FLOPS_PER_STEP_42 / launch_s / 1e12
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "trajectory-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "persistent_262k: 42-d DOP853 state+STM, rtol=atol=1e-12, "
+                        "mu per trajectory in {0.012150585609624, 3.0035e-6, 2.528e-5}, "
+                        "states about each L1, t_f log-uniform [0.5, 20]",
+            "n_trajectories": args.n,
+            "kernel": "k_dop853_final42_persist",
+            "max_steps": args.max_steps,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": None,
+        },
+        "accepted_steps_per_pass": steps_per_pass,
+        "rejected_steps_per_pass": rejected,
+        "status_counts": {str(k): int((status == k).sum()) for k in (0, 1, 2)},
+        "launch_s": launches,
+        "plain": {
+            "value": plain_value,
+            "launch_s": plaunch,
+            "persistent_over_plain": (value / plain_value) if plain_value else None,
+            "bitwise_equal_to_persistent": same,
+        },
+    }
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = cpu_baseline(w, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -128,6 +128,64 @@ def rk8_fixed(mu, y0, t0, tf, steps: int, forward: int = 1, flip_indices=None,
     return out.t()
 
 
+@dataclass
+class FinalBatch:
+    """Device-resident inputs and outputs of one DOP853 final-value batch."""
+    dim: int
+    n: int
+    mu: torch.Tensor
+    t0: torch.Tensor
+    tf: torch.Tensor
+    y0: torch.Tensor             # (dim, n) structure-of-arrays
+    y: torch.Tensor              # (dim, n)
+    status: torch.Tensor
+    n_accepted: torch.Tensor
+    n_rejected: torch.Tensor
+    counter: torch.Tensor
+    order: Optional[torch.Tensor] = None
+
+    def result(self) -> FinalResult:
+        return FinalResult(self.y.t(), self.status, self.n_accepted, self.n_rejected)
+
+
+def prepare_final(mu, y0, t0, tf, dim: int = 6, order: str = "span") -> FinalBatch:
+    """Upload a batch once (rows -> SoA) and allocate its outputs.
+
+    ``order`` sets the hand-out order of the persistent kernel: "span" gives
+    the longest integration spans |tf - t0| first (longest-processing-time
+    first, so the slowest trajectories do not start last), "index" keeps the
+    row order.
+    """
+    y = _soa(y0, dim)
+    n = y.shape[1]
+    i32 = dict(dtype=torch.int32, device=y.device)
+    t0_t, tf_t = _vec(t0, n, "t0"), _vec(tf, n, "tf")
+    perm = None
+    if order == "span":
+        perm = torch.argsort(torch.abs(tf_t - t0_t), descending=True, stable=True).to(torch.int32)
+    elif order != "index":
+        raise ValueError(f"unknown order {order!r}")
+    return FinalBatch(dim, n, _vec(mu, n, "mu"), t0_t, tf_t, y, torch.empty_like(y),
+                      torch.empty(n, **i32), torch.empty(n, **i32), torch.empty(n, **i32),
+                      torch.zeros(1, **i32), perm)
+
+
+def launch_final(b: FinalBatch, *, rtol: float = 1e-12, atol: float = 1e-12,
+                 max_step: float = 1e4, min_step: float = MIN_STEP,
+                 max_steps: int = DEFAULT_MAX_STEPS, forward: int = 1, flip_indices=None,
+                 variant: str = "plain", block: int = 0, grid: int = 0, stream=None) -> None:
+    """Enqueue hh_dop853_final on the stream (asynchronous)."""
+    lib = _lib.load()
+    var = {"plain": _lib.HH_VARIANT_PLAIN, "persistent": _lib.HH_VARIANT_PERSISTENT}[variant]
+    rc = lib.hh_dop853_final(b.n, b.dim, _ptr(b.mu), _ptr(b.t0), _ptr(b.tf), float(rtol),
+                             float(atol), float(max_step), float(min_step), int(max_steps),
+                             int(forward), flip_mask(b.dim, flip_indices), _ptr(b.y0), _ptr(b.y),
+                             _ptr(b.status), _ptr(b.n_accepted), _ptr(b.n_rejected), var,
+                             int(block), int(grid), _ptr(b.counter), _ptr(b.order),
+                             _lib.stream_handle(stream))
+    _lib.check(rc, "hh_dop853_final")
+
+
 def dop853_final(mu, y0, t0, tf, *, dim: int = 6, rtol: float = 1e-12, atol: float = 1e-12,
                  max_step: float = 1e4, min_step: float = MIN_STEP,
                  max_steps: int = DEFAULT_MAX_STEPS, forward: int = 1, flip_indices=None,
@@ -136,22 +194,11 @@ def dop853_final(mu, y0, t0, tf, *, dim: int = 6, rtol: float = 1e-12, atol: flo
     """Adaptive DOP853 to tf; the state returned is the dense-output value at tf
     (the last row of _propagate_dynsys(..., steps=2)).
     """
-    lib = _lib.load()
-    y = _soa(y0, dim)
-    n = y.shape[1]
-    mu_t, t0_t, tf_t = _vec(mu, n, "mu"), _vec(t0, n, "t0"), _vec(tf, n, "tf")
-    out = torch.empty_like(y)
-    i32 = dict(dtype=torch.int32, device=y.device)
-    status, nacc, nrej = torch.empty(n, **i32), torch.empty(n, **i32), torch.empty(n, **i32)
-    counter = torch.zeros(1, **i32)
-    var = {"plain": _lib.HH_VARIANT_PLAIN, "persistent": _lib.HH_VARIANT_PERSISTENT}[variant]
-    rc = lib.hh_dop853_final(n, dim, _ptr(mu_t), _ptr(t0_t), _ptr(tf_t), float(rtol), float(atol),
-                             float(max_step), float(min_step), int(max_steps), int(forward),
-                             flip_mask(dim, flip_indices), _ptr(y), _ptr(out), _ptr(status),
-                             _ptr(nacc), _ptr(nrej), var, int(block), int(grid), _ptr(counter),
-                             _lib.stream_handle(stream))
-    _lib.check(rc, "hh_dop853_final")
-    return FinalResult(out.t(), status, nacc, nrej)
+    b = prepare_final(mu, y0, t0, tf, dim)
+    launch_final(b, rtol=rtol, atol=atol, max_step=max_step, min_step=min_step,
+                 max_steps=max_steps, forward=forward, flip_indices=flip_indices, variant=variant,
+                 block=block, grid=grid, stream=stream)
+    return b.result()
 
 
 def dop853_event(mu, y0, t0, tmax, *, ev_index: int = 1, ev_offset: float = 0.0,

@@ -17,8 +17,9 @@ double that Python computed here.
   formula.  The reference (``integrators/coefficients/rk8.py:4-21``) writes each
   coefficient as ``p.0/q.0``; ``float(Fraction(p, q))`` is the same correctly
   rounded quotient.
-* pow tables: ``log(1 + i/128)`` and ``2**(j/128)`` as double-double pairs,
-  evaluated with mpmath at 60 significant digits.
+* pow tables: ``log(1 + i/128)`` and ``2**(j/128)`` as double-double pairs and
+  the rounded reciprocals ``1/(1 + i/128)``, evaluated with mpmath at 60
+  significant digits.
 """
 
 from fractions import Fraction as Fr
@@ -107,8 +108,9 @@ def dd(v):
 
 def pow_tables():
     mpmath.mp.dps = 60
-    log_hi, log_lo, exp_hi, exp_lo = [], [], [], []
+    log_hi, log_lo, exp_hi, exp_lo, invc = [], [], [], [], []
     for i in range(128):
+        invc.append(float(1 / (1 + mpmath.mpf(i) / 128)))
         h, l = dd(mpmath.log(1 + mpmath.mpf(i) / 128))
         log_hi.append(h)
         log_lo.append(l)
@@ -127,7 +129,7 @@ def pow_tables():
     ln2_hi, ln2_lo = split(ln2, 42)
     l128_hi, l128_lo = split(ln2 / 128, 36)
     inv_l128 = float(128 / ln2)
-    return dict(log_hi=log_hi, log_lo=log_lo, exp_hi=exp_hi, exp_lo=exp_lo,
+    return dict(log_hi=log_hi, log_lo=log_lo, exp_hi=exp_hi, exp_lo=exp_lo, invc=invc,
                 ln2_hi=ln2_hi, ln2_lo=ln2_lo, l128_hi=l128_hi, l128_lo=l128_lo,
                 inv_l128=inv_l128)
 
@@ -165,6 +167,7 @@ def main():
     parts.append(emit_array("HH_DOP_D", dop.D))
     parts.append(emit_array("HH_POW_LOG_HI", pt["log_hi"]))
     parts.append(emit_array("HH_POW_LOG_LO", pt["log_lo"]))
+    parts.append(emit_array("HH_POW_INVC", pt["invc"]))
     parts.append(emit_array("HH_POW_EXP_HI", pt["exp_hi"]))
     parts.append(emit_array("HH_POW_EXP_LO", pt["exp_lo"]))
     for k in ("ln2_hi", "ln2_lo", "l128_hi", "l128_lo", "inv_l128"):

@@ -6,7 +6,7 @@
  * -ffp-contract=off, so that a kernel's FP64 results are bit-identical to a
  * scalar CPU evaluation of the reference formulas:
  *   rhs6            dynamics/rtbp.py:65-74   (_crtbp_accel)
- *   Geom42          dynamics/rtbp.py:115-141, 227-242 (Jacobian entries and
+ *   Lane42::rhs     dynamics/rtbp.py:115-141, 227-242 (Jacobian entries and
  *                   the state part of _var_equations)
  *   column_rhs      dynamics/rtbp.py:219-225 (F @ Phi; the zero entries of F
  *                   contribute exactly +-0 to the reference's running sum, so
@@ -20,8 +20,9 @@
  *   Lane42  one trajectory per 8-lane group; lane b of the group owns one
  *           6-double block of the 42-d variational state (b = 0: the state
  *           x, b = 1..6: STM column b-1, b = 7: idle padding).  The state
- *           block is broadcast to the group with ds_swizzle at every stage,
- *           error norms are reduced across the group with an xor butterfly.
+ *           position is broadcast to the group with DPP moves at every stage,
+ *           error norms are reduced across the group with an xor butterfly
+ *           (ds_swizzle).
  */
 #ifndef HH_DEVICE_H
 #define HH_DEVICE_H
@@ -74,6 +75,27 @@ __device__ __forceinline__ double group_get(double v) { return swz<0x18 | (K << 
 template <int X>
 __device__ __forceinline__ double group_xor(double v) { return swz<0x1f | (X << 10)>(v); }
 
+/* Broadcast lane 0 of each aligned 8-lane group to the group with two DPP
+ * moves per dword: quad_perm [0,0,0,0] (lanes 4..7 then hold lane 4), then
+ * row_shr:4 written only to banks 1 and 3 (lanes 4..7 and 12..15 of each
+ * row take lanes 0..3 and 8..11).                                          */
+__device__ __forceinline__ int bcast8_dw(int v) {
+    const int t = __builtin_amdgcn_mov_dpp(v, 0x00, 0xF, 0xF, false);
+    return __builtin_amdgcn_update_dpp(t, t, 0x114, 0xF, 0xA, false);
+}
+
+__device__ __forceinline__ double bcast8(double v) {
+    const int lo = bcast8_dw(__double2loint(v));
+    const int hi = bcast8_dw(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+/* exact negation when bit d of m is set (x * -1.0 as the reference does) */
+__device__ __forceinline__ double neg_if(double x, unsigned m, int d) {
+    const long long b = (long long)((m >> d) & 1u) << 63;
+    return __longlong_as_double(__double_as_longlong(x) ^ b);
+}
+
 /* ---------------------------------------------------------------------- */
 /* CR3BP vector fields                                                      */
 /* ---------------------------------------------------------------------- */
@@ -97,51 +119,17 @@ __device__ __forceinline__ void rhs6(const double (&s)[6], double mu, double (&o
     o[5] = ((-one_mu) * z) / r1c - (mu * z) / r2c;
 }
 
-/* Quantities of _var_equations that depend only on the state x:
- * the Hessian entries of the Jacobian (rtbp.py:115-141) and the state
- * acceleration (rtbp.py:227-242).                                          */
-struct Geom42 {
-    double uxx, uyy, uzz, uxy, uxz, uyz;
-    double ax, ay, az;
-};
-
-__device__ __forceinline__ Geom42 geom42(double x, double y, double z, double vx, double vy,
-                                         double mu, double mu2) {
-    Geom42 g;
-    const double xm = x + mu;
-    const double xr = x - mu2;
-    const double r2 = (xm * xm + y * y) + z * z;
-    const double R2 = (xr * xr + y * y) + z * z;
-    const double r3 = hh_pow15(r2);
-    const double r5 = hh_pow25(r2);
-    const double R3 = hh_pow15(R2);
-    const double R5 = hh_pow25(R2);
-    const double a5 = mu2 / r5;
-    const double b5 = mu / R5;
-    const double c3 = mu2 / r3 + mu / R3;
-    const double a53 = a5 * 3.0;
-    const double b53 = b5 * 3.0;
-    g.uxx = ((1.0 + a53 * (xm * xm)) + b53 * (xr * xr)) - c3;
-    g.uyy = ((1.0 + a53 * (y * y)) + b53 * (y * y)) - c3;
-    g.uzz = ((0.0 + a53 * (z * z)) + b53 * (z * z)) - c3;
-    const double gx = (mu2 * xm) / r5 + (mu * xr) / R5;
-    g.uxy = (3.0 * y) * gx;
-    g.uxz = (3.0 * z) * gx;
-    g.uyz = ((3.0 * y) * z) * (a5 + b5);
-    g.ax = ((x - mu2 * (xm / r3)) - mu * (xr / R3)) + 2.0 * vy;
-    g.ay = ((y - mu2 * (y / r3)) - mu * (y / R3)) - 2.0 * vx;
-    g.az = (-mu2 * (z / r3)) - mu * (z / R3);
-    return g;
-}
-
-/* One STM column p = Phi[:, j] -> (F @ Phi)[:, j]. */
-__device__ __forceinline__ void column_rhs(const Geom42& g, const double (&p)[6], double (&o)[6]) {
+/* One STM column p = Phi[:, j] -> (F @ Phi)[:, j] given the Hessian entries
+ * of the Jacobian (rtbp.py:124-141).                                       */
+__device__ __forceinline__ void column_rhs(double uxx, double uyy, double uzz, double uxy,
+                                           double uxz, double uyz, const double (&p)[6],
+                                           double (&o)[6]) {
     o[0] = 0.0 + p[3];
     o[1] = 0.0 + p[4];
     o[2] = 0.0 + p[5];
-    o[3] = (((0.0 + g.uxx * p[0]) + g.uxy * p[1]) + g.uxz * p[2]) + 2.0 * p[4];
-    o[4] = (((0.0 + g.uxy * p[0]) + g.uyy * p[1]) + g.uyz * p[2]) + (-2.0) * p[3];
-    o[5] = ((0.0 + g.uxz * p[0]) + g.uyz * p[1]) + g.uzz * p[2];
+    o[3] = (((0.0 + uxx * p[0]) + uxy * p[1]) + uxz * p[2]) + 2.0 * p[4];
+    o[4] = (((0.0 + uxy * p[0]) + uyy * p[1]) + uyz * p[2]) + (-2.0) * p[3];
+    o[5] = ((0.0 + uxz * p[0]) + uyz * p[1]) + uzz * p[2];
 }
 
 /* ---------------------------------------------------------------------- */
@@ -152,43 +140,79 @@ __device__ __forceinline__ void column_rhs(const Geom42& g, const double (&p)[6]
 struct Lane6 {
     static constexpr int N = 6;
     double mu;
-    double sgn[6];      /* _DirectedSystem flip signs: +1 or -1 per component */
+    unsigned flipbits;  /* bit d: negate component d (backward _DirectedSystem) */
+    bool flip_on;       /* kernel-uniform: any flip at all                     */
 
     __device__ __forceinline__ void rhs(const double (&s)[6], double (&o)[6]) const {
         rhs6(s, mu, o);
+        if (flip_on) {
 #pragma unroll
-        for (int d = 0; d < 6; ++d) o[d] = o[d] * sgn[d];
+            for (int d = 0; d < 6; ++d) o[d] = neg_if(o[d], flipbits, d);
+        }
     }
     __device__ __forceinline__ double reduce(double v) const { return v; }
-    __device__ __forceinline__ bool group_any(bool v) const { return v; }
 };
 
-/* One trajectory per aligned group of 8 lanes, 42-d variational state. */
+/* One trajectory per aligned group of 8 lanes, 42-d variational state.
+ *
+ * The variational field needs twelve quotients (rtbp.py:124-141 and
+ * 233-242).  The STM-column lanes use only
+ *   mu2/r5, mu/R5, mu2/r3, mu/R3, (mu2 xm)/r5, (mu xr)/R5
+ * and the state lane only
+ *   xm/r3, xr/R3, y/r3, y/R3, z/r3, z/R3,
+ * so each lane forms the six it needs (operands selected per lane) instead of
+ * all twelve.  Only x, y, z are broadcast from the state lane.             */
 struct Lane42 {
     static constexpr int N = 42;
     double mu, mu2;
-    double sgn[6];
+    unsigned flipbits;
+    bool flip_on;
     int blk;            /* lane & 7 */
 
     __device__ __forceinline__ void rhs(const double (&s)[6], double (&o)[6]) const {
-        const double x = group_get<0>(s[0]);
-        const double y = group_get<0>(s[1]);
-        const double z = group_get<0>(s[2]);
-        const double vx = group_get<0>(s[3]);
-        const double vy = group_get<0>(s[4]);
-        const double vz = group_get<0>(s[5]);
-        const Geom42 g = geom42(x, y, z, vx, vy, mu, mu2);
-        double c[6];
-        column_rhs(g, s, c);
+        const double x = bcast8(s[0]);
+        const double y = bcast8(s[1]);
+        const double z = bcast8(s[2]);
         const bool st = (blk == 0);
-        o[0] = st ? vx : c[0];
-        o[1] = st ? vy : c[1];
-        o[2] = st ? vz : c[2];
-        o[3] = st ? g.ax : c[3];
-        o[4] = st ? g.ay : c[4];
-        o[5] = st ? g.az : c[5];
+        const double xm = x + mu;
+        const double xr = x - mu2;
+        const double r2 = (xm * xm + y * y) + z * z;
+        const double R2 = (xr * xr + y * y) + z * z;
+        double r3, r5, R3, R5;
+        hh_pow15_25(r2, &r3, &r5);
+        hh_pow15_25(R2, &R3, &R5);
+        const double q1 = (st ? xm : mu2) / (st ? r3 : r5);
+        const double q2 = (st ? xr : mu) / (st ? R3 : R5);
+        const double q3 = (st ? y : mu2) / r3;
+        const double q4 = (st ? y : mu) / R3;
+        const double q5 = (st ? z : mu2 * xm) / (st ? r3 : r5);
+        const double q6 = (st ? z : mu * xr) / (st ? R3 : R5);
+        if (st) {
+            /* rtbp.py:233-242 */
+            o[0] = s[3];
+            o[1] = s[4];
+            o[2] = s[5];
+            o[3] = ((x - mu2 * q1) - mu * q2) + 2.0 * s[4];
+            o[4] = ((y - mu2 * q3) - mu * q4) - 2.0 * s[3];
+            o[5] = (-mu2 * q5) - mu * q6;
+        } else {
+            /* rtbp.py:124-141 */
+            const double a53 = q1 * 3.0;
+            const double b53 = q2 * 3.0;
+            const double c3 = q3 + q4;
+            const double gx = q5 + q6;
+            const double uxx = ((1.0 + a53 * (xm * xm)) + b53 * (xr * xr)) - c3;
+            const double uyy = ((1.0 + a53 * (y * y)) + b53 * (y * y)) - c3;
+            const double uzz = ((0.0 + a53 * (z * z)) + b53 * (z * z)) - c3;
+            const double uxy = (3.0 * y) * gx;
+            const double uxz = (3.0 * z) * gx;
+            const double uyz = ((3.0 * y) * z) * (q1 + q2);
+            column_rhs(uxx, uyy, uzz, uxy, uxz, uyz, s, o);
+        }
+        if (flip_on) {
 #pragma unroll
-        for (int d = 0; d < 6; ++d) o[d] = o[d] * sgn[d];
+            for (int d = 0; d < 6; ++d) o[d] = neg_if(o[d], flipbits, d);
+        }
     }
     /* canonical block tree: ((b0+b1)+(b2+b3)) + ((b4+b5)+(b6+b7)), b7 = 0 */
     __device__ __forceinline__ double reduce(double v) const {
@@ -342,12 +366,11 @@ __device__ __forceinline__ double dop853_attempt(const P& p, const double (&y)[6
     return (__builtin_fabs(h) * s5) / __builtin_sqrt(denom * (double)P::N);
 }
 
-/* Dense-output interpolant of one step (rk.py:1790-1875); Fc[7][6]. */
+/* The three extra stages of the dense-output extension (rk.py:1836-1857). */
 template <class P>
-__device__ __forceinline__ void dop853_dense(const P& p, const double (&y_old)[6],
-                                             const double (&y_new)[6], double (&k)[DOP_K][6],
-                                             double hs, double (&Fc)[DOP_P][6]) {
-    double kx[DOP_SX - DOP_K][6];
+__device__ __forceinline__ void dop853_ext_stages(const P& p, const double (&y_old)[6],
+                                                  const double (&k)[DOP_K][6], double hs,
+                                                  double (&kx)[DOP_SX - DOP_K][6]) {
 #pragma unroll
     for (int sr = DOP_K; sr < DOP_SX; ++sr) {
         double ys[6];
@@ -363,39 +386,42 @@ __device__ __forceinline__ void dop853_dense(const P& p, const double (&y_old)[6
         }
         p.rhs(ys, kx[sr - DOP_K]);
     }
-#pragma unroll
-    for (int d = 0; d < 6; ++d) {
-        const double dy = y_new[d] - y_old[d];
-        Fc[0][d] = dy;
-        Fc[1][d] = hs * k[0][d] - dy;
-        Fc[2][d] = 2.0 * dy - hs * (k[DOP_S][d] + k[0][d]);
-    }
-#pragma unroll
-    for (int i = 0; i < DOP_P - 3; ++i) {
-#pragma unroll
-        for (int d = 0; d < 6; ++d) {
-            double acc = 0.0;
-#pragma unroll
-            for (int r = 0; r < DOP_SX; ++r) {
-                const double c = HH_DOP_D[i][r];
-                if (c != 0.0) acc = acc + c * (r < DOP_K ? k[r][d] : kx[r - DOP_K][d]);
-            }
-            Fc[3 + i][d] = hs * acc;
-        }
-    }
 }
 
-/* rk.py:1989-2003 for one component */
-__device__ __forceinline__ double dense_eval1(double y_old, const double (&Fc)[DOP_P][6], int d,
-                                              double x) {
-    double v = 0.0;
+/* Interpolant row i (0..6) of component d (rk.py:1859-1874). */
+__device__ __forceinline__ double dense_row(int i, int d, const double (&y_old)[6],
+                                            const double (&y_new)[6], const double (&k)[DOP_K][6],
+                                            const double (&kx)[DOP_SX - DOP_K][6], double hs) {
+    const double dy = y_new[d] - y_old[d];
+    if (i == 0) return dy;
+    if (i == 1) return hs * k[0][d] - dy;
+    if (i == 2) return 2.0 * dy - hs * (k[DOP_S][d] + k[0][d]);
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < DOP_SX; ++r) {
+        const double c = HH_DOP_D[i - 3][r];
+        if (c != 0.0) acc = acc + c * (r < DOP_K ? k[r][d] : kx[r - DOP_K][d]);
+    }
+    return hs * acc;
+}
+
+/* Dense output at abscissa x (rk.py:1989-2003), each row formed as the
+ * Horner recurrence consumes it, so the 7x6 interpolant is never held.    */
+__device__ __forceinline__ void dense_value(const double (&y_old)[6], const double (&y_new)[6],
+                                            const double (&k)[DOP_K][6],
+                                            const double (&kx)[DOP_SX - DOP_K][6], double hs,
+                                            double x, double (&out)[6]) {
     const double omx = 1.0 - x;
 #pragma unroll
-    for (int i = DOP_P - 1; i >= 0; --i) {
-        v = v + Fc[i][d];
-        v = v * (((DOP_P - 1 - i) % 2 == 0) ? x : omx);
+    for (int d = 0; d < 6; ++d) {
+        double v = 0.0;
+#pragma unroll
+        for (int i = DOP_P - 1; i >= 0; --i) {
+            v = v + dense_row(i, d, y_old, y_new, k, kx, hs);
+            v = v * (((DOP_P - 1 - i) % 2 == 0) ? x : omx);
+        }
+        out[d] = v + y_old[d];
     }
-    return v + y_old;
 }
 
 /* rk.py:2005-2102 with the event g = y[ev_index] - ev_offset.  Only the event
@@ -421,20 +447,12 @@ __device__ __forceinline__ double pick6(const double (&v)[6], int i) {
     return r;
 }
 
-__device__ __forceinline__ double refine_bisect(double y_ev0, const double (&Fc)[DOP_P][6], int ev,
+__device__ __forceinline__ double refine_bisect(double y_ev0, const double (&fc)[DOP_P],
                                                 double ev_off, int dir, double h, double xtol,
                                                 double gtol) {
-    /* returns the abscissa x in [0, 1] of the hit */
+    /* returns the abscissa x in [0, 1] of the hit (rk.py:2079-2102) */
     double a = 0.0, b = 1.0;
     double gl = y_ev0 - ev_off;
-    double fc[DOP_P];
-#pragma unroll
-    for (int i = 0; i < DOP_P; ++i) {
-        double c = Fc[i][0];
-#pragma unroll
-        for (int d = 1; d < 6; ++d) c = (ev == d) ? Fc[i][d] : c;
-        fc[i] = c;
-    }
     for (int it = 0; it < 128; ++it) {
         const double mid = 0.5 * (a + b);
         double v = 0.0;

@@ -38,8 +38,18 @@ int hip_fail(hipError_t e, const char* what) {
     return HH_ERR_HIP;
 }
 
-__device__ __forceinline__ double flip_sign(int fwd, unsigned long long flip, int c) {
-    return (fwd == -1 && ((flip >> c) & 1ULL)) ? -1.0 : 1.0;
+/* bits of the 6 components a lane owns that a backward _DirectedSystem
+ * negates (dynamics/base.py:298-304)                                       */
+__device__ __forceinline__ unsigned lane_flip_bits(int fwd, unsigned long long flip, int b,
+                                                   bool is42) {
+    unsigned m = 0;
+    if (fwd != -1) return 0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const int c = is42 ? comp42(b < 7 ? b : 1, d) : d;
+        m |= (unsigned)((flip >> c) & 1ULL) << d;
+    }
+    return m;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -108,11 +118,9 @@ __device__ __forceinline__ void traj_iterate(const P& p, Traj<P>& s, const StepP
 #pragma unroll
                 for (int d = 0; d < 6; ++d) yout[d] = s.y[d];
             } else {
-                double Fc[DOP_P][6];
-                dop853_dense(p, s.y, yh, s.k, hs, Fc);
-                const double x = (s.tf - s.t) / hs;
-#pragma unroll
-                for (int d = 0; d < 6; ++d) yout[d] = dense_eval1(s.y[d], Fc, d, x);
+                double kx[DOP_SX - DOP_K][6];
+                dop853_ext_stages(p, s.y, s.k, hs, kx);
+                dense_value(s.y, yh, s.k, kx, hs, (s.tf - s.t) / hs, yout);
             }
             s.done = true;
         }
@@ -146,8 +154,8 @@ __global__ __launch_bounds__(256) void k_rk8_fixed6(int n, const double* __restr
     if (i >= n) return;
     Lane6 p;
     p.mu = mu[i];
-#pragma unroll
-    for (int d = 0; d < 6; ++d) p.sgn[d] = flip_sign(fwd, flip, d);
+    p.flipbits = lane_flip_bits(fwd, flip, 0, false);
+    p.flip_on = (fwd == -1) && flip != 0;
     double y[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) y[d] = y0[(size_t)d * n + i];
@@ -187,8 +195,8 @@ __global__ __launch_bounds__(256) void k_dop853_final6(
     if (i >= n) return;
     Lane6 p;
     p.mu = mu[i];
-#pragma unroll
-    for (int d = 0; d < 6; ++d) p.sgn[d] = flip_sign(fwd, flip, d);
+    p.flipbits = lane_flip_bits(fwd, flip, 0, false);
+    p.flip_on = (fwd == -1) && flip != 0;
     Traj<Lane6> s;
 #pragma unroll
     for (int d = 0; d < 6; ++d) s.y[d] = y0[(size_t)d * n + i];
@@ -221,8 +229,8 @@ __global__ __launch_bounds__(256) void k_dop853_event6(
     if (i >= n) return;
     Lane6 p;
     p.mu = mu[i];
-#pragma unroll
-    for (int d = 0; d < 6; ++d) p.sgn[d] = flip_sign(fwd, flip, d);
+    p.flipbits = lane_flip_bits(fwd, flip, 0, false);
+    p.flip_on = (fwd == -1) && flip != 0;
     double y[6], k[DOP_K][6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) y[d] = y0[(size_t)d * n + i];
@@ -252,16 +260,23 @@ __global__ __launch_bounds__(256) void k_dop853_event6(
             const double gn = pick6(yh, ev.index) - ev.offset;
             na += 1;
             if (event_crossed(gp, gn, ev.direction)) {
-                double Fc[DOP_P][6];
-                dop853_dense(p, y, yh, k, h, Fc);
-                const double x = refine_bisect(pick6(y, ev.index), Fc, ev.index, ev.offset,
-                                               ev.direction, h, ev.xtol, ev.gtol);
-                th = t + x * h;
+                double kx[DOP_SX - DOP_K][6];
+                dop853_ext_stages(p, y, k, h, kx);
+                double fc[DOP_P];
 #pragma unroll
-                for (int d = 0; d < 6; ++d) {
-                    yh_out[d] = dense_eval1(y[d], Fc, d, x);
-                    ylast[d] = yh[d];
+                for (int r = 0; r < DOP_P; ++r) {
+                    double v = dense_row(r, 0, y, yh, k, kx, h);
+#pragma unroll
+                    for (int d = 1; d < 6; ++d)
+                        v = (ev.index == d) ? dense_row(r, d, y, yh, k, kx, h) : v;
+                    fc[r] = v;
                 }
+                const double x = refine_bisect(pick6(y, ev.index), fc, ev.offset, ev.direction,
+                                               h, ev.xtol, ev.gtol);
+                th = t + x * h;
+                dense_value(y, yh, k, kx, h, x, yh_out);
+#pragma unroll
+                for (int d = 0; d < 6; ++d) ylast[d] = yh[d];
                 found = 1;
                 break;
             }
@@ -315,16 +330,18 @@ struct Final42Args {
     int* nacc;
     int* nrej;
     int* counter;
+    const int* order;   /* work order of the persistent form, or NULL (index order) */
 };
 
 __device__ __forceinline__ void load42(const Final42Args& a, int traj, int b, Lane42& p,
                                        Traj<Lane42>& s) {
     p.mu = a.mu[traj];
     p.mu2 = 1.0 - p.mu;
+    p.flipbits = lane_flip_bits(a.fwd, a.flip, b, true);
+    p.flip_on = (a.fwd == -1) && a.flip != 0;
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
         const int c = comp42(b < 7 ? b : 1, d);
-        p.sgn[d] = flip_sign(a.fwd, a.flip, c);
         s.y[d] = (b < 7) ? a.y0[(size_t)c * a.n + traj] : 0.0;
     }
     traj_init(p, s, a.t0[traj], a.tf[traj], a.sp);
@@ -344,7 +361,8 @@ __device__ __forceinline__ void store42(const Final42Args& a, int traj, int b,
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_dop853_final42_plain(Final42Args a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_dop853_final42_plain(Final42Args a) {
     const int gtid = blockIdx.x * BLOCK + threadIdx.x;
     const int traj = gtid >> 3;
     const int b = gtid & 7;
@@ -367,7 +385,8 @@ __global__ __launch_bounds__(BLOCK) void k_dop853_final42_plain(Final42Args a) {
  * terminates once every trajectory has (each trajectory is bounded by
  * max_steps).                                                               */
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_dop853_final42_persist(Final42Args a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_dop853_final42_persist(Final42Args a) {
     const int lane = threadIdx.x & 63;
     const int b = lane & 7;
     const int grp = lane >> 3;
@@ -391,7 +410,7 @@ __global__ __launch_bounds__(BLOCK) void k_dop853_final42_persist(Final42Args a)
                 const unsigned long long below = m & ((1ULL << (grp * 8)) - 1ULL);
                 const int idx = base + __popcll(below);
                 if (idx < a.n) {
-                    traj = idx;
+                    traj = a.order ? a.order[idx] : idx;
                     load42(a, traj, b, p, s);
 #pragma unroll
                     for (int d = 0; d < 6; ++d) yo[d] = s.y[d];
@@ -434,8 +453,8 @@ __global__ void k_eval_rhs(int n, int dim, const double* __restrict__ mu,
         if (i >= n) return;
         Lane6 p;
         p.mu = mu[i];
-#pragma unroll
-        for (int d = 0; d < 6; ++d) p.sgn[d] = 1.0;
+        p.flipbits = 0;
+        p.flip_on = false;
         double s[6], o[6];
 #pragma unroll
         for (int d = 0; d < 6; ++d) s[d] = y[(size_t)d * n + i];
@@ -450,12 +469,11 @@ __global__ void k_eval_rhs(int n, int dim, const double* __restrict__ mu,
         p.blk = b;
         p.mu = mu[i];
         p.mu2 = 1.0 - p.mu;
+        p.flipbits = 0;
+        p.flip_on = false;
         double s[6], o[6];
 #pragma unroll
-        for (int d = 0; d < 6; ++d) {
-            p.sgn[d] = 1.0;
-            s[d] = (b < 7) ? y[(size_t)comp42(b, d) * n + i] : 0.0;
-        }
+        for (int d = 0; d < 6; ++d) s[d] = (b < 7) ? y[(size_t)comp42(b, d) * n + i] : 0.0;
         p.rhs(s, o);
         if (b < 7) {
 #pragma unroll
@@ -525,7 +543,7 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
                     double rtol, double atol, double max_step, double min_step,
                     long long max_steps, int fwd, unsigned long long flip_mask, const double* y0,
                     double* y_out, int* status, int* n_acc, int* n_rej, int variant, int block,
-                    int grid, int* work_counter, void* stream) {
+                    int grid, int* work_counter, const int* order, void* stream) {
     if (n < 0) return fail(HH_ERR_ARGS, "hh_dop853_final: n < 0");
     if (n == 0) return HH_OK;
     if (dim != 6 && dim != 42) return fail(HH_ERR_ARGS, "hh_dop853_final: dim must be 6 or 42");
@@ -546,6 +564,7 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
     a.n = n; a.mu = mu; a.t0 = t0; a.tf = tf; a.sp = sp; a.fwd = fwd; a.flip = flip_mask;
     a.y0 = y0; a.yout = y_out; a.status = status; a.nacc = n_acc; a.nrej = n_rej;
     a.counter = work_counter;
+    a.order = order;
     if (block <= 0) block = 256;
     if (block != 64 && block != 128 && block != 256 && block != 512)
         return fail(HH_ERR_ARGS, "hh_dop853_final: block must be 64, 128, 256 or 512");

@@ -11,9 +11,9 @@
  * return the same bits:
  *
  *   hh_pow15(x), hh_pow25(x)  x^1.5 and x^2.5, correctly rounded: the square
- *                             root is carried as a double-double (Markstein
- *                             remainder via fma) and the product is summed
- *                             before one final rounding.
+ *                             root's remainder (one fma) gives a double-double
+ *                             correction and the product is summed before one
+ *                             final rounding.
  *   hh_pow(x, y)              x^y for y a double, via a table-driven
  *                             double-double log and exp (~2^-68 relative error
  *                             before the final rounding), i.e. correctly
@@ -68,28 +68,59 @@ HH_FN double hh_pow_pos_special(double x) {
     return __builtin_nan("");                         /* x < 0 or NaN    */
 }
 
-/* Correctly rounded x^1.5. */
+/* Correctly rounded x^1.5 and x^2.5.
+ *
+ * With s = RN(sqrt(x)) and the exact remainder e = x - s*s (one fma),
+ * sqrt(x) = s + d with d = e / (sqrt(x) + s), so
+ *   x * d   = (s * e / 2)     * (1 + O(2^-52))
+ *   x^2 * d = (x * s * e / 2) * (1 + O(2^-52)).
+ * Each correction is ~2^-53 of the result, so evaluating it to double
+ * precision leaves an error of ~2^-104 relative before the single final
+ * rounding of (p + (pe + correction)), where p + pe is the exact product of
+ * the leading terms.  No division is needed.                              */
 HH_FN double hh_pow15(double x) {
     if (!(x > 0.0) || x > 1.7976931348623157e308) return hh_pow_pos_special(x);
     double s = __builtin_sqrt(x);
-    double rem = __builtin_fma(-s, s, x);             /* x - s*s, exact    */
-    double sl = rem / (s + s);                        /* sqrt(x) = s + sl  */
+    double e = __builtin_fma(-s, s, x);               /* x - s*s, exact    */
     double p, pe;
     HH_TWO_PROD(x, s, p, pe);
-    return p + (pe + x * sl);
+    double r = p + (pe + s * (0.5 * e));
+    return (p > 1.7976931348623157e308) ? p : r;      /* overflow -> +inf   */
 }
 
-/* Correctly rounded x^2.5. */
 HH_FN double hh_pow25(double x) {
     if (!(x > 0.0) || x > 1.7976931348623157e308) return hh_pow_pos_special(x);
     double s = __builtin_sqrt(x);
-    double rem = __builtin_fma(-s, s, x);
-    double sl = rem / (s + s);
+    double e = __builtin_fma(-s, s, x);
     double q, qe;
     HH_TWO_PROD(x, x, q, qe);                         /* x^2 = q + qe      */
     double p, pe;
     HH_TWO_PROD(q, s, p, pe);
-    return p + (pe + (q * sl + qe * s));
+    double r = p + (pe + ((x * s) * (0.5 * e) + qe * s));
+    return (p > 1.7976931348623157e308) ? p : r;
+}
+
+/* Both powers of one argument sharing the square root (the variational
+ * field needs r^3 and r^5 of the same r^2).                                */
+HH_FN void hh_pow15_25(double x, double* p15, double* p25) {
+    if (!(x > 0.0) || x > 1.7976931348623157e308) {
+        *p15 = hh_pow_pos_special(x);
+        *p25 = *p15;
+        return;
+    }
+    double s = __builtin_sqrt(x);
+    double e = __builtin_fma(-s, s, x);
+    double he = 0.5 * e;
+    double p, pe;
+    HH_TWO_PROD(x, s, p, pe);
+    double r15 = p + (pe + s * he);
+    *p15 = (p > 1.7976931348623157e308) ? p : r15;
+    double q, qe;
+    HH_TWO_PROD(x, x, q, qe);
+    double p2, pe2;
+    HH_TWO_PROD(q, s, p2, pe2);
+    double r25 = p2 + (pe2 + (p * he + qe * s));
+    *p25 = (p2 > 1.7976931348623157e308) ? p2 : r25;
 }
 
 /* x^y for finite y.  Special cases follow C99 pow for the cases a step-size
@@ -117,8 +148,10 @@ HH_FN double hh_pow(double x, double y) {
     int i = (int)((ux >> 45) & 0x7f);
     double c = 1.0 + (double)i * 0.0078125;
     double dlt = m - c;                               /* exact (Sterbenz)   */
-    double zh = dlt / c;
-    double zl = __builtin_fma(-zh, c, dlt) / c;
+    double ic = HH_POW_INVC[i];                       /* RN(1 / c_i)        */
+    double zh = dlt * ic;
+    /* c_i has 8 significant bits, so dlt - zh*c_i is exact in one fma      */
+    double zl = __builtin_fma(-zh, c, dlt) * ic;
 
     /* log1p(z) = z - z^2/2 + ... ; leading part as a double-double */
     double sq, sqe;

@@ -71,12 +71,16 @@ int hh_rk8_fixed(int n, const double* mu, const double* t0, const double* tf, in
  *   block      : threads per block (dim 6: 64..256; dim 42: 64/128/256/512)
  *   grid       : persistent blocks (0 = 4 waves per CU)
  *   work_counter : one device int, required for the persistent variant;
- *                reset by the library on the stream before the launch     */
+ *                reset by the library on the stream before the launch
+ *   order      : [n] permutation of 0..n-1 giving the order in which the
+ *                persistent variant hands out trajectories (e.g. longest
+ *                t_f first), or NULL for index order; ignored by plain     */
 int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const double* tf,
                     double rtol, double atol, double max_step, double min_step,
                     long long max_steps, int fwd, unsigned long long flip_mask,
                     const double* y0, double* y_out, int* status, int* n_acc, int* n_rej,
-                    int variant, int block, int grid, int* work_counter, void* stream);
+                    int variant, int block, int grid, int* work_counter, const int* order,
+                    void* stream);
 
 /* Adaptive DOP853 (6-d) from t0[i] until the first crossing of the plane
  * y[ev_index] = ev_offset in `direction` (-1, 0, +1), refined by bisection

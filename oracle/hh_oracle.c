@@ -21,6 +21,10 @@
  *                       own test_rk.py / test_events.py known answers.
  *   or_dop853_event     the reference's closed-form event tests
  *                       (integrators/_tests/test_events.py:10-274).
+ * Termination extensions (the reference loops forever in both cases,
+ * rk.py:2452, 2762): an attempt count of max_steps stops with
+ * OR_ERR_MAXSTEPS, and a NaN error norm stops with OR_ERR_NONFINITE; the
+ * state returned is the last accepted one.
  * The step-size sequence (accept/reject pattern) of or_dop853 is pinned only
  * by this restatement of integrators/utils.py and rk.py:2443-2484: no golden
  * sequence exists outside the reference's numba runtime.
@@ -557,6 +561,7 @@ int or_dop853(or_sys *s, const double *y0, const double *t_eval, int m, double r
         h = adjust_step_to_endpoint(t, h, tf);
         dop853_step(s, t, y, h, k, yh, e5, e3);
         double err = dop853_err_norm(n, y, yh, e5, e3, h, rtol, atol);
+        if (err != err) { status = OR_ERR_NONFINITE; break; }
         if (err <= 1.0) {
             double t_new = t + h;
             or_rhs(s, t_new, yh, fn);                                  /* :2474 */
@@ -580,8 +585,10 @@ int or_dop853(or_sys *s, const double *y0, const double *t_eval, int m, double r
     long long last_j = -1;
     for (int q = 0; q < m; ++q) {
         double tq = t_eval[q];
-        if (nn < 2) {                        /* no step taken (status error) */
-            memcpy(y_out + (size_t)q * n, nd.ys, sizeof(double) * n);
+        if (nn < 2 || (status != OR_OK && tq > nd.ts[nn - 1])) {
+            /* no step taken, or past the last accepted node of a run stopped
+             * by max_steps / a non-finite error: the last accepted state */
+            memcpy(y_out + (size_t)q * n, nd.ys + (nn - 1) * n, sizeof(double) * n);
             continue;
         }
         long long j = search_right(nd.ts, nn, tq) - 1;
@@ -664,6 +671,7 @@ int or_dop853_event(or_sys *s, const double *y0, double t0, double tmax, double 
         h = adjust_step_to_endpoint(t, h, tmax);
         dop853_step(s, t, y, h, k, yh, e5, e3);
         double err = dop853_err_norm(n, y, yh, e5, e3, h, rtol, atol);
+        if (err != err) { status = OR_ERR_NONFINITE; break; }
         if (err <= 1.0) {
             double t_new = t + h;
             or_rhs(s, t_new, yh, fn);

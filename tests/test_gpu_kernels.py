@@ -142,3 +142,16 @@ def test_dop853_event_fan(gpu, oracle):
     assert_bitwise(r.t_hit.cpu().numpy(), th, "t_hit", rtol=1e-10)
     assert_bitwise(r.y_hit.cpu().numpy(), yh, "y_hit", rtol=1e-10)
     assert_bitwise(r.n_accepted.cpu().numpy(), na, "accepted")
+
+
+def test_dop853_42d_early_stop_status(gpu, oracle):
+    say("hh_dop853_final dim=42: trajectories that end in a non-finite error norm")
+    w = W.persistent_262k()
+    idx = np.array([183507, 248628, 145063, 223092, 0, 1, 2, 3])
+    args = (w["mu"][idx], w["y0"][idx], w["t0"][idx], w["tf"][idx])
+    r = batch.dop853_final(*args, dim=42, max_step=1e4, max_steps=200_000, variant="persistent")
+    want, na, nr, st = oracle.batch_dop853_final(*args, max_step=1e4, max_steps=200_000)
+    assert_bitwise(r.status.cpu().numpy(), st, "status")
+    assert_bitwise(r.n_accepted.cpu().numpy(), na, "accepted")
+    assert_bitwise(r.n_rejected.cpu().numpy(), nr, "rejected")
+    assert_bitwise(r.y.cpu().numpy(), want, "final/last state")
