@@ -786,3 +786,15 @@ int or_batch_dop853_event(int n, const double *mu, const double *y0, const doubl
     }
     return OR_OK;
 }
+
+/* One DOP853 attempt (rk.py:1636-1699) and its scaled error norm
+ * (rk.py:2457-2467), exposed for single-step tests.                       */
+double or_dop853_single_step(or_sys *s, double t, const double *y, double h, double rtol,
+                             double atol, double *y_high, double *err5, double *err3,
+                             double *k_out) {
+    double k[DOP_S + 1][DIM_MAX];
+    dop853_step(s, t, y, h, k, y_high, err5, err3);
+    if (k_out)
+        for (int r = 0; r < DOP_S + 1; ++r) memcpy(k_out + r * s->dim, k[r], sizeof(double) * s->dim);
+    return dop853_err_norm(s->dim, y, y_high, err5, err3, h, rtol, atol);
+}

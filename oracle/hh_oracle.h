@@ -89,6 +89,11 @@ int or_dop853_event(or_sys *s, const double *y0, double t0, double tmax,
                     int direction, double xtol, double gtol,
                     int *hit, double *t_hit, double *y_hit, double *y_last, or_stats *st);
 
+/* one DOP853 attempt; returns the scaled error norm.  k_out: 13 x dim or NULL */
+double or_dop853_single_step(or_sys *s, double t, const double *y, double h, double rtol,
+                             double atol, double *y_high, double *err5, double *err3,
+                             double *k_out);
+
 /* batched drivers for the CPU baseline (OpenMP, one trajectory per thread).
  * Arrays are trajectory-major: y0[i*dim + c].                              */
 int or_batch_rk8_final(int n, const double *mu, const double *y0, double t0, double tf,

@@ -94,6 +94,9 @@ def _declare(L):
                                   C.c_double, C.c_double, C.c_double, C.c_longlong, C.c_int,
                                   C.c_double, C.c_int, C.c_double, C.c_double, _I, _D, _D, _D,
                                   C.POINTER(OrStats)]
+    L.or_dop853_single_step.argtypes = [C.POINTER(OrSys), C.c_double, _D, C.c_double, C.c_double,
+                                        C.c_double, _D, _D, _D, _D]
+    L.or_dop853_single_step.restype = C.c_double
     L.or_batch_rk8_final.argtypes = [C.c_int, _D, _D, C.c_double, C.c_double, C.c_int, _D, C.c_int]
     L.or_batch_dop853_final.argtypes = [C.c_int, C.c_int, _D, _D, _D, _D, C.c_double, C.c_double,
                                         C.c_double, C.c_longlong, _D, _L, _L, _I, C.c_int]
@@ -210,6 +213,17 @@ def rk_fixed_event(system, y0, t_vals, ev_index, ev_offset=0.0, direction=0, xto
                             ev_offset, direction, xtol, gtol, C.byref(hit), C.byref(th), _p(yh),
                             _p(yl))
     return bool(hit.value), th.value, yh, yl
+
+
+def dop853_single_step(system: System, y, h, t=0.0, rtol=1e-12, atol=1e-12):
+    """One DOP853 attempt: (y_high, err5, err3, k (13, dim), scaled error norm)."""
+    y = _f64(y)
+    n = system.dim
+    yh, e5, e3 = np.empty(n), np.empty(n), np.empty(n)
+    k = np.empty((13, n))
+    err = lib().or_dop853_single_step(C.byref(system.s), float(t), _p(y), float(h), float(rtol),
+                                      float(atol), _p(yh), _p(e5), _p(e3), _p(k))
+    return yh, e5, e3, k, err
 
 
 def dop853(system: System, y0, t_eval, rtol=1e-13, atol=1e-13, max_step=np.inf,
