@@ -201,6 +201,37 @@ def dop853_final(mu, y0, t0, tf, *, dim: int = 6, rtol: float = 1e-12, atol: flo
     return b.result()
 
 
+@dataclass
+class GridResult:
+    """DOP853 dense output on numpy.linspace(t0, tf, m) per trajectory."""
+    states: torch.Tensor         # (n, m, dim)
+    status: torch.Tensor
+    n_accepted: torch.Tensor
+    n_rejected: torch.Tensor
+
+
+def dop853_grid(mu, y0, t0, tf, m: int, *, dim: int = 6, rtol: float = 1e-12,
+                atol: float = 1e-12, max_step: float = 1e4, min_step: float = MIN_STEP,
+                max_steps: int = DEFAULT_MAX_STEPS, forward: int = 1, flip_indices=None,
+                block: int = 0, stream=None) -> GridResult:
+    """States at the m points of numpy.linspace(t0, tf, m), as the reference's
+    dense-output phase produces them (rk.py:2497-2543)."""
+    lib = _lib.load()
+    y = _soa(y0, dim)
+    n = y.shape[1]
+    mu_t, t0_t, tf_t = _vec(mu, n, "mu"), _vec(t0, n, "t0"), _vec(tf, n, "tf")
+    out = torch.empty((n, int(m), dim), dtype=torch.float64, device=y.device)
+    i32 = dict(dtype=torch.int32, device=y.device)
+    status, nacc, nrej = torch.empty(n, **i32), torch.empty(n, **i32), torch.empty(n, **i32)
+    rc = lib.hh_dop853_grid(n, dim, _ptr(mu_t), _ptr(t0_t), _ptr(tf_t), int(m), float(rtol),
+                            float(atol), float(max_step), float(min_step), int(max_steps),
+                            int(forward), flip_mask(dim, flip_indices), _ptr(y), _ptr(out),
+                            _ptr(status), _ptr(nacc), _ptr(nrej), int(block),
+                            _lib.stream_handle(stream))
+    _lib.check(rc, "hh_dop853_grid")
+    return GridResult(out, status, nacc, nrej)
+
+
 def dop853_event(mu, y0, t0, tmax, *, ev_index: int = 1, ev_offset: float = 0.0,
                  direction: int = 0, rtol: float = 1e-12, atol: float = 1e-12,
                  max_step: float = math.inf, min_step: float = MIN_STEP,

@@ -22,6 +22,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <type_traits>
+
 using namespace hh;
 
 namespace {
@@ -86,17 +88,99 @@ __device__ __forceinline__ void traj_init(const P& p, Traj<P>& s, double t0, dou
     s.h = initial_step(p, s.y, s.k[0], sp.rtol, sp.atol, sp.min_step, sp.max_step);
 }
 
-/* One pass of the `while (t - tf) < 0` loop of rk.py:2452-2484.  On the
- * accepted step that reaches t_f the dense-output value at t_f is written to
- * yout (rk.py:2503-2543 for t_eval[-1]) and the trajectory is marked done.  */
-template <class P>
-__device__ __forceinline__ void traj_iterate(const P& p, Traj<P>& s, const StepParams& sp,
-                                             double (&yout)[6]) {
+/* Output policies of traj_iterate.
+ *
+ * FinalOut keeps the dense-output value at tf (rk.py:2503-2543 for
+ * t_eval[-1]); GridOut streams the dense output at every point of
+ * numpy.linspace(t0, tf, m) as soon as the accepted step that covers it is
+ * known: point t_q belongs to the segment [ts[j], ts[j+1]) with
+ * j = searchsorted(ts, t_q, 'right') - 1, clipped to the last segment
+ * (rk.py:2503-2509), so it is emitted after the step ending at ts[j+1] when
+ * t_q < ts[j+1], or after the final step.                                 */
+struct FinalOut {
+    double y[6];
+    __device__ __forceinline__ void init(const double (&y0)[6]) {
+#pragma unroll
+        for (int d = 0; d < 6; ++d) y[d] = y0[d];
+    }
+    __device__ __forceinline__ void constant(const double (&y0)[6]) { init(y0); }
+    __device__ __forceinline__ void stop(const double (&ylast)[6]) { init(ylast); }
+    template <class P>
+    __device__ __forceinline__ void accepted(const P& p, const double (&y)[6], const double (&yh)[6],
+                                             const double (&k)[DOP_K][6], double t, double t_new,
+                                             double tf, bool last) {
+        if (!last) return;
+        const double hs = t_new - t;
+        if (hs == 0.0) {
+            init(y);
+            return;
+        }
+        double kx[DOP_SX - DOP_K][6];
+        dop853_ext_stages(p, y, k, hs, kx);
+        dense_value(y, yh, k, kx, hs, (tf - t) / hs, this->y);
+    }
+};
+
+struct GridOut {
+    double* out;        /* [n][m][dim] base of this trajectory */
+    int m, q, dim, blk;
+    double t0, tf, step, delta;
+    bool zstep;
+
+    __device__ __forceinline__ double tq(int i) const {
+        /* numpy.linspace(t0, tf, m) */
+        if (i == m - 1) return tf;
+        if (zstep) return ((double)i / (double)(m - 1)) * delta + t0;
+        return (double)i * step + t0;
+    }
+    __device__ __forceinline__ void write(int i, const double (&v)[6]) const {
+        if (dim == 6) {
+#pragma unroll
+            for (int d = 0; d < 6; ++d) out[(size_t)i * 6 + d] = v[d];
+        } else if (blk < 7) {
+#pragma unroll
+            for (int d = 0; d < 6; ++d) out[(size_t)i * 42 + comp42(blk, d)] = v[d];
+        }
+    }
+    __device__ __forceinline__ void init(const double (&)[6]) { q = 0; }
+    __device__ __forceinline__ void constant(const double (&y0)[6]) {
+        for (int i = 0; i < m; ++i) write(i, y0);
+        q = m;
+    }
+    __device__ __forceinline__ void stop(const double (&ylast)[6]) {
+        for (; q < m; ++q) write(q, ylast);
+    }
+    template <class P>
+    __device__ __forceinline__ void accepted(const P& p, const double (&y)[6], const double (&yh)[6],
+                                             const double (&k)[DOP_K][6], double t, double t_new,
+                                             double, bool last) {
+        if (q >= m || !(last || tq(q) < t_new)) return;
+        const double hs = t_new - t;
+        if (hs == 0.0) {
+            for (; q < m; ++q) write(q, y);
+            return;
+        }
+        double kx[DOP_SX - DOP_K][6];
+        dop853_ext_stages(p, y, k, hs, kx);
+        for (; q < m; ++q) {
+            const double t_q = tq(q);
+            if (!(last || t_q < t_new)) break;
+            double v[6];
+            dense_value(y, yh, k, kx, hs, (t_q - t) / hs, v);
+            write(q, v);
+        }
+    }
+};
+
+/* One pass of the `while (t - tf) < 0` loop of rk.py:2452-2484.  The
+ * output policy sees every accepted step; the step that reaches tf marks
+ * the trajectory done.                                                      */
+template <class P, class O>
+__device__ __forceinline__ void traj_iterate(const P& p, Traj<P>& s, const StepParams& sp, O& o) {
     if (s.nacc + s.nrej >= sp.max_steps) {
         s.status = ST_MAXSTEPS;
         s.done = true;
-#pragma unroll
-        for (int d = 0; d < 6; ++d) yout[d] = s.y[d];
+        o.stop(s.y);
         return;
     }
     double h = clamp_step(s.h, sp.max_step, sp.min_step);
@@ -106,24 +190,14 @@ __device__ __forceinline__ void traj_iterate(const P& p, Traj<P>& s, const StepP
     if (err != err) {
         s.status = ST_NONFINITE;
         s.done = true;
-#pragma unroll
-        for (int d = 0; d < 6; ++d) yout[d] = s.y[d];
+        o.stop(s.y);
         return;
     }
     if (err <= 1.0) {
         const double t_new = s.t + h;
-        if (!((t_new - s.tf) * 1.0 < 0.0)) {
-            const double hs = t_new - s.t;
-            if (hs == 0.0) {
-#pragma unroll
-                for (int d = 0; d < 6; ++d) yout[d] = s.y[d];
-            } else {
-                double kx[DOP_SX - DOP_K][6];
-                dop853_ext_stages(p, s.y, s.k, hs, kx);
-                dense_value(s.y, yh, s.k, kx, hs, (s.tf - s.t) / hs, yout);
-            }
-            s.done = true;
-        }
+        const bool last = !((t_new - s.tf) * 1.0 < 0.0);
+        o.accepted(p, s.y, yh, s.k, s.t, t_new, s.tf, last);
+        s.done = last;
         s.t = t_new;
 #pragma unroll
         for (int d = 0; d < 6; ++d) {
@@ -186,10 +260,11 @@ __global__ __launch_bounds__(256) void k_rk8_fixed6(int n, const double* __restr
     for (int d = 0; d < 6; ++d) yout[(size_t)d * n + i] = y[d];
 }
 
-__global__ __launch_bounds__(256) void k_dop853_final6(
+template <bool GRID>
+__global__ __launch_bounds__(256) void k_dop853_run6(
     int n, const double* __restrict__ mu, const double* __restrict__ t0a,
     const double* __restrict__ tfa, StepParams sp, int fwd, unsigned long long flip,
-    const double* __restrict__ y0, double* __restrict__ yout, int* __restrict__ status,
+    const double* __restrict__ y0, double* __restrict__ yout, int m, int* __restrict__ status,
     int* __restrict__ nacc, int* __restrict__ nrej) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -200,13 +275,26 @@ __global__ __launch_bounds__(256) void k_dop853_final6(
     Traj<Lane6> s;
 #pragma unroll
     for (int d = 0; d < 6; ++d) s.y[d] = y0[(size_t)d * n + i];
-    double yo[6];
-#pragma unroll
-    for (int d = 0; d < 6; ++d) yo[d] = s.y[d];
+    typename std::conditional<GRID, GridOut, FinalOut>::type o;
+    if constexpr (GRID) {
+        o.out = yout + (size_t)i * m * 6;
+        o.m = m;
+        o.dim = 6;
+        o.blk = 0;
+        o.t0 = t0a[i];
+        o.tf = tfa[i];
+        o.delta = o.tf - o.t0;
+        o.step = o.delta / (double)(m - 1);
+        o.zstep = (o.step == 0.0);
+    }
+    o.init(s.y);
     traj_init(p, s, t0a[i], tfa[i], sp);
-    while (!s.done) traj_iterate(p, s, sp, yo);
+    if (s.done) o.constant(s.y);
+    while (!s.done) traj_iterate(p, s, sp, o);
+    if constexpr (!GRID) {
 #pragma unroll
-    for (int d = 0; d < 6; ++d) yout[(size_t)d * n + i] = yo[d];
+        for (int d = 0; d < 6; ++d) yout[(size_t)d * n + i] = o.y[d];
+    }
     if (status) status[i] = s.status;
     if (nacc) nacc[i] = s.nacc;
     if (nrej) nrej[i] = s.nrej;
@@ -331,6 +419,7 @@ struct Final42Args {
     int* nrej;
     int* counter;
     const int* order;   /* work order of the persistent form, or NULL (index order) */
+    int m;              /* grid points per trajectory (grid form)                  */
 };
 
 __device__ __forceinline__ void load42(const Final42Args& a, int traj, int b, Lane42& p,
@@ -360,7 +449,7 @@ __device__ __forceinline__ void store42(const Final42Args& a, int traj, int b,
     }
 }
 
-template <int BLOCK>
+template <int BLOCK, bool GRID>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_dop853_final42_plain(Final42Args a) {
     const int gtid = blockIdx.x * BLOCK + threadIdx.x;
@@ -371,11 +460,30 @@ void k_dop853_final42_plain(Final42Args a) {
     p.blk = b;
     Traj<Lane42> s;
     load42(a, traj, b, p, s);
-    double yo[6];
-#pragma unroll
-    for (int d = 0; d < 6; ++d) yo[d] = s.y[d];
-    while (!s.done) traj_iterate(p, s, a.sp, yo);
-    store42(a, traj, b, s, yo);
+    typename std::conditional<GRID, GridOut, FinalOut>::type o;
+    if constexpr (GRID) {
+        o.out = a.yout + (size_t)traj * a.m * 42;
+        o.m = a.m;
+        o.dim = 42;
+        o.blk = b;
+        o.t0 = a.t0[traj];
+        o.tf = a.tf[traj];
+        o.delta = o.tf - o.t0;
+        o.step = o.delta / (double)(a.m - 1);
+        o.zstep = (o.step == 0.0);
+    }
+    o.init(s.y);
+    if (s.done) o.constant(s.y);
+    while (!s.done) traj_iterate(p, s, a.sp, o);
+    if constexpr (GRID) {
+        if (b == 0) {
+            if (a.status) a.status[traj] = s.status;
+            if (a.nacc) a.nacc[traj] = s.nacc;
+            if (a.nrej) a.nrej[traj] = s.nrej;
+        }
+    } else {
+        store42(a, traj, b, s, o.y);
+    }
 }
 
 /* Work-stealing form.  Each wave keeps its 8 groups busy: after every step
@@ -393,7 +501,7 @@ void k_dop853_final42_persist(Final42Args a) {
     Lane42 p;
     p.blk = b;
     Traj<Lane42> s;
-    double yo[6];
+    FinalOut o;
     int traj = -1;
     bool exhausted = false;
     s.done = true;
@@ -412,8 +520,7 @@ void k_dop853_final42_persist(Final42Args a) {
                 if (idx < a.n) {
                     traj = a.order ? a.order[idx] : idx;
                     load42(a, traj, b, p, s);
-#pragma unroll
-                    for (int d = 0; d < 6; ++d) yo[d] = s.y[d];
+                    o.init(s.y);
                 } else {
                     exhausted = true;
                 }
@@ -421,7 +528,7 @@ void k_dop853_final42_persist(Final42Args a) {
         }
         if (traj >= 0 && s.done) {
             /* zero-span trajectories finish at load time */
-            store42(a, traj, b, s, yo);
+            store42(a, traj, b, s, o.y);
             traj = -1;
             continue;
         }
@@ -430,9 +537,9 @@ void k_dop853_final42_persist(Final42Args a) {
             continue;
         }
         if (traj >= 0) {
-            traj_iterate(p, s, a.sp, yo);
+            traj_iterate(p, s, a.sp, o);
             if (s.done) {
-                store42(a, traj, b, s, yo);
+                store42(a, traj, b, s, o.y);
                 traj = -1;
             }
         }
@@ -556,8 +663,8 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
         if (variant != HH_VARIANT_PLAIN) return fail(HH_ERR_ARGS, "hh_dop853_final: dim 6 supports the plain variant only");
         if (block <= 0) block = 128;
         if (block > 256 || block % 64) return fail(HH_ERR_ARGS, "hh_dop853_final: block must be 64..256, multiple of 64");
-        hipLaunchKernelGGL(k_dop853_final6, dim3(blocks_for(n, block)), dim3(block), 0, st, n, mu,
-                           t0, tf, sp, fwd, flip_mask, y0, y_out, status, n_acc, n_rej);
+        hipLaunchKernelGGL(k_dop853_run6<false>, dim3(blocks_for(n, block)), dim3(block), 0, st, n,
+                           mu, t0, tf, sp, fwd, flip_mask, y0, y_out, 0, status, n_acc, n_rej);
         return check_launch("hh_dop853_final(6) launch");
     }
     Final42Args a;
@@ -565,16 +672,17 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
     a.y0 = y0; a.yout = y_out; a.status = status; a.nacc = n_acc; a.nrej = n_rej;
     a.counter = work_counter;
     a.order = order;
+    a.m = 0;
     if (block <= 0) block = 256;
     if (block != 64 && block != 128 && block != 256 && block != 512)
         return fail(HH_ERR_ARGS, "hh_dop853_final: block must be 64, 128, 256 or 512");
     if (variant == HH_VARIANT_PLAIN) {
         const int g = blocks_for((long long)n * 8, block);
         switch (block) {
-            case 64: hipLaunchKernelGGL(k_dop853_final42_plain<64>, dim3(g), dim3(64), 0, st, a); break;
-            case 128: hipLaunchKernelGGL(k_dop853_final42_plain<128>, dim3(g), dim3(128), 0, st, a); break;
-            case 256: hipLaunchKernelGGL(k_dop853_final42_plain<256>, dim3(g), dim3(256), 0, st, a); break;
-            default: hipLaunchKernelGGL(k_dop853_final42_plain<512>, dim3(g), dim3(512), 0, st, a); break;
+            case 64: hipLaunchKernelGGL((k_dop853_final42_plain<64, false>), dim3(g), dim3(64), 0, st, a); break;
+            case 128: hipLaunchKernelGGL((k_dop853_final42_plain<128, false>), dim3(g), dim3(128), 0, st, a); break;
+            case 256: hipLaunchKernelGGL((k_dop853_final42_plain<256, false>), dim3(g), dim3(256), 0, st, a); break;
+            default: hipLaunchKernelGGL((k_dop853_final42_plain<512, false>), dim3(g), dim3(512), 0, st, a); break;
         }
         return check_launch("hh_dop853_final(42, plain) launch");
     }
@@ -598,6 +706,42 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
         default: hipLaunchKernelGGL(k_dop853_final42_persist<512>, dim3(grid), dim3(512), 0, st, a); break;
     }
     return check_launch("hh_dop853_final(42, persistent) launch");
+}
+
+int hh_dop853_grid(int n, int dim, const double* mu, const double* t0, const double* tf, int m,
+                   double rtol, double atol, double max_step, double min_step, long long max_steps,
+                   int fwd, unsigned long long flip_mask, const double* y0, double* out,
+                   int* status, int* n_acc, int* n_rej, int block, void* stream) {
+    if (n < 0) return fail(HH_ERR_ARGS, "hh_dop853_grid: n < 0");
+    if (n == 0) return HH_OK;
+    if (dim != 6 && dim != 42) return fail(HH_ERR_ARGS, "hh_dop853_grid: dim must be 6 or 42");
+    if (m < 2) return fail(HH_ERR_ARGS, "hh_dop853_grid: m must be >= 2");
+    if (!mu || !t0 || !tf || !y0 || !out) return fail(HH_ERR_ARGS, "hh_dop853_grid: null pointer");
+    if (!(rtol >= 0.0) || !(atol > 0.0) || !(min_step > 0.0) || !(max_step >= min_step) || max_steps < 1)
+        return fail(HH_ERR_ARGS, "hh_dop853_grid: invalid tolerances/step bounds");
+    StepParams sp = make_sp(rtol, atol, max_step, min_step, max_steps);
+    hipStream_t st = (hipStream_t)stream;
+    if (dim == 6) {
+        if (block <= 0) block = 128;
+        if (block > 256 || block % 64) return fail(HH_ERR_ARGS, "hh_dop853_grid: block must be 64..256, multiple of 64");
+        hipLaunchKernelGGL(k_dop853_run6<true>, dim3(blocks_for(n, block)), dim3(block), 0, st, n,
+                           mu, t0, tf, sp, fwd, flip_mask, y0, out, m, status, n_acc, n_rej);
+        return check_launch("hh_dop853_grid(6) launch");
+    }
+    Final42Args a;
+    memset(&a, 0, sizeof a);
+    a.n = n; a.mu = mu; a.t0 = t0; a.tf = tf; a.sp = sp; a.fwd = fwd; a.flip = flip_mask;
+    a.y0 = y0; a.yout = out; a.status = status; a.nacc = n_acc; a.nrej = n_rej; a.m = m;
+    if (block <= 0) block = 256;
+    const int g = blocks_for((long long)n * 8, block);
+    switch (block) {
+        case 64: hipLaunchKernelGGL((k_dop853_final42_plain<64, true>), dim3(g), dim3(64), 0, st, a); break;
+        case 128: hipLaunchKernelGGL((k_dop853_final42_plain<128, true>), dim3(g), dim3(128), 0, st, a); break;
+        case 256: hipLaunchKernelGGL((k_dop853_final42_plain<256, true>), dim3(g), dim3(256), 0, st, a); break;
+        case 512: hipLaunchKernelGGL((k_dop853_final42_plain<512, true>), dim3(g), dim3(512), 0, st, a); break;
+        default: return fail(HH_ERR_ARGS, "hh_dop853_grid: block must be 64, 128, 256 or 512");
+    }
+    return check_launch("hh_dop853_grid(42) launch");
 }
 
 int hh_dop853_event(int n, const double* mu, const double* t0, const double* tmax, double rtol,

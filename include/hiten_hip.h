@@ -82,6 +82,20 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
                     int variant, int block, int grid, int* work_counter, const int* order,
                     void* stream);
 
+/* Adaptive DOP853 from t0[i] to tf[i] with dense output at the m points of
+ * numpy.linspace(t0[i], tf[i], m): the states of _propagate_dynsys(...,
+ * steps=m, method="adaptive") and the PHI of _compute_stm(..., steps=m).
+ * Replaces: _DOP853._integrate_dop853 with its dense-output phase
+ * (integrators/rk.py:2375-2549) and _propagate_dynsys
+ * (dynamics/base.py:346-458) for the CR3BP systems.  A span that
+ * numpy.isclose calls zero repeats y0 (dynamics/base.py:421-424).
+ *   out : [n][m][dim] (trajectory-major, the reference's (steps, dim)
+ *         array per trajectory)                                           */
+int hh_dop853_grid(int n, int dim, const double* mu, const double* t0, const double* tf, int m,
+                   double rtol, double atol, double max_step, double min_step, long long max_steps,
+                   int fwd, unsigned long long flip_mask, const double* y0, double* out,
+                   int* status, int* n_acc, int* n_rej, int block, void* stream);
+
 /* Adaptive DOP853 (6-d) from t0[i] until the first crossing of the plane
  * y[ev_index] = ev_offset in `direction` (-1, 0, +1), refined by bisection
  * on the dense interpolant, or until tmax[i].

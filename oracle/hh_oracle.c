@@ -585,7 +585,7 @@ int or_dop853(or_sys *s, const double *y0, const double *t_eval, int m, double r
     long long last_j = -1;
     for (int q = 0; q < m; ++q) {
         double tq = t_eval[q];
-        if (nn < 2 || (status != OR_OK && tq > nd.ts[nn - 1])) {
+        if (nn < 2 || (status != OR_OK && tq >= nd.ts[nn - 1])) {
             /* no step taken, or past the last accepted node of a run stopped
              * by max_steps / a non-finite error: the last accepted state */
             memcpy(y_out + (size_t)q * n, nd.ys + (nn - 1) * n, sizeof(double) * n);

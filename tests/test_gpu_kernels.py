@@ -155,3 +155,36 @@ def test_dop853_42d_early_stop_status(gpu, oracle):
     assert_bitwise(r.n_accepted.cpu().numpy(), na, "accepted")
     assert_bitwise(r.n_rejected.cpu().numpy(), nr, "rejected")
     assert_bitwise(r.y.cpu().numpy(), want, "final/last state")
+
+
+@pytest.mark.parametrize("dim,forward,flip", [(6, 1, None), (6, -1, None), (42, 1, None),
+                                              (42, -1, slice(36, 42))])
+def test_dop853_grid_matches_oracle(gpu, oracle, dim, forward, flip):
+    say(f"hh_dop853_grid dim={dim} forward={forward}: 48 trajectories x 257 points")
+    w = W.stm_uneven(n=48)
+    y0 = w["y0"] if dim == 42 else w["y0"][:, 36:]
+    m = 257
+    r = batch.dop853_grid(w["mu"], y0, 0.0, w["tf"], m, dim=dim, forward=forward,
+                          flip_indices=flip)
+    got = r.states.cpu().numpy()
+    kind = "cr3bp42" if dim == 42 else "cr3bp6"
+    for i in range(0, 48, 5):
+        sysm = oracle.System(kind, mu=w["mu"], fwd=forward, flip=flip)
+        want, info = oracle.dop853(sysm, y0[i], np.linspace(0.0, w["tf"][i], m), rtol=1e-12,
+                                   atol=1e-12, max_step=1e4)
+        assert int(r.n_accepted[i]) == info["n_accepted"]
+        assert_bitwise(got[i], want, f"grid traj {i}", rtol=1e-10)
+
+
+def test_dop853_grid_zero_span_and_reversed(gpu, oracle):
+    say("hh_dop853_grid: zero span and reversed span repeat y0")
+    w = W.stm_uneven(n=4)
+    y0 = w["y0"][:, 36:]
+    tf = np.array([1e-12, 0.0, -0.5, 2.0])
+    r = batch.dop853_grid(w["mu"], y0, 0.0, tf, 9, dim=6)
+    got = r.states.cpu().numpy()
+    for i in range(3):
+        assert np.array_equal(got[i], np.repeat(y0[i][None], 9, axis=0))
+    want, _ = oracle.dop853(oracle.System("cr3bp6", mu=w["mu"]), y0[3], np.linspace(0, 2.0, 9),
+                            rtol=1e-12, atol=1e-12, max_step=1e4)
+    assert_bitwise(got[3], want, "regular span")
