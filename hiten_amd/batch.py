@@ -106,22 +106,30 @@ class EventResult:
     n_rejected: torch.Tensor
 
 
-def rk8_fixed(mu, y0, t0, tf, steps: int, forward: int = 1, flip_indices=None,
-              return_trajectory: bool = False, block: int = 0, stream=None):
-    """Fixed-step RK8 over numpy.linspace(t0, tf, steps) (rk.py:531-588).
+def rk8_fixed(mu, y0, t0, tf, steps: int, *, dim: int = 6, t_vals=None, forward: int = 1,
+              flip_indices=None, return_trajectory: bool = False, block: int = 0, stream=None):
+    """Fixed-step RK8 over numpy.linspace(t0, tf, steps), or over the explicit
+    grid ``t_vals`` shared by all trajectories (rk.py:531-588).
 
-    Returns the final states (n, 6); with ``return_trajectory`` also the
-    grid states as an (n, steps, 6) view.
+    Returns the final states (n, dim); with ``return_trajectory`` also the
+    grid states as an (n, steps, dim) view.
     """
     lib = _lib.load()
-    y = _soa(y0, 6)
+    y = _soa(y0, dim)
     n = y.shape[1]
-    mu_t, t0_t, tf_t = _vec(mu, n, "mu"), _vec(t0, n, "t0"), _vec(tf, n, "tf")
+    mu_t = _vec(mu, n, "mu")
+    tv = None
+    if t_vals is not None:
+        tv = _vec(t_vals, len(t_vals), "t_vals")
+        steps = tv.shape[0]
+        t0_t = tf_t = None
+    else:
+        t0_t, tf_t = _vec(t0, n, "t0"), _vec(tf, n, "tf")
     out = torch.empty_like(y)
-    traj = torch.empty((steps, 6, n), dtype=torch.float64, device=y.device) if return_trajectory else None
-    rc = lib.hh_rk8_fixed(n, _ptr(mu_t), _ptr(t0_t), _ptr(tf_t), int(steps), int(forward),
-                          flip_mask(6, flip_indices), _ptr(y), _ptr(out), _ptr(traj), int(block),
-                          _lib.stream_handle(stream))
+    traj = torch.empty((steps, dim, n), dtype=torch.float64, device=y.device) if return_trajectory else None
+    rc = lib.hh_rk8_fixed(n, dim, _ptr(mu_t), _ptr(t0_t), _ptr(tf_t), int(steps), _ptr(tv),
+                          int(forward), flip_mask(dim, flip_indices), _ptr(y), _ptr(out),
+                          _ptr(traj), int(block), _lib.stream_handle(stream))
     _lib.check(rc, "hh_rk8_fixed")
     if return_trajectory:
         return out.t(), traj.permute(2, 0, 1)
@@ -210,20 +218,26 @@ class GridResult:
     n_rejected: torch.Tensor
 
 
-def dop853_grid(mu, y0, t0, tf, m: int, *, dim: int = 6, rtol: float = 1e-12,
+def dop853_grid(mu, y0, t0, tf, m: int, *, dim: int = 6, t_eval=None, rtol: float = 1e-12,
                 atol: float = 1e-12, max_step: float = 1e4, min_step: float = MIN_STEP,
                 max_steps: int = DEFAULT_MAX_STEPS, forward: int = 1, flip_indices=None,
                 block: int = 0, stream=None) -> GridResult:
-    """States at the m points of numpy.linspace(t0, tf, m), as the reference's
+    """States at the m points of numpy.linspace(t0, tf, m) -- or at the explicit
+    times ``t_eval`` shared by all trajectories -- as the reference's
     dense-output phase produces them (rk.py:2497-2543)."""
     lib = _lib.load()
     y = _soa(y0, dim)
     n = y.shape[1]
+    te = None
+    if t_eval is not None:
+        te = _vec(t_eval, len(t_eval), "t_eval")
+        m = te.shape[0]
+        t0, tf = float(te[0]), float(te[-1])
     mu_t, t0_t, tf_t = _vec(mu, n, "mu"), _vec(t0, n, "t0"), _vec(tf, n, "tf")
     out = torch.empty((n, int(m), dim), dtype=torch.float64, device=y.device)
     i32 = dict(dtype=torch.int32, device=y.device)
     status, nacc, nrej = torch.empty(n, **i32), torch.empty(n, **i32), torch.empty(n, **i32)
-    rc = lib.hh_dop853_grid(n, dim, _ptr(mu_t), _ptr(t0_t), _ptr(tf_t), int(m), float(rtol),
+    rc = lib.hh_dop853_grid(n, dim, _ptr(mu_t), _ptr(t0_t), _ptr(tf_t), int(m), _ptr(te), float(rtol),
                             float(atol), float(max_step), float(min_step), int(max_steps),
                             int(forward), flip_mask(dim, flip_indices), _ptr(y), _ptr(out),
                             _ptr(status), _ptr(nacc), _ptr(nrej), int(block),

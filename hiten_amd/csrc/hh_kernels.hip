@@ -123,11 +123,13 @@ struct FinalOut {
 
 struct GridOut {
     double* out;        /* [n][m][dim] base of this trajectory */
+    const double* teval;  /* explicit output times shared by all trajectories, or NULL */
     int m, q, dim, blk;
     double t0, tf, step, delta;
     bool zstep;
 
     __device__ __forceinline__ double tq(int i) const {
+        if (teval) return teval[i];
         /* numpy.linspace(t0, tf, m) */
         if (i == m - 1) return tf;
         if (zstep) return ((double)i / (double)(m - 1)) * delta + t0;
@@ -217,10 +219,51 @@ __device__ __forceinline__ void traj_iterate(const P& p, Traj<P>& s, const StepP
 /* kernels                                                                   */
 /* ------------------------------------------------------------------------ */
 
+/* time of grid point s: t_vals[s] when given, else numpy.linspace(t0, tf, steps) */
+__device__ __forceinline__ double grid_time(const double* t_vals, int s, int steps, double t0,
+                                            double tf, double step, double delta, bool zstep) {
+    if (t_vals) return t_vals[s];
+    if (s == steps - 1) return tf;
+    if (zstep) return ((double)s / (double)(steps - 1)) * delta + t0;
+    return (double)s * step + t0;
+}
+
+/* rk.py:531-588 with h = t[i+1] - t[i].  A span that numpy.isclose calls zero
+ * keeps y0 at every point (dynamics/base.py:421-424).                       */
+template <class P>
+__device__ __forceinline__ void rk8_run(const P& p, double (&y)[6], const double* t_vals,
+                                        int steps, double t0, double tf, double* traj_base,
+                                        size_t traj_stride, int blk) {
+    const double delta = tf - t0;
+    const double step = delta / (double)(steps - 1);
+    const bool zstep = (step == 0.0);
+    const bool constant = t_vals ? isclose(t_vals[0], t_vals[steps - 1]) : isclose(t0, tf);
+    auto store = [&](int s_idx) {
+        if (!traj_base) return;
+        if (P::N == 6) {
+#pragma unroll
+            for (int d = 0; d < 6; ++d) traj_base[((size_t)s_idx * 6 + d) * traj_stride] = y[d];
+        } else if (blk < 7) {
+#pragma unroll
+            for (int d = 0; d < 6; ++d)
+                traj_base[((size_t)s_idx * 42 + comp42(blk, d)) * traj_stride] = y[d];
+        }
+    };
+    store(0);
+    double t_prev = grid_time(t_vals, 0, steps, t0, tf, step, delta, zstep);
+    for (int s = 1; s < steps; ++s) {
+        const double t_s = grid_time(t_vals, s, steps, t0, tf, step, delta, zstep);
+        if (!constant) rk8_step(p, y, t_s - t_prev);
+        t_prev = t_s;
+        store(s);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_rk8_fixed6(int n, const double* __restrict__ mu,
                                                     const double* __restrict__ t0a,
                                                     const double* __restrict__ tfa, int steps,
-                                                    int fwd, unsigned long long flip,
+                                                    const double* __restrict__ t_vals, int fwd,
+                                                    unsigned long long flip,
                                                     const double* __restrict__ y0,
                                                     double* __restrict__ yout,
                                                     double* __restrict__ traj) {
@@ -233,39 +276,44 @@ __global__ __launch_bounds__(256) void k_rk8_fixed6(int n, const double* __restr
     double y[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) y[d] = y0[(size_t)d * n + i];
-    const double t0 = t0a[i], tf = tfa[i];
-    /* numpy.linspace(t0, tf, steps) */
-    const double delta = tf - t0;
-    const double step = delta / (double)(steps - 1);
-    const bool zstep = (step == 0.0);
-    const bool constant = isclose(t0, tf);
-    if (traj) {
-#pragma unroll
-        for (int d = 0; d < 6; ++d) traj[((size_t)0 * 6 + d) * n + i] = y[d];
-    }
-    double t_prev = t0;
-    for (int s = 1; s < steps; ++s) {
-        double t_s;
-        if (s == steps - 1) t_s = tf;
-        else if (zstep) t_s = ((double)s / (double)(steps - 1)) * delta + t0;
-        else t_s = (double)s * step + t0;
-        if (!constant) rk8_step(p, y, t_s - t_prev);
-        t_prev = t_s;
-        if (traj) {
-#pragma unroll
-            for (int d = 0; d < 6; ++d) traj[((size_t)s * 6 + d) * n + i] = y[d];
-        }
-    }
+    rk8_run(p, y, t_vals, steps, t0a[i], tfa[i], traj ? traj + i : nullptr, (size_t)n, 0);
 #pragma unroll
     for (int d = 0; d < 6; ++d) yout[(size_t)d * n + i] = y[d];
+}
+
+/* 42-d fixed-step RK8, one trajectory per 8-lane group; traj: [steps][42][n] */
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_rk8_fixed42(int n, const double* __restrict__ mu, const double* __restrict__ t0a,
+                   const double* __restrict__ tfa, int steps, const double* __restrict__ t_vals,
+                   int fwd, unsigned long long flip, const double* __restrict__ y0,
+                   double* __restrict__ yout, double* __restrict__ traj) {
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gtid >> 3;
+    const int b = gtid & 7;
+    if (i >= n) return;
+    Lane42 p;
+    p.blk = b;
+    p.mu = mu[i];
+    p.mu2 = 1.0 - p.mu;
+    p.flipbits = lane_flip_bits(fwd, flip, b, true);
+    p.flip_on = (fwd == -1) && flip != 0;
+    double y[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) y[d] = (b < 7) ? y0[(size_t)comp42(b, d) * n + i] : 0.0;
+    rk8_run(p, y, t_vals, steps, t0a[i], tfa[i], traj ? traj + i : nullptr, (size_t)n, b);
+    if (b < 7) {
+#pragma unroll
+        for (int d = 0; d < 6; ++d) yout[(size_t)comp42(b, d) * n + i] = y[d];
+    }
 }
 
 template <bool GRID>
 __global__ __launch_bounds__(256) void k_dop853_run6(
     int n, const double* __restrict__ mu, const double* __restrict__ t0a,
     const double* __restrict__ tfa, StepParams sp, int fwd, unsigned long long flip,
-    const double* __restrict__ y0, double* __restrict__ yout, int m, int* __restrict__ status,
-    int* __restrict__ nacc, int* __restrict__ nrej) {
+    const double* __restrict__ y0, double* __restrict__ yout, int m,
+    const double* __restrict__ teval, int* __restrict__ status, int* __restrict__ nacc,
+    int* __restrict__ nrej) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Lane6 p;
@@ -278,6 +326,7 @@ __global__ __launch_bounds__(256) void k_dop853_run6(
     typename std::conditional<GRID, GridOut, FinalOut>::type o;
     if constexpr (GRID) {
         o.out = yout + (size_t)i * m * 6;
+        o.teval = teval;
         o.m = m;
         o.dim = 6;
         o.blk = 0;
@@ -420,6 +469,7 @@ struct Final42Args {
     int* counter;
     const int* order;   /* work order of the persistent form, or NULL (index order) */
     int m;              /* grid points per trajectory (grid form)                  */
+    const double* teval;  /* explicit output times (grid form), or NULL for linspace */
 };
 
 __device__ __forceinline__ void load42(const Final42Args& a, int traj, int b, Lane42& p,
@@ -463,6 +513,7 @@ void k_dop853_final42_plain(Final42Args a) {
     typename std::conditional<GRID, GridOut, FinalOut>::type o;
     if constexpr (GRID) {
         o.out = a.yout + (size_t)traj * a.m * 42;
+        o.teval = a.teval;
         o.m = a.m;
         o.dim = 42;
         o.blk = b;
@@ -631,18 +682,28 @@ const char* hh_last_error(void) { return g_err; }
 
 int hh_abi_version(void) { return HH_ABI_VERSION; }
 
-int hh_rk8_fixed(int n, const double* mu, const double* t0, const double* tf, int steps, int fwd,
-                 unsigned long long flip_mask, const double* y0, double* y_out, double* traj_out,
-                 int block, void* stream) {
+int hh_rk8_fixed(int n, int dim, const double* mu, const double* t0, const double* tf, int steps,
+                 const double* t_vals, int fwd, unsigned long long flip_mask, const double* y0,
+                 double* y_out, double* traj_out, int block, void* stream) {
     if (n < 0) return fail(HH_ERR_ARGS, "hh_rk8_fixed: n < 0");
     if (n == 0) return HH_OK;
+    if (dim != 6 && dim != 42) return fail(HH_ERR_ARGS, "hh_rk8_fixed: dim must be 6 or 42");
     if (steps < 2) return fail(HH_ERR_ARGS, "hh_rk8_fixed: steps must be >= 2");
-    if (!mu || !t0 || !tf || !y0 || !y_out) return fail(HH_ERR_ARGS, "hh_rk8_fixed: null pointer");
-    if (block <= 0) block = 128;
+    if (!mu || !y0 || !y_out || (!t_vals && (!t0 || !tf)))
+        return fail(HH_ERR_ARGS, "hh_rk8_fixed: null pointer");
+    if (block <= 0) block = dim == 6 ? 128 : 256;
     if (block > 256 || block % 64) return fail(HH_ERR_ARGS, "hh_rk8_fixed: block must be 64..256, multiple of 64");
-    hipLaunchKernelGGL(k_rk8_fixed6, dim3(blocks_for(n, block)), dim3(block), 0,
-                       (hipStream_t)stream, n, mu, t0, tf, steps, fwd, flip_mask, y0, y_out,
-                       traj_out);
+    const double* t0p = t0 ? t0 : mu;   /* unused when t_vals is given */
+    const double* tfp = tf ? tf : mu;
+    if (dim == 6) {
+        hipLaunchKernelGGL(k_rk8_fixed6, dim3(blocks_for(n, block)), dim3(block), 0,
+                           (hipStream_t)stream, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0,
+                           y_out, traj_out);
+    } else {
+        hipLaunchKernelGGL(k_rk8_fixed42, dim3(blocks_for((long long)n * 8, block)), dim3(block),
+                           0, (hipStream_t)stream, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask,
+                           y0, y_out, traj_out);
+    }
     return check_launch("hh_rk8_fixed launch");
 }
 
@@ -664,7 +725,8 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
         if (block <= 0) block = 128;
         if (block > 256 || block % 64) return fail(HH_ERR_ARGS, "hh_dop853_final: block must be 64..256, multiple of 64");
         hipLaunchKernelGGL(k_dop853_run6<false>, dim3(blocks_for(n, block)), dim3(block), 0, st, n,
-                           mu, t0, tf, sp, fwd, flip_mask, y0, y_out, 0, status, n_acc, n_rej);
+                           mu, t0, tf, sp, fwd, flip_mask, y0, y_out, 0, (const double*)nullptr,
+                           status, n_acc, n_rej);
         return check_launch("hh_dop853_final(6) launch");
     }
     Final42Args a;
@@ -709,9 +771,10 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
 }
 
 int hh_dop853_grid(int n, int dim, const double* mu, const double* t0, const double* tf, int m,
-                   double rtol, double atol, double max_step, double min_step, long long max_steps,
-                   int fwd, unsigned long long flip_mask, const double* y0, double* out,
-                   int* status, int* n_acc, int* n_rej, int block, void* stream) {
+                   const double* t_eval, double rtol, double atol, double max_step,
+                   double min_step, long long max_steps, int fwd, unsigned long long flip_mask,
+                   const double* y0, double* out, int* status, int* n_acc, int* n_rej, int block,
+                   void* stream) {
     if (n < 0) return fail(HH_ERR_ARGS, "hh_dop853_grid: n < 0");
     if (n == 0) return HH_OK;
     if (dim != 6 && dim != 42) return fail(HH_ERR_ARGS, "hh_dop853_grid: dim must be 6 or 42");
@@ -725,13 +788,15 @@ int hh_dop853_grid(int n, int dim, const double* mu, const double* t0, const dou
         if (block <= 0) block = 128;
         if (block > 256 || block % 64) return fail(HH_ERR_ARGS, "hh_dop853_grid: block must be 64..256, multiple of 64");
         hipLaunchKernelGGL(k_dop853_run6<true>, dim3(blocks_for(n, block)), dim3(block), 0, st, n,
-                           mu, t0, tf, sp, fwd, flip_mask, y0, out, m, status, n_acc, n_rej);
+                           mu, t0, tf, sp, fwd, flip_mask, y0, out, m, t_eval, status, n_acc,
+                           n_rej);
         return check_launch("hh_dop853_grid(6) launch");
     }
     Final42Args a;
     memset(&a, 0, sizeof a);
     a.n = n; a.mu = mu; a.t0 = t0; a.tf = tf; a.sp = sp; a.fwd = fwd; a.flip = flip_mask;
     a.y0 = y0; a.yout = out; a.status = status; a.nacc = n_acc; a.nrej = n_rej; a.m = m;
+    a.teval = t_eval;
     if (block <= 0) block = 256;
     const int g = blocks_for((long long)n * 8, block);
     switch (block) {

@@ -1,0 +1,315 @@
+"""Integrator objects with the reference's interface, running on the GPU.
+
+Mirrors ``hiten.algorithms.integrators``:
+
+* ``_Solution``                     integrators/types.py:29-156
+* ``EventConfig`` / ``EventOptions`` types/configs.py:179-201, types/options.py:265-288
+* ``RungeKutta`` / ``AdaptiveRK`` / ``FixedRK`` factories   rk.py:2871-2979
+* ``_RK8.integrate``                rk.py:422-529 (fixed-step, with events)
+* ``_DOP853.integrate``             rk.py:2221-2373 (adaptive, dense output, events)
+
+Only the CR3BP right-hand sides of :mod:`hiten_amd.dynamics` (6-d field and
+42-d variational system, optionally wrapped by ``_DirectedSystem``) are
+integrated, and events must be coordinate planes ``y[i] - c``
+(:class:`PlaneEventFn`) -- the only event functions the reference's CR3BP
+callers pass (poincare/singlehit/backend.py:30-68).  Anything else raises
+``NotImplementedError``; there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Literal, Optional, Union
+
+import numpy as np
+
+from . import batch
+
+TOL = 1e-13                                  # algorithms/utils/config.py:5
+
+
+@dataclass(frozen=True)
+class EventConfig:
+    """Which crossings to detect (types/configs.py:179-201)."""
+    direction: Literal[-1, 0, 1] = 0
+    terminal: bool = True
+
+    def __post_init__(self):
+        if self.direction not in (-1, 0, 1):
+            raise ValueError(f"Invalid direction: {self.direction}")
+
+
+@dataclass(frozen=True)
+class EventOptions:
+    """Event refinement tolerances (types/options.py:265-288)."""
+    xtol: float = 1e-12
+    gtol: float = 1e-12
+
+    def __post_init__(self):
+        if self.xtol <= 0:
+            raise ValueError("xtol must be positive")
+        if self.gtol <= 0:
+            raise ValueError("gtol must be positive")
+
+
+@dataclass(frozen=True)
+class PlaneEventFn:
+    """g(t, y) = y[index] - offset.  Callable like the reference's event
+    functions (``_g_x0``/``_g_y0``/``_g_z0`` and the cached plane functions of
+    poincare/singlehit/backend.py:30-68)."""
+    index: int
+    offset: float = 0.0
+
+    def __call__(self, t: float, y: np.ndarray) -> float:
+        return float(y[self.index] - self.offset)
+
+
+_g_x0 = PlaneEventFn(0)
+_g_y0 = PlaneEventFn(1)
+_g_z0 = PlaneEventFn(2)
+
+
+@dataclass
+class _Solution:
+    """Discrete solution (integrators/types.py:29-156)."""
+    times: np.ndarray
+    states: np.ndarray
+    derivatives: Optional[np.ndarray] = None
+
+    def __post_init__(self):
+        if len(self.times) != len(self.states):
+            raise ValueError(f"Times and states must have same length: "
+                             f"{len(self.times)} != {len(self.states)}")
+        if self.derivatives is not None and len(self.derivatives) != len(self.times):
+            raise ValueError("If provided, derivatives must have the same length as times "
+                             f"({len(self.derivatives)} != {len(self.times)})")
+
+    def interpolate(self, t: Union[np.ndarray, float]) -> np.ndarray:
+        """Cubic Hermite (with derivatives) or linear interpolation
+        (integrators/types.py:72-156)."""
+        t_arr = np.atleast_1d(t).astype(float)
+        ascending = self.times[0] <= self.times[-1]
+        t_min = self.times[0] if ascending else self.times[-1]
+        t_max = self.times[-1] if ascending else self.times[0]
+        if np.any(t_arr < t_min) or np.any(t_arr > t_max):
+            raise ValueError("Interpolation times must lie within the solution interval.")
+        if ascending:
+            idxs = np.searchsorted(self.times, t_arr, side="right") - 1
+        else:
+            idxs = np.searchsorted(-self.times, -t_arr, side="right") - 1
+        idxs = np.clip(idxs, 0, len(self.times) - 2)
+        t0, t1 = self.times[idxs], self.times[idxs + 1]
+        y0, y1 = self.states[idxs], self.states[idxs + 1]
+        h = t1 - t0
+        s = (t_arr - t0) / h
+        if self.derivatives is None:
+            y_out = y0 + ((y1 - y0).T * s).T
+        else:
+            f0, f1 = self.derivatives[idxs], self.derivatives[idxs + 1]
+            s2 = s * s
+            s3 = s2 * s
+            h00 = 2 * s3 - 3 * s2 + 1
+            h10 = s3 - 2 * s2 + s
+            h01 = -2 * s3 + 3 * s2
+            h11 = s3 - s2
+            y_out = ((h00[:, None] * y0) + (h10[:, None] * (h[:, None] * f0))
+                     + (h01[:, None] * y1) + (h11[:, None] * (h[:, None] * f1)))
+        return y_out[0] if np.isscalar(t) else y_out
+
+
+def _plane_event(event_fn) -> PlaneEventFn:
+    if isinstance(event_fn, PlaneEventFn):
+        return event_fn
+    idx = getattr(event_fn, "index", None)
+    if idx is not None:
+        return PlaneEventFn(int(idx), float(getattr(event_fn, "offset", 0.0)))
+    raise NotImplementedError(
+        "GPU event search supports coordinate-plane events g = y[i] - c "
+        "(PlaneEventFn); arbitrary Python event functions are not supported")
+
+
+def _system_params(system):
+    """(mu, dim, forward, flip_indices) of a CR3BP system, possibly directed."""
+    from .dynamics import _DirectedSystem, _RTBPRHS, _VarEqRHS
+    fwd, flip = 1, None
+    base = system
+    if isinstance(system, _DirectedSystem):
+        fwd, flip, base = system._fwd, system._flip_idx, system._base
+    if isinstance(base, (_RTBPRHS, _VarEqRHS)):
+        return base.mu, base.dim, fwd, flip
+    raise NotImplementedError(
+        f"{type(system).__name__}: only the CR3BP systems rtbp_dynsys / variational_dynsys "
+        "are integrated on the GPU")
+
+
+def _validate(system, y0, t_vals):
+    """integrators/base.py:148-191."""
+    if not hasattr(system, "rhs"):
+        raise ValueError("System must implement 'rhs'")
+    if len(y0) != system.dim:
+        raise ValueError(f"Initial state dimension {len(y0)} != system dimension {system.dim}")
+    if len(t_vals) < 2:
+        raise ValueError("Must provide at least 2 time points")
+    dt = np.diff(t_vals)
+    if np.all(dt == 0.0):
+        return
+    if not (np.all(dt > 0) or np.all(dt < 0)):
+        raise ValueError("Time values must be strictly monotonic (either increasing or decreasing)")
+
+
+def _constant_solution(system, y0, t_vals):
+    """integrators/base.py:199-234: zero-span short-circuit."""
+    if t_vals.size >= 2 and np.isclose(t_vals[0], t_vals[-1]):
+        d0 = system.rhs(t_vals[0], y0)
+        return _Solution(times=t_vals.copy(), states=np.repeat(y0[None, :], t_vals.size, axis=0),
+                         derivatives=np.repeat(np.asarray(d0)[None, :], t_vals.size, axis=0))
+    return None
+
+
+def _derivatives(system, mu, dim, fwd, flip, states):
+    """f(t, y) at every output state, evaluated by the GPU vector field."""
+    d = batch.eval_rhs(mu, states, dim=dim).cpu().numpy()
+    if fwd == -1:
+        mask = batch.flip_mask(dim, flip)
+        sign = np.array([-1.0 if (mask >> i) & 1 else 1.0 for i in range(dim)])
+        d = d * sign
+    return d
+
+
+class _Integrator:
+    name = "integrator"
+
+    def __init__(self, name: str = "", **options):
+        self.name = name or self.name
+        self.options = options
+
+    def __str__(self):
+        return f"HITEN-{self.name}"
+
+
+class _FixedStepRK(_Integrator):
+    """Fixed-step explicit RK over the supplied time nodes (rk.py:394-752)."""
+    _p = 8
+
+    @property
+    def order(self) -> int:
+        return self._p
+
+    def integrate(self, system, y0, t_vals, *, event_fn=None, event_cfg: EventConfig | None = None,
+                  event_options: EventOptions | None = None, **kwargs) -> _Solution:
+        y0 = np.asarray(y0, dtype=np.float64)
+        t_vals = np.asarray(t_vals, dtype=np.float64)
+        _validate(system, y0, t_vals)
+        const = _constant_solution(system, y0, t_vals)
+        if const is not None:
+            return const
+        mu, dim, fwd, flip = _system_params(system)
+        if event_fn is not None:
+            raise NotImplementedError("fixed-step event search runs through DOP853 on the GPU; "
+                                      "use AdaptiveRK(order=8)")
+        fin, traj = batch.rk8_fixed(mu, y0[None, :], None, None, 0, dim=dim, t_vals=t_vals,
+                                    forward=fwd, flip_indices=flip, return_trajectory=True)
+        states = traj[0].cpu().numpy()
+        derivs = _derivatives(system, mu, dim, fwd, flip, states)
+        return _Solution(times=t_vals.copy(), states=states, derivatives=derivs)
+
+
+class _RK8(_FixedStepRK):
+    name = "_RK8"
+
+
+class _AdaptiveStepRK(_Integrator):
+    """Adaptive RK options (rk.py:786-838)."""
+    SAFETY, MIN_FACTOR, MAX_FACTOR = 0.9, 0.2, 10.0
+
+    def __init__(self, name: str = "AdaptiveRK", rtol: float = TOL, atol: float = TOL,
+                 max_step: float = np.inf, min_step: Optional[float] = None,
+                 max_steps: int = batch.DEFAULT_MAX_STEPS, **options):
+        super().__init__(name, **options)
+        self._rtol = rtol
+        self._atol = atol
+        self._max_step = max_step
+        self._min_step = 10.0 * np.finfo(float).eps if min_step is None else min_step
+        self._max_steps = int(max_steps)
+
+
+class _DOP853(_AdaptiveStepRK):
+    """Dormand-Prince 8(5,3) with dense output and events (rk.py:2172-2868)."""
+    _p = 8
+
+    def __init__(self, **opts):
+        super().__init__("_DOP853", **opts)
+
+    @property
+    def order(self) -> int:
+        return self._p
+
+    def integrate(self, system, y0, t_vals, *, event_fn=None, event_cfg: EventConfig | None = None,
+                  event_options: EventOptions | None = None, **kwargs) -> _Solution:
+        y0 = np.asarray(y0, dtype=np.float64)
+        t_vals = np.asarray(t_vals, dtype=np.float64)
+        _validate(system, y0, t_vals)
+        const = _constant_solution(system, y0, t_vals)
+        if const is not None:
+            return const
+        mu, dim, fwd, flip = _system_params(system)
+        step_kw = dict(rtol=self._rtol, atol=self._atol, max_step=self._max_step,
+                       min_step=self._min_step, max_steps=self._max_steps)
+        if event_fn is not None:
+            if dim != 6:
+                raise NotImplementedError("event search runs on the 6-d CR3BP field")
+            ev = _plane_event(event_fn)
+            direction = 0 if event_cfg is None else int(event_cfg.direction)
+            xtol = float(event_options.xtol if event_options is not None else 1e-12)
+            gtol = float(event_options.gtol if event_options is not None else 1e-12)
+            t0, tmax = float(t_vals[0]), float(t_vals[-1])
+            r = batch.dop853_event(mu, y0[None, :], t0, tmax, ev_index=ev.index,
+                                   ev_offset=ev.offset, direction=direction, xtol=xtol, gtol=gtol,
+                                   forward=fwd, flip_indices=flip, **step_kw)
+            if bool(r.hit[0]):
+                return _Solution(times=np.array([t0, float(r.t_hit[0])]),
+                                 states=np.vstack([y0, r.y_hit[0].cpu().numpy()]))
+            return _Solution(times=np.array([t0, tmax]),
+                             states=np.vstack([y0, r.y_last[0].cpu().numpy()]))
+        if t_vals.size == 2 and t_vals[-1] > t_vals[0]:
+            res = batch.dop853_final(mu, y0[None, :], t_vals[0], t_vals[-1], dim=dim, forward=fwd,
+                                     flip_indices=flip, **step_kw)
+            states = np.vstack([y0, res.y[0].cpu().numpy()])
+        else:
+            res = batch.dop853_grid(mu, y0[None, :], None, None, 0, dim=dim, t_eval=t_vals,
+                                    forward=fwd, flip_indices=flip, **step_kw)
+            states = res.states[0].cpu().numpy()
+        derivs = _derivatives(system, mu, dim, fwd, flip, states)
+        return _Solution(times=t_vals.copy(), states=states, derivatives=derivs)
+
+
+class FixedRK:
+    """rk.py:2871-2906 (order 8 only: the configurations use RK8)."""
+    _map = {8: _RK8}
+
+    def __new__(cls, order=8, **opts):
+        if order not in cls._map:
+            raise NotImplementedError("only the 8th-order fixed-step method runs on the GPU")
+        return cls._map[order](**opts)
+
+
+class AdaptiveRK:
+    """rk.py:2909-2943 (order 8: DOP853)."""
+    _map = {8: _DOP853}
+
+    def __new__(cls, order=8, **opts):
+        if order not in cls._map:
+            raise NotImplementedError("only DOP853 (order 8) runs on the GPU")
+        return cls._map[order](**opts)
+
+
+class RungeKutta:
+    """rk.py:2945-2979: 8 -> fixed RK8, 853 -> DOP853."""
+
+    def __new__(cls, order=8, **opts):
+        if order == 853:
+            return AdaptiveRK(order=8, **opts)
+        if order == 8:
+            return FixedRK(order=8, **opts)
+        raise NotImplementedError("Runge-Kutta orders 4, 6 and 45 are not part of this build")

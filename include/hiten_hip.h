@@ -44,17 +44,20 @@ const char* hh_last_error(void);
 /* HH_ABI_VERSION of the built library. */
 int hh_abi_version(void);
 
-/* Fixed-step 13-stage RK8 on the 6-d CR3BP field over the grid
- * numpy.linspace(t0[i], tf[i], steps).
+/* Fixed-step 13-stage RK8 on the 6-d CR3BP field (dim 6) or the 42-d
+ * variational system (dim 42) over the grid t_vals[0..steps-1] when t_vals
+ * is given (shared by all trajectories), else numpy.linspace(t0[i], tf[i],
+ * steps).  Step k uses h = t[k+1] - t[k].
  * Replaces: _FixedStepRK._integrate_fixed_rk (src/hiten/algorithms/
  * integrators/rk.py:531-588) reached through _propagate_dynsys(...,
  * method="fixed", order=8) (dynamics/base.py:430-434).
- *   mu, t0, tf : [n]          y0, y_out : [6][n]
- *   traj_out   : [steps][6][n] every grid state, or NULL
+ *   mu, t0, tf : [n] (t0, tf may be NULL when t_vals is given)
+ *   y0, y_out  : [dim][n]
+ *   traj_out   : [steps][dim][n] every grid state, or NULL
  *   block      : threads per block (64..256, 0 = default)              */
-int hh_rk8_fixed(int n, const double* mu, const double* t0, const double* tf, int steps,
-                 int fwd, unsigned long long flip_mask, const double* y0, double* y_out,
-                 double* traj_out, int block, void* stream);
+int hh_rk8_fixed(int n, int dim, const double* mu, const double* t0, const double* tf, int steps,
+                 const double* t_vals, int fwd, unsigned long long flip_mask, const double* y0,
+                 double* y_out, double* traj_out, int block, void* stream);
 
 /* Adaptive DOP853 from t0[i] to tf[i]; y_out is the dense-output value at
  * tf[i], i.e. the last row of _propagate_dynsys(..., steps=2,
@@ -83,8 +86,11 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
                     void* stream);
 
 /* Adaptive DOP853 from t0[i] to tf[i] with dense output at the m points of
- * numpy.linspace(t0[i], tf[i], m): the states of _propagate_dynsys(...,
- * steps=m, method="adaptive") and the PHI of _compute_stm(..., steps=m).
+ * numpy.linspace(t0[i], tf[i], m), or at t_eval[0..m-1] when t_eval is given
+ * (shared by all trajectories; then t0[i] = t_eval[0] and tf[i] =
+ * t_eval[m-1] are expected): the states of _propagate_dynsys(..., steps=m,
+ * method="adaptive"), the PHI of _compute_stm(..., steps=m), and the
+ * states of _DOP853.integrate(system, y0, t_vals).
  * Replaces: _DOP853._integrate_dop853 with its dense-output phase
  * (integrators/rk.py:2375-2549) and _propagate_dynsys
  * (dynamics/base.py:346-458) for the CR3BP systems.  A span that
@@ -92,9 +98,10 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
  *   out : [n][m][dim] (trajectory-major, the reference's (steps, dim)
  *         array per trajectory)                                           */
 int hh_dop853_grid(int n, int dim, const double* mu, const double* t0, const double* tf, int m,
-                   double rtol, double atol, double max_step, double min_step, long long max_steps,
-                   int fwd, unsigned long long flip_mask, const double* y0, double* out,
-                   int* status, int* n_acc, int* n_rej, int block, void* stream);
+                   const double* t_eval, double rtol, double atol, double max_step,
+                   double min_step, long long max_steps, int fwd, unsigned long long flip_mask,
+                   const double* y0, double* out, int* status, int* n_acc, int* n_rej, int block,
+                   void* stream);
 
 /* Adaptive DOP853 (6-d) from t0[i] until the first crossing of the plane
  * y[ev_index] = ev_offset in `direction` (-1, 0, +1), refined by bisection

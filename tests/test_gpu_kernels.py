@@ -188,3 +188,27 @@ def test_dop853_grid_zero_span_and_reversed(gpu, oracle):
     want, _ = oracle.dop853(oracle.System("cr3bp6", mu=w["mu"]), y0[3], np.linspace(0, 2.0, 9),
                             rtol=1e-12, atol=1e-12, max_step=1e4)
     assert_bitwise(got[3], want, "regular span")
+
+
+def test_rk8_fixed_42d_and_explicit_grid(gpu, oracle):
+    say("hh_rk8_fixed dim=42 over an explicit non-uniform grid")
+    w = W.stm_uneven(n=24)
+    tv = np.concatenate([np.linspace(0.0, 1.0, 120), 1.0 + np.cumsum(np.full(60, 0.01))])
+    fin, traj = batch.rk8_fixed(w["mu"], w["y0"], None, None, 0, dim=42, t_vals=tv,
+                                return_trajectory=True)
+    traj = traj.cpu().numpy()
+    for i in range(0, 24, 7):
+        want = oracle.rk_fixed(oracle.System("cr3bp42", mu=w["mu"]), w["y0"][i], tv)
+        assert_bitwise(traj[i], want, f"rk8 42-d traj {i}")
+
+
+def test_dop853_grid_explicit_t_eval(gpu, oracle):
+    say("hh_dop853_grid with an explicit t_eval array")
+    w = W.stm_uneven(n=16)
+    te = np.sort(np.concatenate([[0.0, 2.2], np.random.default_rng(1).uniform(0.0, 2.2, 40)]))
+    r = batch.dop853_grid(w["mu"], w["y0"][:, 36:], None, None, 0, dim=6, t_eval=te)
+    got = r.states.cpu().numpy()
+    for i in range(0, 16, 3):
+        want, _ = oracle.dop853(oracle.System("cr3bp6", mu=w["mu"]), w["y0"][i, 36:], te,
+                                rtol=1e-12, atol=1e-12, max_step=1e4)
+        assert_bitwise(got[i], want, f"t_eval traj {i}")
