@@ -407,3 +407,27 @@ def batch_dop853_event(mu, y0, tmax, ev_index=1, ev_offset=0.0, direction=-1, rt
                                 hit.ctypes.data_as(_I), _p(th), _p(yh), na.ctypes.data_as(_L),
                                 nr.ctypes.data_as(_L), int(n_threads))
     return hit.astype(bool), th, yh, na, nr
+
+
+class CorrectorEngine:
+    """Oracle-backed operators for hiten_amd.corrector.correct_batch.
+
+    event_batch restates _y_plane_crossing (poincare/singlehit/backend.py:
+    237-280) and stm_batch restates the operators' _compute_stm call
+    (corrector/operators.py:298-307: 500 output points, Phi at the last one).
+    """
+
+    def event_batch(self, mu, states, forward):
+        n = states.shape[0]
+        t = np.full(n, np.nan)
+        xe = np.full((n, 6), np.nan)
+        ok = np.zeros(n, bool)
+        for j in range(n):
+            th, st = plane_crossing(float(mu[j]), states[j], forward=forward)
+            if th is not None:
+                t[j], xe[j], ok[j] = th, st, True
+        return t, xe, ok
+
+    def stm_batch(self, mu, states, t_event):
+        return np.stack([compute_stm(float(m), s, te, steps=500)[2]
+                         for m, s, te in zip(mu, states, t_event)])
