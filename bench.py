@@ -72,6 +72,7 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--heavy", type=int, default=1000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -85,6 +86,8 @@ def main():
 
     from hiten_amd import batch, workloads as W
 
+    from hiten_amd import _lib
+    _lib.check(_lib.load().hh_set_tuning(args.heavy), "hh_set_tuning")
     w = W.persistent_262k(args.n)
     w["max_steps"] = args.max_steps
     b = batch.prepare_final(w["mu"], w["y0"], w["t0"], w["tf"], dim=42)

@@ -17,6 +17,7 @@
  * [c * n + i], c in the reference's order ([Phi row-major | x] for 42-d).
  */
 #include "hh_device.h"
+#include "hh_wave.h"
 #include "../../include/hiten_hip.h"
 
 #include <stdio.h>
@@ -29,6 +30,7 @@ using namespace hh;
 namespace {
 
 thread_local char g_err[512] = "";
+int g_heavy = 1000;   /* persistent 42-d kernel: see hh_set_tuning */
 
 int fail(int code, const char* msg) {
     snprintf(g_err, sizeof g_err, "%s", msg);
@@ -470,6 +472,8 @@ struct Final42Args {
     const int* order;   /* work order of the persistent form, or NULL (index order) */
     int m;              /* grid points per trajectory (grid form)                  */
     const double* teval;  /* explicit output times (grid form), or NULL for linspace */
+    int heavy;            /* persistent form: attempts after which a trajectory is
+                             moved to a wave of its own, 0 = only in the tail */
 };
 
 __device__ __forceinline__ void load42(const Final42Args& a, int traj, int b, Lane42& p,
@@ -537,6 +541,85 @@ void k_dop853_final42_plain(Final42Args a) {
     }
 }
 
+/* Finish the trajectory of the group whose lane 0 is `src` with the
+ * wave-wide layout (Wave42): lane 8*b + d takes entry d of block b from lane
+ * src + b, the loop below is traj_iterate with FinalOut, and the results are
+ * stored as store42 does.  Raises the wave's issue priority meanwhile: such
+ * a trajectory is the longest-running work on its SIMD.                     */
+__device__ __forceinline__ void run_wide(const Final42Args& a, int lane, int src, int traj_lane,
+                                         const Lane42& p, const Traj<Lane42>& s) {
+    Wave42 W;
+    W.setup(lane, rlane(p.mu, src), a.fwd, a.flip);
+    const int from = src + (lane >> 3);
+    double y = 0.0;
+    double k[DOP_K];
+#pragma unroll
+    for (int j = 0; j < DOP_K; ++j) k[j] = 0.0;
+#pragma unroll
+    for (int dd = 0; dd < 6; ++dd) {
+        const double vy = __shfl(s.y[dd], from);
+        const double vk = __shfl(s.k[0][dd], from);
+        if (W.act && W.d == dd) {
+            y = vy;
+            k[0] = vk;
+        }
+    }
+    double t = rlane(s.t, src);
+    const double tf = rlane(s.tf, src);
+    double h = rlane(s.h, src);
+    double err_prev = rlane(s.err_prev, src);
+    int nacc = __builtin_amdgcn_readlane(s.nacc, src);
+    int nrej = __builtin_amdgcn_readlane(s.nrej, src);
+    const int traj = __builtin_amdgcn_readlane(traj_lane, src);
+    const StepParams& sp = a.sp;
+    __builtin_amdgcn_s_setprio(3);
+    double out = y;
+    int status = ST_OK;
+    for (;;) {
+        if (nacc + nrej >= sp.max_steps) {
+            status = ST_MAXSTEPS;
+            out = y;
+            break;
+        }
+        double hh = clamp_step(h, sp.max_step, sp.min_step);
+        hh = adjust_step_to_endpoint(t, hh, tf);
+        double yh;
+        const double err = W.attempt(y, k, hh, sp.rtol, sp.atol, yh);
+        if (err != err) {
+            status = ST_NONFINITE;
+            out = y;
+            break;
+        }
+        double fa, fr;
+        W.step_factors(err, err_prev, fa, fr);
+        if (err <= 1.0) {
+            const double t_new = t + hh;
+            const bool last = !((t_new - tf) * 1.0 < 0.0);
+            if (last) {
+                const double hs = t_new - t;
+                out = (hs == 0.0) ? y : W.dense_at(y, yh, k, hs, (tf - t) / hs);
+            }
+            t = t_new;
+            y = yh;
+            k[0] = k[DOP_S];
+            h = hh * fa;
+            err_prev = err;
+            nacc += 1;
+            if (last) break;
+        } else {
+            h = clamp_step(hh * fr, sp.max_step, sp.min_step);
+            nrej += 1;
+        }
+    }
+    if (W.act) a.yout[(size_t)comp42(lane >> 3, W.d) * a.n + traj] = out;
+    if (lane == 0) {
+        if (a.status) a.status[traj] = status;
+        if (a.nacc) a.nacc[traj] = nacc;
+        if (a.nrej) a.nrej[traj] = nrej;
+    }
+    __builtin_amdgcn_s_setprio(0);
+}
+
 /* Work-stealing form.  Each wave keeps its 8 groups busy: after every step
  * attempt, groups whose trajectory ended take the next indices from a global
  * counter (one atomic per wave, ranks by ballot popcount).  A wave exits when
@@ -557,7 +640,10 @@ void k_dop853_final42_persist(Final42Args a) {
     bool exhausted = false;
     s.done = true;
     for (;;) {
-        const bool need = (traj < 0) && !exhausted;
+        /* A trajectory that has taken `heavy` attempts stops the refill of its
+         * wave, so that it is soon alone there and moves to the wave-wide form */
+        const bool hold = (a.heavy > 0) && __any(traj >= 0 && (s.nacc + s.nrej) >= a.heavy);
+        const bool need = (traj < 0) && !exhausted && !hold;
         const unsigned long long m = __ballot(need && b == 0);
         if (m) {
             const int cnt = __popcll(m);
@@ -585,6 +671,13 @@ void k_dop853_final42_persist(Final42Args a) {
         }
         if (!__any(traj >= 0)) {
             if (__all(exhausted || traj >= 0)) break;
+            continue;
+        }
+        const unsigned long long live = __ballot(traj >= 0 && b == 0);
+        if (__popcll(live) == 1 && (hold || __any(exhausted))) {
+            /* one trajectory left in the wave: finish it one component per lane */
+            run_wide(a, lane, __ffsll((long long)live) - 1, traj, p, s);
+            traj = -1;
             continue;
         }
         if (traj >= 0) {
@@ -682,6 +775,12 @@ const char* hh_last_error(void) { return g_err; }
 
 int hh_abi_version(void) { return HH_ABI_VERSION; }
 
+int hh_set_tuning(int heavy_attempts) {
+    if (heavy_attempts < 0) return fail(HH_ERR_ARGS, "hh_set_tuning: heavy_attempts < 0");
+    g_heavy = heavy_attempts;
+    return HH_OK;
+}
+
 int hh_rk8_fixed(int n, int dim, const double* mu, const double* t0, const double* tf, int steps,
                  const double* t_vals, int fwd, unsigned long long flip_mask, const double* y0,
                  double* y_out, double* traj_out, int block, void* stream) {
@@ -735,6 +834,8 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
     a.counter = work_counter;
     a.order = order;
     a.m = 0;
+    a.teval = nullptr;
+    a.heavy = g_heavy;
     if (block <= 0) block = 256;
     if (block != 64 && block != 128 && block != 256 && block != 512)
         return fail(HH_ERR_ARGS, "hh_dop853_final: block must be 64, 128, 256 or 512");
@@ -751,12 +852,19 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
     if (variant != HH_VARIANT_PERSISTENT) return fail(HH_ERR_ARGS, "hh_dop853_final: unknown variant");
     if (!work_counter) return fail(HH_ERR_ARGS, "hh_dop853_final: persistent variant needs work_counter");
     if (grid <= 0) {
-        int dev = 0, cus = 0;
+        int dev = 0, cus = 0, per_cu = 0;
         hipError_t e = hipGetDevice(&dev);
         if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
         e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         if (e != hipSuccess) return hip_fail(e, "hipDeviceGetAttribute");
-        grid = cus * (1024 / block);
+        switch (block) {
+            case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dop853_final42_persist<64>, 64, 0); break;
+            case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dop853_final42_persist<128>, 128, 0); break;
+            case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dop853_final42_persist<256>, 256, 0); break;
+            default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dop853_final42_persist<512>, 512, 0); break;
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        grid = cus * (per_cu > 0 ? per_cu : 1);
     }
     const long long max_grid = ((long long)n * 8 + block - 1) / block;
     if (grid > max_grid) grid = (int)max_grid;

@@ -157,6 +157,59 @@ def test_dop853_42d_early_stop_status(gpu, oracle):
     assert_bitwise(r.y.cpu().numpy(), want, "final/last state")
 
 
+@pytest.fixture
+def heavy_threshold():
+    """Set the persistent kernel's heavy-trajectory threshold; restore 1000."""
+    from hiten_amd import _lib
+    lib = _lib.load()
+
+    def set_(k):
+        _lib.check(lib.hh_set_tuning(int(k)), "hh_set_tuning")
+    yield set_
+    set_(1000)
+
+
+@pytest.mark.parametrize("heavy", [0, 3, 40])
+def test_dop853_42d_wave_wide_forward(gpu, oracle, heavy_threshold, heavy):
+    """Trajectories finished one component per lane (after `heavy` attempts,
+    or in the tail for 0) match the oracle bit for bit, including the
+    non-finite early stops."""
+    say(f"hh_dop853_final dim=42 persistent, heavy={heavy}: 200 trajectories + 4 early stops")
+    heavy_threshold(heavy)
+    w = W.stm_uneven(n=200)
+    w2 = W.persistent_262k()
+    idx = np.array([183507, 248628, 145063, 223092])
+    mu = np.concatenate([np.full(200, w["mu"]), w2["mu"][idx]])
+    y0 = np.concatenate([w["y0"], w2["y0"][idx]])
+    t0 = np.concatenate([w["t0"], w2["t0"][idx]])
+    tf = np.concatenate([w["tf"], w2["tf"][idx]])
+    r = batch.dop853_final(mu, y0, t0, tf, dim=42, max_step=1e4, max_steps=200_000,
+                           variant="persistent")
+    want, na, nr, st = oracle.batch_dop853_final(mu, y0, t0, tf, max_step=1e4, max_steps=200_000)
+    assert_bitwise(r.status.cpu().numpy(), st, "status")
+    assert_bitwise(r.n_accepted.cpu().numpy(), na, "accepted")
+    assert_bitwise(r.n_rejected.cpu().numpy(), nr, "rejected")
+    assert_bitwise(r.y.cpu().numpy(), want, f"42-d final, heavy={heavy}", rtol=1e-10)
+
+
+def test_dop853_42d_wave_wide_backward_flip(gpu, oracle, heavy_threshold):
+    """Backward _DirectedSystem with the STM-part flip (compute_stm's
+    slice(36, 42)) through the one-component-per-lane form."""
+    say("hh_dop853_final dim=42 persistent backward flip, heavy=2: 24 trajectories")
+    heavy_threshold(2)
+    w = W.stm_uneven(n=24)
+    flip = slice(36, 42)
+    r = batch.dop853_final(w["mu"], w["y0"], 0.0, w["tf"], dim=42, max_step=1e4, forward=-1,
+                           flip_indices=flip, variant="persistent")
+    got = r.y.cpu().numpy()
+    for i in range(0, 24, 3):
+        sysm = oracle.System("cr3bp42", mu=w["mu"], fwd=-1, flip=flip)
+        want, info = oracle.dop853(sysm, w["y0"][i], np.array([0.0, w["tf"][i]]), rtol=1e-12,
+                                   atol=1e-12, max_step=1e4)
+        assert int(r.n_accepted[i]) == info["n_accepted"]
+        assert_bitwise(got[i], want[-1], f"backward traj {i}", rtol=1e-10)
+
+
 @pytest.mark.parametrize("dim,forward,flip", [(6, 1, None), (6, -1, None), (42, 1, None),
                                               (42, -1, slice(36, 42))])
 def test_dop853_grid_matches_oracle(gpu, oracle, dim, forward, flip):
