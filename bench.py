@@ -32,6 +32,22 @@ FLOPS_PER_STEP_42 = 8622          # algorithmic FLOPs of one accepted 42-d DOP85
 FP64_PEAK_TFLOPS = 78.6           # MI355X FP64 vector/matrix peak
 
 
+PROFILE = os.path.join(ROOT, "profiles", "k_dop853_final42_persist__persistent_262k.md")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the persistent kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated)."""
+    try:
+        with open(PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return int(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(w, n_threads: int, target_s: float = 12.0):
     """Oracle (C restatement, OpenMP) on a strided sample of the same workload."""
     import numpy as np
@@ -150,7 +166,7 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": profiled_traffic(),
         },
         "accepted_steps_per_pass": steps_per_pass,
         "rejected_steps_per_pass": rejected,
