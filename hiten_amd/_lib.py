@@ -52,6 +52,7 @@ _SIGNATURES = {
                          _P, _P, _P, _P, _P, _P, _P, _I, _P], _I),
     "hh_eval_rhs": ([_I, _I, _P, _P, _P, _P], _I),
     "hh_eval_math": ([_I, _P, _P, _P, _P], _I),
+    "hh_traj_filters": ([_I, _I, _P, _P, _P, _P, _P, _P], _I),
 }
 
 EXPORTED = tuple(_SIGNATURES)

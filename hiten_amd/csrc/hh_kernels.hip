@@ -695,6 +695,70 @@ __global__ void k_fill_int(int* p, int v, int n) {
     if (i < n) p[i] = v;
 }
 
+/* ---- trajectory filters of the manifold computation ---------------------- */
+
+/* Jacobi constant as common/energy.py:51-54 writes it (x**2 -> x*x, **0.5 ->
+ * the correctly rounded square root) */
+__device__ __forceinline__ double jacobi_c(double x, double y, double z, double vx, double vy,
+                                           double vz, double mu1, double mu2) {
+    const double r1 = __builtin_sqrt(((x + mu2) * (x + mu2) + y * y) + z * z);
+    const double r2 = __builtin_sqrt(((x - mu1) * (x - mu1) + y * y) + z * z);
+    return ((x * x + y * y) + 2.0 * (mu1 / r1 + mu2 / r2)) - ((vx * vx + vy * vy) + vz * vz);
+}
+
+/* NaN-propagating minimum (numpy.min) */
+__device__ __forceinline__ double nan_min(double a, double b) {
+    if (a != a || b != b) return a + b;
+    return b < a ? b : a;
+}
+
+/* One wave per trajectory of an [n][m][6] state grid; lanes stride over the
+ * samples.  min_r1/min_r2: numpy min of the primary distances of
+ * services/manifold.py:416-420; max_err: _max_rel_energy_error
+ * (common/energy.py:26-74), whose running maximum skips NaN terms.  Minimum
+ * and maximum are exact, so the lane order does not change the result.    */
+__global__ __launch_bounds__(256) void k_traj_filters(int n, int m, const double* __restrict__ mu,
+                                                      const double* __restrict__ states,
+                                                      double* __restrict__ min_r1,
+                                                      double* __restrict__ min_r2,
+                                                      double* __restrict__ max_err) {
+    const int traj = (blockIdx.x * 256 + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (traj >= n) return;
+    const double mu2 = mu[traj];
+    const double mu1 = 1.0 - mu2;
+    const double* S = states + (size_t)traj * m * 6;
+    const double C0 = jacobi_c(S[0], S[1], S[2], S[3], S[4], S[5], mu1, mu2);
+    const double absC0 = __builtin_fabs(C0);
+    double r1m = __builtin_inf(), r2m = __builtin_inf(), em = 0.0;
+    for (int i = lane; i < m; i += 64) {
+        const double* q = S + (size_t)i * 6;
+        const double x = q[0], y = q[1], z = q[2];
+        const double xa = x + mu2;
+        const double xb = (x - 1.0) + mu2;
+        r1m = nan_min(r1m, __builtin_sqrt((xa * xa + y * y) + z * z));
+        r2m = nan_min(r2m, __builtin_sqrt((xb * xb + y * y) + z * z));
+        if (i >= 1) {
+            const double Ci = jacobi_c(x, y, z, q[3], q[4], q[5], mu1, mu2);
+            const double rel = (absC0 > 1e-14) ? __builtin_fabs(Ci - C0) / absC0
+                                               : __builtin_fabs(Ci - C0);
+            if (rel > em) em = rel;
+        }
+    }
+#pragma unroll
+    for (int sh = 32; sh >= 1; sh >>= 1) {
+        r1m = nan_min(r1m, __shfl_xor(r1m, sh));
+        r2m = nan_min(r2m, __shfl_xor(r2m, sh));
+        const double o = __shfl_xor(em, sh);
+        em = (o > em) ? o : em;
+    }
+    if (lane == 0) {
+        if (min_r1) min_r1[traj] = r1m;
+        if (min_r2) min_r2[traj] = r2m;
+        if (max_err) max_err[traj] = em;
+    }
+}
+
 /* ---- unit probes: RHS and pow on given inputs --------------------------- */
 __global__ void k_eval_rhs(int n, int dim, const double* __restrict__ mu,
                            const double* __restrict__ y, double* __restrict__ out) {
@@ -959,6 +1023,18 @@ int hh_eval_math(int n, const double* x, const double* y, double* out, void* str
     hipLaunchKernelGGL(k_eval_math, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream,
                        n, x, y, out);
     return check_launch("hh_eval_math launch");
+}
+
+int hh_traj_filters(int n, int m, const double* mu, const double* states, double* min_r1,
+                    double* min_r2, double* max_energy_err, void* stream) {
+    if (n < 0 || m < 1) return fail(HH_ERR_ARGS, "hh_traj_filters: n < 0 or m < 1");
+    if (n == 0) return HH_OK;
+    if (!mu || !states) return fail(HH_ERR_ARGS, "hh_traj_filters: null pointer");
+    const long long threads = (long long)n * 64;
+    const int blocks = (int)((threads + 255) / 256);
+    hipLaunchKernelGGL(k_traj_filters, dim3(blocks), dim3(256), 0, (hipStream_t)stream, n, m, mu,
+                       states, min_r1, min_r2, max_energy_err);
+    return check_launch("hh_traj_filters launch");
 }
 
 }  /* extern "C" */

@@ -140,6 +140,15 @@ int hh_eval_rhs(int n, int dim, const double* mu, const double* y, double* out, 
  * obtains from libm: np.sqrt, '/', and '**' in rtbp.py / utils.py).       */
 int hh_eval_math(int n, const double* x, const double* y, double* out, void* stream);
 
+/* Filters of a dense-output state grid states[n][m][6] (row-major per
+ * trajectory), as the manifold computation applies them to every arc:
+ * min_r1[i], min_r2[i] = minimum distance to the primaries, NaN if any
+ * sample is NaN (services/manifold.py:416-420), and max_energy_err[i] =
+ * _max_rel_energy_error of the samples (common/energy.py:26-74).
+ * Any output pointer may be NULL.                                          */
+int hh_traj_filters(int n, int m, const double* mu, const double* states, double* min_r1,
+                    double* min_r2, double* max_energy_err, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

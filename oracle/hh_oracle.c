@@ -798,3 +798,48 @@ double or_dop853_single_step(or_sys *s, double t, const double *y, double h, dou
         for (int r = 0; r < DOP_S + 1; ++r) memcpy(k_out + r * s->dim, k[r], sizeof(double) * s->dim);
     return dop853_err_norm(s->dim, y, y_high, err5, err3, h, rtol, atol);
 }
+
+/* common/energy.py:51-54: r = (...)**0.5 (the correctly rounded root) */
+double or_jacobi(const double *s, double mu) {
+    const double mu1 = 1.0 - mu, mu2 = mu;
+    const double x = s[0], y = s[1], z = s[2], vx = s[3], vy = s[4], vz = s[5];
+    const double r1 = sqrt((x + mu2) * (x + mu2) + y * y + z * z);
+    const double r2 = sqrt((x - mu1) * (x - mu1) + y * y + z * z);
+    return x * x + y * y + 2.0 * (mu1 / r1 + mu2 / r2) - (vx * vx + vy * vy + vz * vz);
+}
+
+/* common/energy.py:26-74 */
+double or_max_rel_energy_error(const double *states, int m, double mu) {
+    const double C0 = or_jacobi(states, mu);
+    const double absC0 = fabs(C0);
+    double max_err = 0.0;
+    for (int i = 1; i < m; ++i) {
+        const double Ci = or_jacobi(states + (size_t)i * 6, mu);
+        double rel_err;
+        if (absC0 > 1e-14) rel_err = fabs(Ci - C0) / absC0;
+        else rel_err = fabs(Ci - C0);
+        if (rel_err > max_err) max_err = rel_err;
+    }
+    return max_err;
+}
+
+/* services/manifold.py:413-420: r1 = sqrt((x+mu)**2 + y**2 + z**2),
+ * r2 = sqrt((x-1+mu)**2 + ...), then numpy .min() (NaN-propagating) */
+void or_traj_filters(int n, int m, const double *mu, const double *states, double *min_r1,
+                     double *min_r2, double *max_err) {
+    for (int t = 0; t < n; ++t) {
+        const double *S = states + (size_t)t * m * 6;
+        double a = INFINITY, b = INFINITY;
+        int nan_a = 0, nan_b = 0;
+        for (int i = 0; i < m; ++i) {
+            const double x = S[i * 6], y = S[i * 6 + 1], z = S[i * 6 + 2];
+            const double r1 = sqrt((x + mu[t]) * (x + mu[t]) + y * y + z * z);
+            const double r2 = sqrt((x - 1 + mu[t]) * (x - 1 + mu[t]) + y * y + z * z);
+            if (r1 != r1) nan_a = 1; else if (r1 < a) a = r1;
+            if (r2 != r2) nan_b = 1; else if (r2 < b) b = r2;
+        }
+        if (min_r1) min_r1[t] = nan_a ? NAN : a;
+        if (min_r2) min_r2[t] = nan_b ? NAN : b;
+        if (max_err) max_err[t] = or_max_rel_energy_error(S, m, mu[t]);
+    }
+}

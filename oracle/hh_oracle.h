@@ -108,4 +108,13 @@ int or_batch_dop853_event(int n, const double *mu, const double *y0, const doubl
                           double gtol, int *hit, double *t_hit, double *y_hit,
                           long long *n_acc, long long *n_rej, int n_threads);
 
+/* Jacobi constant of one state as _max_rel_energy_error's inner _jacobi
+ * writes it (common/energy.py:51-54) and that function over an m-sample
+ * trajectory (common/energy.py:26-74), plus the primary-distance minima of
+ * the manifold filter (services/manifold.py:416-420).                      */
+double or_jacobi(const double *s, double mu);
+double or_max_rel_energy_error(const double *states, int m, double mu);
+void or_traj_filters(int n, int m, const double *mu, const double *states, double *min_r1,
+                     double *min_r2, double *max_err);
+
 #endif /* HH_ORACLE_H */

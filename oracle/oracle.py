@@ -100,6 +100,11 @@ def _declare(L):
     L.or_batch_rk8_final.argtypes = [C.c_int, _D, _D, C.c_double, C.c_double, C.c_int, _D, C.c_int]
     L.or_batch_dop853_final.argtypes = [C.c_int, C.c_int, _D, _D, _D, _D, C.c_double, C.c_double,
                                         C.c_double, C.c_longlong, _D, _L, _L, _I, C.c_int]
+    L.or_jacobi.argtypes = [_D, C.c_double]
+    L.or_jacobi.restype = C.c_double
+    L.or_max_rel_energy_error.argtypes = [_D, C.c_int, C.c_double]
+    L.or_max_rel_energy_error.restype = C.c_double
+    L.or_traj_filters.argtypes = [C.c_int, C.c_int, _D, _D, _D, _D, _D]
     L.or_batch_dop853_event.argtypes = [C.c_int, _D, _D, _D, C.c_double, C.c_double, C.c_double,
                                         C.c_longlong, C.c_int, C.c_double, C.c_int, C.c_double,
                                         C.c_double, _I, _D, _D, _L, _L, C.c_int]
@@ -388,6 +393,26 @@ def batch_dop853_final(mu, y0, t0, tf, rtol=1e-12, atol=1e-12, max_step=1e4,
                                 max_steps, _p(out), na.ctypes.data_as(_L), nr.ctypes.data_as(_L),
                                 stt.ctypes.data_as(_I), int(n_threads))
     return out, na, nr, stt
+
+
+def jacobi(state, mu):
+    return lib().or_jacobi(_p(_f64(state)), float(mu))
+
+
+def max_rel_energy_error(states, mu):
+    """_max_rel_energy_error (common/energy.py:26-74) of one (m, 6) trajectory."""
+    S = _f64(states)
+    return lib().or_max_rel_energy_error(_p(S), S.shape[0], float(mu))
+
+
+def traj_filters(mu, states):
+    """(min_r1, min_r2, max_energy_err) of each (m, 6) trajectory in states (n, m, 6)."""
+    S = _f64(states)
+    n, m = S.shape[0], S.shape[1]
+    mu = _f64(np.broadcast_to(mu, (n,)))
+    a, b, e = np.empty(n), np.empty(n), np.empty(n)
+    lib().or_traj_filters(n, m, _p(mu), _p(S), _p(a), _p(b), _p(e))
+    return a, b, e
 
 
 def batch_dop853_event(mu, y0, tmax, ev_index=1, ev_offset=0.0, direction=-1, rtol=1e-12,
