@@ -166,121 +166,157 @@ class _Engine:
         return phi
 
 
-def correct_batch(problems: Sequence[CorrectionProblem], engine: _Engine | None = None,
-                  compute_half_period: bool = True) -> list[CorrectionResult]:
-    """Newton-Armijo for every problem; each phase is one batched launch."""
-    eng = engine or _Engine()
-    P = list(problems)
-    n = len(P)
-    mu = np.array([p.mu for p in P])
-    fwd = {p.forward for p in P}
-    if len(fwd) != 1:
-        raise ValueError("all problems of a batch must share `forward`")
-    forward = fwd.pop()
-    base = [np.asarray(p.x0, float).copy() for p in P]
-    ctrl = [list(p.control_indices) for p in P]
-    res = [list(p.residual_indices) for p in P]
-    tgt = [np.asarray(p.target, float) for p in P]
-    x = [base[i][ctrl[i]].copy() for i in range(n)]
-    k = np.zeros(n, int)
-    results: list[Optional[CorrectionResult]] = [None] * n
-    counts = np.zeros((n, 2), int)
+class _Newton:
+    """One correction as a resumable state machine.
 
-    def full(i, params):
-        v = base[i].copy()
-        v[ctrl[i]] = params
+    The stages follow _NewtonBackend.run: residual at x (an event search) ->
+    Jacobian (the STM to the event time) -> Armijo line search -> residual at
+    the new x.  The line search is evaluated speculatively: the residuals of
+    every trial step alpha = 1, r, r^2, ... >= min_alpha are requested in one
+    batch, then armijo.py:111-164 is replayed over them in order, so the
+    accepted step, the fallback to the best point and the failure are the
+    sequential ones.  The residual of the accepted trial is the residual at
+    the next iterate (the same state through the same deterministic search)
+    and is reused.  ``n_event_searches`` counts the searches the sequential
+    algorithm makes.
+    """
+
+    def __init__(self, p: CorrectionProblem):
+        self.p = p
+        self.base = np.asarray(p.x0, float).copy()
+        self.ctrl = list(p.control_indices)
+        self.res = list(p.residual_indices)
+        self.tgt = np.asarray(p.target, float)
+        self.x = self.base[self.ctrl].copy()
+        self.k = 0
+        self.n_ev = 0
+        self.n_stm = 0
+        self.result: Optional[CorrectionResult] = None
+        self.kind = "event"                 # what the machine waits for: event / stm / None
+        self.inputs = [self.full(self.x)]
+        self.stage = "residual"
+        self.alphas = []
+        alpha = 1.0
+        while alpha >= p.min_alpha:
+            self.alphas.append(alpha)
+            alpha = alpha * p.alpha_reduction
+
+    def full(self, params) -> np.ndarray:
+        v = self.base.copy()
+        v[self.ctrl] = params
         return v
 
-    def residuals(idx, params_list):
-        states = np.stack([full(i, q) for i, q in zip(idx, params_list)])
-        t, xe, ok = eng.event_batch(mu[idx], states, forward)
-        for i in idx:
-            counts[i, 0] += 1
-        r = [xe[j][res[i]] - tgt[i] if ok[j] else None for j, i in enumerate(idx)]
-        return r, t, xe, ok
+    def _finish(self, ok: bool, rn: float, err: str = "", t: float = math.nan) -> None:
+        self.result = CorrectionResult(ok, self.full(self.x), rn, int(self.k), half_period=t,
+                                       error=err, n_event_searches=self.n_ev, n_stm=self.n_stm)
+        self.kind = None
 
-    active = list(range(n))
-    while active:
-        r, t_ev, x_ev, ok = residuals(active, [x[i] for i in active])
-        still, need_jac = [], []
-        for j, i in enumerate(active):
-            p = P[i]
-            if not ok[j]:
-                results[i] = CorrectionResult(False, full(i, x[i]), math.nan, int(k[i]),
-                                              error="no section crossing for the current state")
-                continue
-            rn = _inf_norm(r[j])
-            if rn < p.tol:
-                results[i] = CorrectionResult(True, full(i, x[i]), rn, int(k[i]))
-                continue
-            if k[i] >= p.max_attempts:
-                results[i] = CorrectionResult(False, full(i, x[i]), rn, int(k[i]),
-                                              error=f"Newton did not converge after {p.max_attempts} "
-                                                    f"iterations (|R|={rn:.2e}).")
-                continue
-            need_jac.append((i, r[j], rn, t_ev[j], x_ev[j]))
-        if not need_jac:
-            break
-        idx = [e[0] for e in need_jac]
-        phis = eng.stm_batch(mu[idx], np.stack([full(i, x[i]) for i in idx]),
-                             np.array([e[3] for e in need_jac]))
-        for i in idx:
-            counts[i, 1] += 1
-        deltas = {}
-        for (i, r_i, rn, te, xe), Phi in zip(need_jac, phis):
-            J = Phi[np.ix_(res[i], ctrl[i])]
-            if P[i].extra_jacobian == "halo":
-                J = J - halo_quadratic_term(P[i].mu, xe, Phi)
-            d = solve_delta_dense(J, r_i)
-            if P[i].max_delta is not None and not np.isinf(P[i].max_delta):
+    def _residual(self, t, xe, ok) -> None:
+        """Top of the Newton loop with the residual at self.x."""
+        p = self.p
+        self.n_ev += 1
+        if not ok:
+            self._finish(False, math.nan, "no section crossing for the current state")
+            return
+        r = xe[self.res] - self.tgt
+        rn = _inf_norm(r)
+        if rn < p.tol:
+            self._finish(True, rn, t=float(t))
+            return
+        if self.k >= p.max_attempts:
+            self._finish(False, rn, f"Newton did not converge after {p.max_attempts} "
+                                    f"iterations (|R|={rn:.2e}).")
+            return
+        self.r, self.rn, self.t_ev, self.x_ev = r, rn, float(t), xe
+        self.stage, self.kind = "jacobian", "stm"
+        self.inputs = [(self.full(self.x), self.t_ev)]
+
+    def feed(self, out) -> None:
+        """Results of the requests in self.inputs: a list of (t, x_event, ok)
+        for event requests, a list with one 6x6 Phi for an STM request."""
+        p = self.p
+        if self.stage == "residual":
+            self._residual(*out[0])
+        elif self.stage == "jacobian":
+            Phi = out[0]
+            self.n_stm += 1
+            J = Phi[np.ix_(self.res, self.ctrl)]
+            if p.extra_jacobian == "halo":
+                J = J - halo_quadratic_term(p.mu, self.x_ev, Phi)
+            d = solve_delta_dense(J, self.r)
+            if p.max_delta is not None and not np.isinf(p.max_delta):
                 dn = np.linalg.norm(d, ord=np.inf)
-                if dn > P[i].max_delta:
-                    d = d * (P[i].max_delta / dn)
-            deltas[i] = (d, rn)
-        # Armijo backtracking (armijo.py:111-164), all searching orbits per launch
-        alpha = {i: 1.0 for i in idx}
-        best = {i: (x[i], deltas[i][1], 0.0) for i in idx}
-        searching = list(idx)
-        while searching:
-            trial = [x[i] + alpha[i] * deltas[i][0] for i in searching]
-            rt, _, _, okt = residuals(searching, trial)
-            nxt = []
-            for j, i in enumerate(searching):
-                cur = deltas[i][1]
-                a = alpha[i]
-                if okt[j]:
-                    nt = _inf_norm(rt[j])
-                    if nt <= (1.0 - P[i].armijo_c * a) * cur:
-                        x[i] = trial[j]
-                        k[i] += 1
-                        continue
-                    if nt < best[i][1]:
-                        best[i] = (trial[j], nt, a)
-                alpha[i] = a * P[i].alpha_reduction
-                if alpha[i] >= P[i].min_alpha:
-                    nxt.append(i)
-                elif best[i][2] > 0:
-                    x[i] = best[i][0]
-                    k[i] += 1
-                else:
-                    results[i] = CorrectionResult(
-                        False, full(i, x[i]), cur, int(k[i]),
-                        error="Armijo line search failed to find a productive step.")
-            searching = nxt
-        active = [i for i in idx if results[i] is None]
+                if dn > p.max_delta:
+                    d = d * (p.max_delta / dn)
+            self.trials = [self.x + a * d for a in self.alphas]
+            self.stage, self.kind = "armijo", "event"
+            self.inputs = [self.full(q) for q in self.trials]
+        elif self.stage == "armijo":
+            best, chosen = None, None
+            best_norm = self.rn
+            for j, a in enumerate(self.alphas):
+                self.n_ev += 1
+                t, xe, ok = out[j]
+                if not ok:
+                    continue
+                nt = _inf_norm(xe[self.res] - self.tgt)
+                if nt <= (1.0 - p.armijo_c * a) * self.rn:
+                    chosen = j
+                    break
+                if nt < best_norm:
+                    best, best_norm = j, nt
+            if chosen is None:
+                chosen = best
+            if chosen is None:
+                self._finish(False, self.rn, "Armijo line search failed to find a productive step.")
+                return
+            self.x = self.trials[chosen]
+            self.k += 1
+            self.stage = "residual"
+            self._residual(*out[chosen])
 
-    out = []
-    for i in range(n):
-        r_ = results[i]
-        r_.n_event_searches, r_.n_stm = int(counts[i, 0]), int(counts[i, 1])
-        out.append(r_)
-    if compute_half_period:
-        conv = [i for i in range(n) if out[i].converged]
-        if conv:
-            t, _, ok = eng.event_batch(mu[conv], np.stack([out[i].x_corrected for i in conv]),
-                                       forward)
-            for j, i in enumerate(conv):
-                out[i].half_period = float(t[j]) if ok[j] else math.nan
+
+def _drive(machines, eng: "_Engine") -> None:
+    """Run state machines to completion, one batched launch per request kind
+    per round."""
+    active = [m for m in machines if m.kind is not None]
+    while active:
+        ev = [m for m in active if m.kind == "event"]
+        st = [m for m in active if m.kind == "stm"]
+        if ev:
+            fwd = {m.p.forward for m in ev}
+            for f in fwd:
+                grp = [m for m in ev if m.p.forward == f]
+                mu = np.concatenate([np.full(len(m.inputs), m.p.mu) for m in grp])
+                states = np.stack([s for m in grp for s in m.inputs])
+                t, xe, ok = eng.event_batch(mu, states, f)
+                o = 0
+                for m in grp:
+                    k = len(m.inputs)
+                    m.feed([(t[o + j], xe[o + j], ok[o + j]) for j in range(k)])
+                    o += k
+        if st:
+            mu = np.array([m.p.mu for m in st])
+            states = np.stack([m.inputs[0][0] for m in st])
+            tev = np.array([m.inputs[0][1] for m in st])
+            phis = eng.stm_batch(mu, states, tev)
+            for m, phi in zip(st, phis):
+                m.feed([phi])
+        active = [m for m in active if m.kind is not None]
+
+
+def correct_batch(problems: Sequence[CorrectionProblem], engine: _Engine | None = None,
+                  compute_half_period: bool = True) -> list[CorrectionResult]:
+    """Newton-Armijo for every problem; each round of the corrections'
+    event searches (residuals, all line-search trials) and each round of
+    their STMs is one batched launch."""
+    eng = engine or _Engine()
+    machines = [_Newton(p) for p in problems]
+    _drive(machines, eng)
+    out = [m.result for m in machines]
+    if not compute_half_period:
+        for r in out:
+            r.half_period = math.nan
     return out
 
 

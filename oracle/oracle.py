@@ -456,3 +456,124 @@ class CorrectorEngine:
     def stm_batch(self, mu, states, t_event):
         return np.stack([compute_stm(float(m), s, te, steps=500)[2]
                          for m, s, te in zip(mu, states, t_event)])
+
+
+# ---------------------------------------------------------------------------
+# sequential Newton-Armijo (the reference's loop, literally)
+# ---------------------------------------------------------------------------
+
+def _halo_term(mu, X_ev, Phi):
+    """services/orbits.py:917-947."""
+    x, y, z, vx, vy, vz = X_ev
+    mu2 = 1 - mu
+    rho_1 = 1 / (((x + mu)**2 + y**2 + z**2)**1.5)
+    rho_2 = 1 / (((x - mu2)**2 + y**2 + z**2)**1.5)
+    omega_x = -(mu2 * (x + mu) * rho_1) - (mu * (x - mu2) * rho_2) + x
+    DDx = 2 * vy + omega_x
+    DDz = -(mu2 * z * rho_1) - (mu * z * rho_2)
+    if abs(vy) < 1e-9:
+        vy = np.sign(vy) * 1e-9 if vy != 0 else 1e-9
+    return np.array([[DDx], [DDz]]) @ Phi[[1], :][:, (0, 4)] / vy
+
+
+def _solve_delta(J, r, cond_threshold=1e8):
+    """corrector/backends/base.py:187-245."""
+    try:
+        cond_J = np.linalg.cond(J)
+    except np.linalg.LinAlgError:
+        cond_J = np.inf
+    if J.shape[0] == J.shape[1]:
+        J_reg = J + np.eye(J.shape[0]) * 1e-12 if (np.isnan(cond_J) or cond_J > cond_threshold) else J
+        try:
+            return np.linalg.solve(J_reg, -r)
+        except np.linalg.LinAlgError:
+            return np.linalg.lstsq(J_reg, -r, rcond=None)[0]
+    lam = 1e-12 if (np.isnan(cond_J) or cond_J > cond_threshold) else 0.0
+    JTJ = J.T @ J + lam * np.eye(J.shape[1])
+    try:
+        return np.linalg.solve(JTJ, -(J.T @ r))
+    except np.linalg.LinAlgError:
+        return np.linalg.lstsq(J, -r, rcond=None)[0]
+
+
+def correct_reference(p, engine=None):
+    """One correction by the reference's sequential loop: _NewtonBackend.run
+    (corrector/backends/newton.py:39-143) with _ArmijoLineSearch
+    (stepping/armijo.py:58-164), residual/Jacobian operators
+    (corrector/operators.py:319-452) on the oracle's event search and STM.
+    ``p`` has the attributes of a correction problem (mu, x0,
+    residual_indices, control_indices, target, extra_jacobian, max_attempts,
+    tol, max_delta, alpha_reduction, min_alpha, armijo_c, forward).
+    Returns a dict: converged, x, iterations, residual_norm, error,
+    n_event_searches, n_stm, half_period."""
+    eng = engine or CorrectorEngine()
+    mu = np.array([p.mu])
+    base = np.asarray(p.x0, float).copy()
+    ctrl, res = list(p.control_indices), list(p.residual_indices)
+    tgt = np.asarray(p.target, float)
+    counts = {"ev": 0, "stm": 0}
+
+    def full(q):
+        v = base.copy()
+        v[ctrl] = q
+        return v
+
+    class _NoCross(Exception):
+        pass
+
+    def residual(q):
+        counts["ev"] += 1
+        t, xe, ok = eng.event_batch(mu, full(q)[None], p.forward)
+        if not ok[0]:
+            raise _NoCross()
+        return xe[0][res] - tgt, t[0], xe[0]
+
+    def out(conv, x, rn, k, err="", t=np.nan):
+        return dict(converged=conv, x=full(x), residual_norm=rn, iterations=k, error=err,
+                    n_event_searches=counts["ev"], n_stm=counts["stm"], half_period=t)
+
+    norm = lambda r: float(np.linalg.norm(r, ord=np.inf))
+    x = base[ctrl].copy()
+    for k in range(p.max_attempts + 1):
+        try:
+            r, t_ev, x_ev = residual(x)
+        except _NoCross:
+            return out(False, x, np.nan, k, "no section crossing for the current state")
+        rn = norm(r)
+        if rn < p.tol:
+            return out(True, x, rn, k, t=t_ev)
+        if k == p.max_attempts:
+            return out(False, x, rn, k, f"Newton did not converge after {p.max_attempts} "
+                                        f"iterations (|R|={rn:.2e}).")
+        counts["stm"] += 1
+        Phi = eng.stm_batch(mu, full(x)[None], np.array([t_ev]))[0]
+        J = Phi[np.ix_(res, ctrl)]
+        if p.extra_jacobian == "halo":
+            J = J - _halo_term(p.mu, x_ev, Phi)
+        delta = _solve_delta(J, r)
+        if p.max_delta is not None and not np.isinf(p.max_delta):
+            dn = np.linalg.norm(delta, ord=np.inf)
+            if dn > p.max_delta:
+                delta = delta * (p.max_delta / dn)
+        alpha, best_x, best_norm, best_alpha = 1.0, x, rn, 0.0
+        x_new = None
+        while alpha >= p.min_alpha:
+            x_trial = x + alpha * delta
+            try:
+                r_trial, _, _ = residual(x_trial)
+            except _NoCross:
+                alpha *= p.alpha_reduction
+                continue
+            nt = norm(r_trial)
+            if nt <= (1.0 - p.armijo_c * alpha) * rn:
+                x_new = x_trial
+                break
+            if nt < best_norm:
+                best_x, best_norm, best_alpha = x_trial, nt, alpha
+            alpha *= p.alpha_reduction
+        if x_new is None:
+            if best_alpha > 0:
+                x_new = best_x
+            else:
+                return out(False, x, rn, k, "Armijo line search failed to find a productive step.")
+        x = x_new

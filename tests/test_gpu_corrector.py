@@ -42,6 +42,11 @@ def test_corrector_matches_oracle_operators(gpu, oracle):
         assert g.error == r.error
     assert [g.converged for g in gpu_res] == [True, True, False, True, False]
     assert "Armijo" in gpu_res[4].error
+    # and the reference's sequential loop on the oracle operators
+    for p, g in zip(problems(), gpu_res):
+        w = oracle.correct_reference(p)
+        assert np.array_equal(g.x_corrected, w["x"]) and g.iterations == w["iterations"]
+        assert g.n_event_searches == w["n_event_searches"] and g.error == w["error"]
 
 
 def test_corrected_orbits_are_periodic(gpu):
