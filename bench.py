@@ -77,6 +77,23 @@ def cpu_baseline(w, n_threads: int, target_s: float = 12.0):
                       f"{steps} accepted steps in {spent:.2f} s"}
 
 
+def serving_run(seconds: float, rate: float = 2000.0):
+    """The serving stream: Poisson arrivals at ``rate``, 70% corrections and
+    30% manifold fans of 32-2048 trajectories, through hiten_amd.serving."""
+    from hiten_amd import serving as S, workloads as W
+    reqs = W.serving_stream(duration=seconds, rate=rate)
+    server = S.Server()
+    # warm the kernels and the pinned-memory pool on a few requests first
+    server.run(reqs[:20], time_limit=60.0)
+    t = time.perf_counter()
+    jobs = server.run(reqs, time_limit=600.0)
+    wall = time.perf_counter() - t
+    out = S.summarize(jobs, wall, server.steps)
+    out.update(requests=len(reqs), rate_per_s=rate, seconds_of_arrivals=seconds,
+               max_queue=server.max_queue)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +106,7 @@ def main():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--heavy", type=int, default=1000)
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -179,6 +197,8 @@ def main():
             "bitwise_equal_to_persistent": same,
         },
     }
+    if args.serve_seconds > 0:
+        out["serving"] = serving_run(args.serve_seconds)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(w, threads)

@@ -21,6 +21,7 @@ from .build import LIB
 HH_OK = 0
 HH_VARIANT_PLAIN = 0
 HH_VARIANT_PERSISTENT = 1
+HH_VARIANT_WIDE = 2
 HH_STATUS_OK, HH_STATUS_MAXSTEPS, HH_STATUS_NONFINITE = 0, 1, 2
 ABI_VERSION = 1
 

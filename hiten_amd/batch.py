@@ -184,7 +184,8 @@ def launch_final(b: FinalBatch, *, rtol: float = 1e-12, atol: float = 1e-12,
                  variant: str = "plain", block: int = 0, grid: int = 0, stream=None) -> None:
     """Enqueue hh_dop853_final on the stream (asynchronous)."""
     lib = _lib.load()
-    var = {"plain": _lib.HH_VARIANT_PLAIN, "persistent": _lib.HH_VARIANT_PERSISTENT}[variant]
+    var = {"plain": _lib.HH_VARIANT_PLAIN, "persistent": _lib.HH_VARIANT_PERSISTENT,
+           "wide": _lib.HH_VARIANT_WIDE}[variant]
     rc = lib.hh_dop853_final(b.n, b.dim, _ptr(b.mu), _ptr(b.t0), _ptr(b.tf), float(rtol),
                              float(atol), float(max_step), float(min_step), int(max_steps),
                              int(forward), flip_mask(b.dim, flip_indices), _ptr(b.y0), _ptr(b.y),

@@ -541,38 +541,13 @@ void k_dop853_final42_plain(Final42Args a) {
     }
 }
 
-/* Finish the trajectory of the group whose lane 0 is `src` with the
- * wave-wide layout (Wave42): lane 8*b + d takes entry d of block b from lane
- * src + b, the loop below is traj_iterate with FinalOut, and the results are
- * stored as store42 does.  Raises the wave's issue priority meanwhile: such
- * a trajectory is the longest-running work on its SIMD.                     */
-__device__ __forceinline__ void run_wide(const Final42Args& a, int lane, int src, int traj_lane,
-                                         const Lane42& p, const Traj<Lane42>& s) {
-    Wave42 W;
-    W.setup(lane, rlane(p.mu, src), a.fwd, a.flip);
-    const int from = src + (lane >> 3);
-    double y = 0.0;
-    double k[DOP_K];
-#pragma unroll
-    for (int j = 0; j < DOP_K; ++j) k[j] = 0.0;
-#pragma unroll
-    for (int dd = 0; dd < 6; ++dd) {
-        const double vy = __shfl(s.y[dd], from);
-        const double vk = __shfl(s.k[0][dd], from);
-        if (W.act && W.d == dd) {
-            y = vy;
-            k[0] = vk;
-        }
-    }
-    double t = rlane(s.t, src);
-    const double tf = rlane(s.tf, src);
-    double h = rlane(s.h, src);
-    double err_prev = rlane(s.err_prev, src);
-    int nacc = __builtin_amdgcn_readlane(s.nacc, src);
-    int nrej = __builtin_amdgcn_readlane(s.nrej, src);
-    const int traj = __builtin_amdgcn_readlane(traj_lane, src);
+/* traj_iterate + FinalOut for one trajectory in the wave-wide layout, from
+ * the current (y, k[0], t, h, err_prev, counters) to its end; stores the
+ * results as store42 does.                                                  */
+__device__ __forceinline__ void wide_finish(const Final42Args& a, const Wave42& W, int traj,
+                                            double y, double (&k)[DOP_K], double t, double tf,
+                                            double h, double err_prev, int nacc, int nrej) {
     const StepParams& sp = a.sp;
-    __builtin_amdgcn_s_setprio(3);
     double out = y;
     int status = ST_OK;
     for (;;) {
@@ -611,13 +586,80 @@ __device__ __forceinline__ void run_wide(const Final42Args& a, int lane, int src
             nrej += 1;
         }
     }
-    if (W.act) a.yout[(size_t)comp42(lane >> 3, W.d) * a.n + traj] = out;
-    if (lane == 0) {
+    if (W.act) a.yout[(size_t)comp42(W.lane >> 3, W.d) * a.n + traj] = out;
+    if (W.lane == 0) {
         if (a.status) a.status[traj] = status;
         if (a.nacc) a.nacc[traj] = nacc;
         if (a.nrej) a.nrej[traj] = nrej;
     }
+}
+
+/* Finish the trajectory of the group whose lane 0 is `src` with the
+ * wave-wide layout: lane 8*b + d takes entry d of block b from lane src + b.
+ * Raises the wave's issue priority meanwhile: such a trajectory is the
+ * longest-running work on its SIMD.                                         */
+__device__ __forceinline__ void run_wide(const Final42Args& a, int lane, int src, int traj_lane,
+                                         const Lane42& p, const Traj<Lane42>& s) {
+    Wave42 W;
+    W.setup(lane, rlane(p.mu, src), a.fwd, a.flip);
+    const int from = src + (lane >> 3);
+    double y = 0.0;
+    double k[DOP_K];
+#pragma unroll
+    for (int j = 0; j < DOP_K; ++j) k[j] = 0.0;
+#pragma unroll
+    for (int dd = 0; dd < 6; ++dd) {
+        const double vy = __shfl(s.y[dd], from);
+        const double vk = __shfl(s.k[0][dd], from);
+        if (W.act && W.d == dd) {
+            y = vy;
+            k[0] = vk;
+        }
+    }
+    __builtin_amdgcn_s_setprio(3);
+    wide_finish(a, W, __builtin_amdgcn_readlane(traj_lane, src), y, k, rlane(s.t, src),
+                rlane(s.tf, src), rlane(s.h, src), rlane(s.err_prev, src),
+                __builtin_amdgcn_readlane(s.nacc, src), __builtin_amdgcn_readlane(s.nrej, src));
     __builtin_amdgcn_s_setprio(0);
+}
+
+/* One trajectory per wave from the start (latency form for small batches):
+ * traj_init (rk.py:2441-2448) in the wave-wide layout, then wide_finish.    */
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_dop853_final42_wide(Final42Args a) {
+    const int traj = (int)((blockIdx.x * (unsigned)BLOCK + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (traj >= a.n) return;
+    Wave42 W;
+    W.setup(lane, a.mu[traj], a.fwd, a.flip);
+    const double y0 = W.act ? a.y0[(size_t)comp42(lane >> 3, W.d) * a.n + traj] : 0.0;
+    const double t0 = a.t0[traj], tf = a.tf[traj];
+    if (isclose(t0, tf) || !((t0 - tf) * 1.0 < 0.0)) {
+        /* zero or reversed span: y0 unchanged (dynamics/base.py:421-424) */
+        if (W.act) a.yout[(size_t)comp42(lane >> 3, W.d) * a.n + traj] = y0;
+        if (lane == 0) {
+            if (a.status) a.status[traj] = ST_OK;
+            if (a.nacc) a.nacc[traj] = 0;
+            if (a.nrej) a.nrej[traj] = 0;
+        }
+        return;
+    }
+    double k[DOP_K];
+#pragma unroll
+    for (int j = 0; j < DOP_K; ++j) k[j] = 0.0;
+    k[0] = W.rhs(y0);
+    /* initial_step with the block-tree sums of Lane42 */
+    const StepParams& sp = a.sp;
+    const double sc = sp.atol + sp.rtol * __builtin_fabs(y0);
+    const double ua = y0 / sc;
+    const double ub = k[0] / sc;
+    const double s0 = W.block_tree(W.act ? ua * ua : 0.0);
+    const double s1 = W.block_tree(W.act ? ub * ub : 0.0);
+    const double sn = __builtin_sqrt(42.0);
+    const double d0 = __builtin_sqrt(s0) / sn;
+    const double d1 = __builtin_sqrt(s1) / sn;
+    const double h = select_initial_step(d0, d1, sp.min_step, sp.max_step);
+    wide_finish(a, W, traj, y0, k, t0, tf, h, -1.0, 0, 0);
 }
 
 /* Work-stealing form.  Each wave keeps its 8 groups busy: after every step
@@ -912,6 +954,16 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
             default: hipLaunchKernelGGL((k_dop853_final42_plain<512, false>), dim3(g), dim3(512), 0, st, a); break;
         }
         return check_launch("hh_dop853_final(42, plain) launch");
+    }
+    if (variant == HH_VARIANT_WIDE) {
+        const int g = blocks_for((long long)n * 64, block);
+        switch (block) {
+            case 64: hipLaunchKernelGGL(k_dop853_final42_wide<64>, dim3(g), dim3(64), 0, st, a); break;
+            case 128: hipLaunchKernelGGL(k_dop853_final42_wide<128>, dim3(g), dim3(128), 0, st, a); break;
+            case 256: hipLaunchKernelGGL(k_dop853_final42_wide<256>, dim3(g), dim3(256), 0, st, a); break;
+            default: hipLaunchKernelGGL(k_dop853_final42_wide<512>, dim3(g), dim3(512), 0, st, a); break;
+        }
+        return check_launch("hh_dop853_final(42, wide) launch");
     }
     if (variant != HH_VARIANT_PERSISTENT) return fail(HH_ERR_ARGS, "hh_dop853_final: unknown variant");
     if (!work_counter) return fail(HH_ERR_ARGS, "hh_dop853_final: persistent variant needs work_counter");

@@ -217,6 +217,20 @@ def _event_window(mu, states, t_start, span, surface, forward, align_steps_kw=No
     return ok, t_start + t_rel, y_out
 
 
+def _default_windows(t_guess=None, t0_offset: float = 0.15):
+    """(start, span) of the first search window of _SingleHitBackend._cross
+    (backend.py:244-256) and of its fallback window (269-279)."""
+    t_start = float(t_guess) if t_guess is not None else float(np.pi / 2.0 - t0_offset)
+    t_start = max(t_start, 0.0)
+    half = float(np.pi) * 0.5
+    t0 = float(max(t_start - half, 0.0))
+    tmax = t0 + 2.0 * half
+    ts = t0 if t0 > 0.0 else 1e-12
+    tsf = max(float(np.pi / 2.0 - t0_offset), 0.0)
+    ts2 = tsf if tsf > 0.0 else 1e-12
+    return (ts, max(0.0, tmax - ts)), (ts2, max(0.0, tsf + float(np.pi) - ts2))
+
+
 def section_crossing_batch(dynsys, states, *, surface: _PlaneEvent | None = None, t_guess=None,
                            forward: int = 1, t0_offset: float = 0.15):
     """_SingleHitBackend._cross for many seeds without a time window: the first
@@ -227,13 +241,8 @@ def section_crossing_batch(dynsys, states, *, surface: _PlaneEvent | None = None
     mu = _system_mu(dynsys)
     states = np.atleast_2d(np.asarray(states, dtype=float))
     n = states.shape[0]
-    t_start = float(t_guess) if t_guess is not None else float(np.pi / 2.0 - t0_offset)
-    t_start = max(t_start, 0.0)
-    half = float(np.pi) * 0.5
-    t0 = float(max(t_start - half, 0.0))
-    tmax = t0 + 2.0 * half
-    ts = t0 if t0 > 0.0 else 1e-12
-    ok, t_hit, y_hit = _event_window(mu, states, ts, max(0.0, tmax - ts), surface, forward)
+    (ts, span), (ts2, span2) = _default_windows(t_guess, t0_offset)
+    ok, t_hit, y_hit = _event_window(mu, states, ts, span, surface, forward)
     hits: list = [None] * n
     for i in np.nonzero(ok)[0]:
         hits[i] = _SectionHit(float(t_hit[i]), y_hit[i].copy(), y_hit[i, :2].copy(), int(i))
@@ -250,11 +259,7 @@ def section_crossing_batch(dynsys, states, *, surface: _PlaneEvent | None = None
             hits[i] = h
         return hits
     if miss.size:
-        tsf = float(np.pi / 2.0 - t0_offset)
-        tsf = max(tsf, 0.0)
-        ts2 = tsf if tsf > 0.0 else 1e-12
-        ok2, t2, y2 = _event_window(mu, states[miss], ts2, max(0.0, tsf + float(np.pi) - ts2),
-                                    surface, forward)
+        ok2, t2, y2 = _event_window(mu, states[miss], ts2, span2, surface, forward)
         for j, i in enumerate(miss):
             if ok2[j]:
                 hits[i] = _SectionHit(float(t2[j]), y2[j].copy(), y2[j, :2].copy(), int(i))

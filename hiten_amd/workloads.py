@@ -116,3 +116,42 @@ def persistent_262k(n: int = 262144, seed: int = SEED + 3):
         x[sel, 4] = base[sel, 4] * s
     tf = np.exp(rng.uniform(math.log(0.5), math.log(20.0), n))
     return dict(mu=mu, y0=with_identity_stm(x), t0=np.zeros(n), tf=tf, rtol=1e-12, atol=1e-12)
+
+
+def serving_stream(duration: float = 1.0, rate: float = 2000.0, seed: int = SEED + 4,
+                   correction_fraction: float = 0.7, fan_min: int = 32, fan_max: int = 2048,
+                   eps: float = 1e-6):
+    """Poisson request stream for the serving loop.  Draw order per request:
+    inter-arrival time ~ Exp(rate), kind (correction with probability
+    ``correction_fraction``), then
+      correction: x ~ U(0.80, 0.88), z ~ U(-0.2, 0.2), vy ~ U(-0.3, 0.3)
+                  (a Halo-style single-shooting problem, <= 10 iterations);
+      fan:        size ~ log-uniform [fan_min, fan_max], one planar seed, then
+                  size/2 Gaussian 6-directions (+ and - each, eps offset) and
+                  t_max ~ U(2, 8) per trajectory (stop at y = 0 with vy < 0).
+    Returns a list of dicts with an arrival time ``t`` in seconds."""
+    rng = np.random.default_rng(seed)
+    out = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            break
+        if rng.uniform() < correction_fraction:
+            x = rng.uniform(0.80, 0.88)
+            z = rng.uniform(-0.2, 0.2)
+            vy = rng.uniform(-0.3, 0.3)
+            out.append(dict(t=t, kind="correction", mu=MU_EM,
+                            x0=np.array([x, 0.0, z, 0.0, vy, 0.0])))
+        else:
+            size = int(round(math.exp(rng.uniform(math.log(fan_min), math.log(fan_max)))))
+            size += size % 2
+            s0 = planar_seeds(rng, 1)[0]
+            g = rng.normal(size=(size // 2, 6))
+            u = g / np.linalg.norm(g, axis=1, keepdims=True)
+            dirs = np.empty((size, 6))
+            dirs[0::2] = u
+            dirs[1::2] = -u
+            out.append(dict(t=t, kind="fan", mu=MU_EM, y0=s0[None, :] + eps * dirs,
+                            tmax=rng.uniform(2.0, 8.0, size)))
+    return out

@@ -37,6 +37,7 @@ extern "C" {
 
 #define HH_VARIANT_PLAIN 0
 #define HH_VARIANT_PERSISTENT 1
+#define HH_VARIANT_WIDE 2
 
 /* Message describing the last error returned on this thread. */
 const char* hh_last_error(void);
@@ -77,10 +78,15 @@ int hh_rk8_fixed(int n, int dim, const double* mu, const double* t0, const doubl
  *   dim        : 6 or 42      mu, t0, tf : [n]   y0, y_out : [dim][n]
  *   status, n_acc, n_rej : [n] int, each may be NULL
  *   variant    : HH_VARIANT_PLAIN (one launch, one trajectory per lane or
- *                8-lane group) or HH_VARIANT_PERSISTENT (dim 42: work-
- *                stealing groups refilled from *work_counter)
+ *                8-lane group), HH_VARIANT_PERSISTENT (dim 42: work-
+ *                stealing groups refilled from *work_counter; a trajectory
+ *                left alone in its wave, or past the heavy threshold of
+ *                hh_set_tuning, finishes one component per lane) or
+ *                HH_VARIANT_WIDE (dim 42: one trajectory per wave, one
+ *                component per lane, from the start: the latency form for
+ *                batches smaller than the GPU's wave slots)
  *   block      : threads per block (dim 6: 64..256; dim 42: 64/128/256/512)
- *   grid       : persistent blocks (0 = 4 waves per CU)
+ *   grid       : persistent blocks (0 = as many as fit at once)
  *   work_counter : one device int, required for the persistent variant;
  *                reset by the library on the stream before the launch
  *   order      : [n] permutation of 0..n-1 giving the order in which the

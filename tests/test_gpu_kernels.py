@@ -117,7 +117,7 @@ def test_dop853_final_6d(gpu, oracle):
     assert_bitwise(got, want, "dop853 6-d final", rtol=1e-10)
 
 
-@pytest.mark.parametrize("variant", ["plain", "persistent"])
+@pytest.mark.parametrize("variant", ["plain", "persistent", "wide"])
 def test_dop853_final_42d(gpu, oracle, variant):
     say(f"hh_dop853_final dim=42 {variant}: 256 trajectories")
     w = W.stm_uneven(n=256)
@@ -192,19 +192,23 @@ def test_dop853_42d_wave_wide_forward(gpu, oracle, heavy_threshold, heavy):
     assert_bitwise(r.y.cpu().numpy(), want, f"42-d final, heavy={heavy}", rtol=1e-10)
 
 
-def test_dop853_42d_wave_wide_backward_flip(gpu, oracle, heavy_threshold):
+@pytest.mark.parametrize("variant", ["persistent", "wide"])
+def test_dop853_42d_wave_wide_backward_flip(gpu, oracle, heavy_threshold, variant):
     """Backward _DirectedSystem with the STM-part flip (compute_stm's
     slice(36, 42)) through the one-component-per-lane form."""
-    say("hh_dop853_final dim=42 persistent backward flip, heavy=2: 24 trajectories")
+    say(f"hh_dop853_final dim=42 {variant} backward flip, heavy=2: 24 trajectories")
     heavy_threshold(2)
     w = W.stm_uneven(n=24)
     flip = slice(36, 42)
-    r = batch.dop853_final(w["mu"], w["y0"], 0.0, w["tf"], dim=42, max_step=1e4, forward=-1,
-                           flip_indices=flip, variant="persistent")
+    tf = w["tf"].copy()
+    tf[5] = 0.0                        # zero span: y0 returned unchanged
+    r = batch.dop853_final(w["mu"], w["y0"], 0.0, tf, dim=42, max_step=1e4, forward=-1,
+                           flip_indices=flip, variant=variant)
+    assert np.array_equal(r.y[5].cpu().numpy(), w["y0"][5])
     got = r.y.cpu().numpy()
     for i in range(0, 24, 3):
         sysm = oracle.System("cr3bp42", mu=w["mu"], fwd=-1, flip=flip)
-        want, info = oracle.dop853(sysm, w["y0"][i], np.array([0.0, w["tf"][i]]), rtol=1e-12,
+        want, info = oracle.dop853(sysm, w["y0"][i], np.array([0.0, tf[i]]), rtol=1e-12,
                                    atol=1e-12, max_step=1e4)
         assert int(r.n_accepted[i]) == info["n_accepted"]
         assert_bitwise(got[i], want[-1], f"backward traj {i}", rtol=1e-10)

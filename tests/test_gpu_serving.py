@@ -1,0 +1,107 @@
+"""GPU tests of the continuous-batching serving loop.
+
+* Every job of a mixed stream (corrections + manifold fans arriving as a
+  Poisson process) gives the same bits as the same request run alone, both
+  through the loop and through the direct batched calls.
+* A sample of results against the oracle (the reference's sequential
+  Newton-Armijo loop; its event integration per fan trajectory).
+* HBM-budget admission: with a budget that holds only a few jobs the loop
+  queues the rest, never exceeds the budget, and still completes every job.
+"""
+
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from hiten_amd import batch, corrector as Cr, serving as S, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _stream():
+    # 60 ms of a 2k req/s stream, fans capped at 256 trajectories
+    return W.serving_stream(duration=0.06, rate=2000.0, fan_max=256)
+
+
+def _same_correction(a, b):
+    assert a.converged == b.converged and a.iterations == b.iterations
+    assert np.array_equal(a.x_corrected, b.x_corrected)
+    assert (a.residual_norm == b.residual_norm) or (math.isnan(a.residual_norm)
+                                                    and math.isnan(b.residual_norm))
+    assert a.error == b.error and a.n_event_searches == b.n_event_searches
+    assert (a.half_period == b.half_period) or (math.isnan(a.half_period)
+                                                and math.isnan(b.half_period))
+
+
+def _same_fan(a, b):
+    assert np.array_equal(a.hit, b.hit)
+    assert np.array_equal(a.t_hit, b.t_hit) and np.array_equal(a.y_hit, b.y_hit)
+
+
+@pytest.fixture(scope="module")
+def mixed_run(gpu):
+    reqs = _stream()
+    say(f"serving loop: {len(reqs)} requests over 60 ms")
+    server = S.Server()
+    jobs = server.run(reqs, time_limit=60.0)
+    assert all(not math.isnan(j.finished) for j in jobs)
+    return reqs, jobs, server
+
+
+def test_mixed_results_equal_direct_calls(mixed_run, oracle):
+    reqs, jobs, _ = mixed_run
+    say("serving results vs direct batched calls and the oracle")
+    corr = [j for j in jobs if j.kind == "correction"]
+    fans = [j for j in jobs if j.kind == "fan"]
+    assert corr and fans
+    direct = Cr.correct_batch([j.request for j in corr])
+    for j, d in zip(corr, direct):
+        _same_correction(j.result, d)
+    for j in fans:
+        r = j.request
+        ev = batch.dop853_event(r.mu, r.y0, 0.0, r.tmax, ev_index=1, direction=-1,
+                                xtol=r.xtol, gtol=r.gtol)
+        _same_fan(j.result, S.FanResult(ev.hit.cpu().numpy(), ev.t_hit.cpu().numpy(),
+                                        ev.y_hit.cpu().numpy(), ev.n_accepted.cpu().numpy()))
+    for j in corr[:4]:
+        w = oracle.correct_reference(j.request)
+        assert np.array_equal(j.result.x_corrected, w["x"])
+        assert j.result.iterations == w["iterations"] and j.result.error == w["error"]
+    for j in fans[:2]:
+        r = j.request
+        hit, th, yh, _, _ = oracle.batch_dop853_event(r.mu, r.y0, r.tmax, ev_index=1,
+                                                      direction=-1, xtol=r.xtol, gtol=r.gtol)
+        assert np.array_equal(j.result.hit.astype(bool), hit.astype(bool))
+        assert np.array_equal(j.result.t_hit, th) and np.array_equal(j.result.y_hit, yh)
+
+
+def test_alone_equals_mixed(mixed_run):
+    reqs, jobs, _ = mixed_run
+    pick = [i for i, r in enumerate(reqs) if r["kind"] == "correction"][:3] + \
+           [i for i, r in enumerate(reqs) if r["kind"] == "fan"][:2]
+    say(f"alone vs mixed: requests {pick}")
+    for i in pick:
+        alone = S.Server().run([dict(reqs[i], t=0.0)], time_limit=30.0)[0]
+        if alone.kind == "correction":
+            _same_correction(alone.result, jobs[i].result)
+        else:
+            _same_fan(alone.result, jobs[i].result)
+
+
+def test_hbm_budget_admission(gpu):
+    reqs = _stream()
+    largest = max(S.Server.fan_bytes(len(r["y0"])) for r in reqs if r["kind"] == "fan")
+    budget = largest + 4 * S.Server.CORRECTION_BYTES
+    say(f"admission with a {budget} B budget")
+    server = S.Server(hbm_budget=budget)
+    jobs = server.run(reqs, time_limit=60.0)
+    assert server.max_queue > 0
+    assert server.peak_used <= budget
+    assert all(not math.isnan(j.finished) and j.admitted >= j.arrival for j in jobs)
