@@ -45,9 +45,15 @@ def flip_mask(dim: int, flip_indices=None) -> int:
     return m
 
 
+_DEVICES: dict = {}
+
+
 def _device() -> torch.device:
-    _lib.load()
-    return torch.device("cuda", torch.cuda.current_device())
+    lib = _lib.load()
+    d = _DEVICES.get(id(lib))
+    if d is None:
+        d = _DEVICES[id(lib)] = torch.device("cuda", torch.cuda.current_device())
+    return d
 
 
 def _vec(x, n: int, name: str) -> torch.Tensor:
@@ -171,7 +177,7 @@ def prepare_final(mu, y0, t0, tf, dim: int = 6, order: str = "span") -> FinalBat
     perm = None
     if order == "span":
         perm = torch.argsort(torch.abs(tf_t - t0_t), descending=True, stable=True).to(torch.int32)
-    elif order != "index":
+    elif order not in ("index", None):
         raise ValueError(f"unknown order {order!r}")
     return FinalBatch(dim, n, _vec(mu, n, "mu"), t0_t, tf_t, y, torch.empty_like(y),
                       torch.empty(n, **i32), torch.empty(n, **i32), torch.empty(n, **i32),
@@ -203,7 +209,7 @@ def dop853_final(mu, y0, t0, tf, *, dim: int = 6, rtol: float = 1e-12, atol: flo
     """Adaptive DOP853 to tf; the state returned is the dense-output value at tf
     (the last row of _propagate_dynsys(..., steps=2)).
     """
-    b = prepare_final(mu, y0, t0, tf, dim)
+    b = prepare_final(mu, y0, t0, tf, dim, order="span" if variant == "persistent" else None)
     launch_final(b, rtol=rtol, atol=atol, max_step=max_step, min_step=min_step,
                  max_steps=max_steps, forward=forward, flip_indices=flip_indices, variant=variant,
                  block=block, grid=grid, stream=stream)

@@ -449,7 +449,9 @@ def correct_batch(problems: Sequence[CorrectionProblem], engine: _Engine | None 
     where = []
     for p in problems:
         key = (len(p.residual_indices), len(p.control_indices))
-        b = banks.setdefault(key, _Bank(*key))
+        b = banks.get(key)
+        if b is None:
+            b = banks[key] = _Bank(*key)
         where.append((b, b.add(p)))
     for b in banks.values():
         _drive(b, eng)

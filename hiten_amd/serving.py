@@ -32,7 +32,7 @@ from typing import List, Optional
 import numpy as np
 import torch
 
-from . import batch
+from . import _lib, batch
 from .corrector import CorrectionProblem, CorrectionResult, _Bank, halo_problem
 from .singlehit import _default_windows
 
@@ -96,6 +96,59 @@ def _up(a) -> torch.Tensor:
     return h.to(batch._device(), non_blocking=True)
 
 
+class _Packed:
+    """Inputs of one launch in one pinned upload and its outputs in one device
+    block per dtype: float64 rows ``fin`` are uploaded as a (rows, k) array;
+    ``fout`` float64 and ``iout`` int32 rows are allocated on the device and
+    copied back together once the launch is enqueued."""
+
+    def __init__(self, fin: np.ndarray, n_fout: int, n_iout: int):
+        self.k = fin.shape[1]
+        self.din = _up(fin)
+        dev = self.din.device
+        self.fout = torch.empty((n_fout, self.k), dtype=torch.float64, device=dev)
+        self.iout = torch.empty((n_iout, self.k), dtype=torch.int32, device=dev)
+
+    @staticmethod
+    def ptr(t: torch.Tensor) -> int:
+        return t.data_ptr()
+
+    def download(self, frows=None):
+        f = self.fout if frows is None else self.fout[frows]
+        self.hf = _pinned(f)
+        self.hi = _pinned(self.iout)
+
+
+def _launch_event(pk: _Packed, y_dev: torch.Tensor, stream, *, ev_index: int, ev_offset: float,
+                  direction: int, xtol: float, gtol: float, rtol: float, atol: float) -> None:
+    """hh_dop853_event on packed buffers: din rows [mu, t0, tmax, ...], y_dev
+    (6, k) SoA start states; fout rows [t_hit, y_hit (6), y_last (6)], iout
+    rows [hit, status, n_accepted]."""
+    lib = _lib.load()
+    P = _Packed.ptr
+    rc = lib.hh_dop853_event(pk.k, P(pk.din[0]), P(pk.din[1]), P(pk.din[2]), float(rtol),
+                             float(atol), math.inf, batch.MIN_STEP, batch.DEFAULT_MAX_STEPS,
+                             int(ev_index), float(ev_offset), int(direction), float(xtol),
+                             float(gtol), 1, 0, P(y_dev), P(pk.iout[0]), P(pk.fout[0]),
+                             P(pk.fout[1]), P(pk.fout[7]), P(pk.iout[1]), P(pk.iout[2]), None, 0,
+                             int(stream.cuda_stream))
+    _lib.check(rc, "hh_dop853_event")
+
+
+def _launch_final(pk: _Packed, dim: int, y_dev: torch.Tensor, y_out: torch.Tensor, stream, *,
+                  forward: int, flip_indices, variant: int, max_step: float,
+                  istatus: int, inacc: int) -> None:
+    """hh_dop853_final on packed buffers: din rows [mu, t0, tf, ...]."""
+    lib = _lib.load()
+    P = _Packed.ptr
+    rc = lib.hh_dop853_final(pk.k, dim, P(pk.din[0]), P(pk.din[1]), P(pk.din[2]), 1e-12, 1e-12,
+                             float(max_step), batch.MIN_STEP, batch.DEFAULT_MAX_STEPS,
+                             int(forward), batch.flip_mask(dim, flip_indices), P(y_dev), P(y_out),
+                             P(pk.iout[istatus]), P(pk.iout[inacc]), None, int(variant), 0, 0,
+                             None, None, int(stream.cuda_stream))
+    _lib.check(rc, "hh_dop853_final")
+
+
 class _Pending:
     """Device work on one stream whose host copies are complete once
     ``done`` has been recorded and has fired."""
@@ -113,19 +166,22 @@ class _FanLaunch(_Pending):
         super().__init__(stream)
         self.job = job
         r: FanRequest = job.request
+        k = r.size
+        fin = np.empty((9, k))
+        fin[0], fin[1], fin[2], fin[3:9] = r.mu, 0.0, r.tmax, r.y0.T
         with torch.cuda.stream(stream):
-            ev = batch.dop853_event(_up(np.full(r.size, r.mu)), _up(r.y0), _up(np.zeros(r.size)),
-                                    _up(r.tmax), ev_index=r.ev_index,
-                                    ev_offset=r.ev_offset, direction=r.direction, rtol=r.rtol,
-                                    atol=r.atol, xtol=r.xtol, gtol=r.gtol)
-            self.h = [_pinned(ev.hit), _pinned(ev.t_hit), _pinned(ev.y_hit),
-                      _pinned(ev.n_accepted)]
+            self.pk = _Packed(fin, 13, 3)
+            _launch_event(self.pk, self.pk.din[3], stream, ev_index=r.ev_index,
+                          ev_offset=r.ev_offset, direction=r.direction, xtol=r.xtol, gtol=r.gtol,
+                          rtol=r.rtol, atol=r.atol)
+            self.pk.download(slice(0, 7))
             self.done.record(stream)
 
     def finish(self, server: "Server") -> None:
-        hit, th, yh, na = (t.numpy() for t in self.h)
-        self.job.result = FanResult(hit.copy(), th.copy(), yh.copy(), na.copy())
-        server.steps += int(na.sum())
+        f, i = self.pk.hf.numpy(), self.pk.hi.numpy()
+        self.job.result = FanResult(i[0].astype(bool), f[0].copy(), f[1:7].T.copy(),
+                                    i[2].copy())
+        server.steps += int(i[2].sum())
         server._complete(self.job)
 
 
@@ -151,35 +207,37 @@ class _EventBatch(_Pending):
     def _launch(self, idx, ts, span):
         self.idx = idx
         self.cur_ts, self.cur_span = ts, span
+        self.aligned = not np.isclose(0.0, ts)
+        self.searched = not np.isclose(0.0, span)
+        k = len(idx)
+        fin = np.empty((10, k))
+        fin[0], fin[1], fin[2], fin[3], fin[4:10] = self.mu[idx], 0.0, span, ts, self.states[idx].T
         with torch.cuda.stream(self.stream):
-            k = len(idx)
-            y_dev = _up(self.states[idx])
-            mu = _up(self.mu[idx])
-            zeros = _up(np.zeros(k))
-            self.h = []
-            if not np.isclose(0.0, ts):
-                al = batch.dop853_final(mu, y_dev, zeros, _up(np.full(k, ts)), dim=6,
-                                        forward=self.forward)
-                y_dev = al.y
-                self.h.append(_pinned(al.n_accepted))
-            if not np.isclose(0.0, span):
-                ev = batch.dop853_event(mu, y_dev, zeros, _up(np.full(k, span)), ev_index=Y_INDEX,
-                                        ev_offset=0.0, direction=0, rtol=1e-12, atol=1e-12,
-                                        max_step=math.inf)
-                self.h += [_pinned(ev.hit), _pinned(ev.t_hit), _pinned(ev.y_hit),
-                           _pinned(ev.y_last), _pinned(ev.n_accepted)]
+            # fout rows: t_hit, y_hit (6), y_last (6), aligned state (6);
+            # iout rows: hit, status, n_accepted (event), status, n_accepted (alignment)
+            pk = self.pk = _Packed(fin, 19, 5)
+            y_dev = pk.din[4:10]
+            if self.aligned:
+                pk_al = _AlignView(pk)
+                _launch_final(pk_al, 6, y_dev, pk.fout[13:19], self.stream,
+                              forward=self.forward, flip_indices=None,
+                              variant=_lib.HH_VARIANT_PLAIN, max_step=1e4, istatus=3, inacc=4)
+                y_dev = pk.fout[13:19]
+            if self.searched:
+                _launch_event(pk, y_dev, self.stream, ev_index=Y_INDEX, ev_offset=0.0,
+                              direction=0, xtol=1e-12, gtol=1e-12, rtol=1e-12, atol=1e-12)
+            pk.download(slice(0, 13))
             self.done.record(self.stream)
 
     def finish(self, server: "Server") -> bool:
         """Consume the finished stage; returns True when the batch is complete."""
-        h = [t.numpy() for t in self.h]
-        if not np.isclose(0.0, self.cur_ts):
-            server.steps += int(h[0].sum())
-            h = h[1:]
-        if h:
-            hit, th, yh, yl, na = h
-            server.steps += int(na.sum())
-            hit = hit.astype(bool)
+        f, i = self.pk.hf.numpy(), self.pk.hi.numpy()
+        if self.aligned:
+            server.steps += int(i[4].sum())
+        if self.searched:
+            server.steps += int(i[2].sum())
+            hit = i[0].astype(bool)
+            th, yh, yl = f[0], f[1:7].T, f[7:13].T
             t_rel = np.where(hit, th, self.cur_span)
             ok = hit & (t_rel < self.cur_span) & (t_rel >= 0.0)
             y_out = np.where(hit[:, None], yh, yl)
@@ -196,6 +254,16 @@ class _EventBatch(_Pending):
         self.bank.feed_events(self.rows, self.t, self.xe, self.ok)
         server._advance(self.bank, self.rows)
         return True
+
+
+class _AlignView:
+    """The alignment launch reads mu from row 0, t0 (zeros) from row 1 and the
+    alignment time from row 3 of an event batch's packed input."""
+
+    def __init__(self, pk: _Packed):
+        self.k = pk.k
+        self.din = (pk.din[0], pk.din[1], pk.din[3])
+        self.iout = pk.iout
 
 
 class _StmBatch(_Pending):
@@ -216,17 +284,21 @@ class _StmBatch(_Pending):
             PHI0[:, 36:] = states
             mu = bank.mu[rows]
             k = int(self.live.sum())
+            fin = np.empty((45, k))
+            fin[0], fin[1], fin[2], fin[3:45] = mu[self.live], 0.0, self.tev[self.live], \
+                PHI0[self.live].T
             with torch.cuda.stream(stream):
-                r = batch.dop853_final(_up(mu[self.live]), _up(PHI0[self.live]),
-                                       _up(np.zeros(k)), _up(self.tev[self.live]), dim=42,
-                                       flip_indices=slice(36, 42), max_step=1e4, variant="wide")
-                self.h = (_pinned(r.y[:, :36]), _pinned(r.n_accepted))
+                pk = self.h = _Packed(fin, 42, 2)
+                _launch_final(pk, 42, pk.din[3:45], pk.fout, stream, forward=1,
+                              flip_indices=slice(36, 42), variant=_lib.HH_VARIANT_WIDE,
+                              max_step=1e4, istatus=0, inacc=1)
+                pk.download(slice(0, 36))
         self.done.record(stream)
 
     def finish(self, server: "Server") -> bool:
         phi = np.repeat(np.eye(6)[None], len(self.rows), axis=0)
         if self.h is not None:
-            y, na = self.h[0].numpy(), self.h[1].numpy()
+            y, na = self.h.hf.numpy().T, self.h.hi.numpy()[1]
             phi[self.live] = y.reshape(-1, 6, 6)
             server.steps += int(na.sum())
         self.bank.feed_stm(self.rows, phi)
@@ -321,7 +393,9 @@ class Server:
             else:
                 p = job.request
                 key = (len(p.residual_indices), len(p.control_indices))
-                bank = self.banks.setdefault(key, _Bank(*key))
+                bank = self.banks.get(key)
+                if bank is None:
+                    bank = self.banks[key] = _Bank(*key)
                 i = bank.add(p)
                 self.owner[(key, i)] = job
                 self.ready.setdefault(key, []).append(i)
@@ -344,18 +418,24 @@ class Server:
                 self.inflight.append(_StmBatch(bank, jac, self._stream()))
 
     def _poll(self) -> bool:
+        """Collect finished batches.  Work on one stream completes in launch
+        order, so only the oldest batch of each stream is queried."""
         progressed = False
-        keep = []
+        seen = set()
+        keep, relaunched = [], []
         for b in self.inflight:
-            if b.ready():
-                progressed = True
-                if isinstance(b, _FanLaunch):
-                    b.finish(self)
-                elif not b.finish(self):
-                    keep.append(b)
-            else:
+            sid = id(b.stream)
+            if sid in seen or not b.ready():
+                seen.add(sid)
                 keep.append(b)
-        self.inflight = keep
+                continue
+            progressed = True
+            if isinstance(b, _FanLaunch):
+                b.finish(self)
+            elif not b.finish(self):
+                relaunched.append(b)       # its next stage is now last on its stream
+                seen.add(sid)
+        self.inflight = keep + relaunched
         return progressed
 
     def busy(self) -> bool:
