@@ -63,8 +63,10 @@ def cpu_baseline(w, n_threads: int, target_s: float = 12.0):
         stride = max(1, n // m)
         idx = np.arange(0, n, stride)[:m]
         t = time.perf_counter()
-        _, na, nr, st = O.batch_dop853_final(w["mu"][idx], w["y0"][idx], w["t0"][idx],
-                                             w["tf"][idx], max_step=1e4, max_steps=w["max_steps"],
+        mu = np.broadcast_to(np.asarray(w["mu"], dtype=float), (n,))
+        _, na, nr, st = O.batch_dop853_final(mu[idx], w["y0"][idx], w["t0"][idx], w["tf"][idx],
+                                             max_step=1e4,
+                                             max_steps=w.get("max_steps", 10_000_000),
                                              n_threads=n_threads)
         dt = time.perf_counter() - t
         spent, steps, used = dt, int(na.sum()), len(idx)
@@ -94,6 +96,132 @@ def serving_run(seconds: float, rate: float = 2000.0):
     return out
 
 
+def _timed(fn, reps, warm):
+    """Wall time of ``reps`` calls after ``warm`` untimed ones, bracketed by
+    device synchronisation, plus the HIP-event time of each call."""
+    import torch
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(reps)]
+    t = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record()
+        fn()
+        e1.record()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t, [e0.elapsed_time(e1) / 1e3 for e0, e1 in evs]
+
+
+def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
+    """The smaller configurations: 1024 x 6-d fixed RK8 on a 2000-point grid
+    to 2 pi; 4096 x 42-d DOP853 with uneven t_f; the 32,768-trajectory
+    manifold fan (DOP853 to the first y = 0 crossing with vy < 0)."""
+    import numpy as np
+    import torch
+    from hiten_amd import batch, workloads as W
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out = {"n_gpus": 1, "steps": steps, "warmup": warmup, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "unit": "trajectory-steps/s"}
+    if name == "rk8_grid":
+        w = W.rk8_grid()
+        n = w["y0"].shape[0]
+        dev = batch._device()
+        y0, mu_d = (torch.as_tensor(w["y0"], device=dev), torch.full((n,), w["mu"], device=dev,
+                                                                     dtype=torch.float64))
+        t0_d = torch.zeros(n, dtype=torch.float64, device=dev)
+        tf_d = torch.full((n,), w["tf"], dtype=torch.float64, device=dev)
+        fn = lambda: batch.rk8_fixed(mu_d, y0, t0_d, tf_d, w["steps"])
+        wall, ev = _timed(fn, steps, warmup)
+        per = n * (w["steps"] - 1)
+        fl = per * 1167
+        out.update(metric="FP64 trajectory-steps/s, fixed-step RK8 6-d", value=per * steps / wall,
+                   ms_per_step=wall / steps * 1e3,
+                   config={"workload": "rk8_grid: 1024 x 6-d, RK8, 2000 grid points to 2 pi, "
+                                       "mu=0.012150585609624", "kernel": "k_rk8_fixed6"},
+                   roofline={"bound": "mfma", "achieved": fl / (sum(ev) / len(ev)) / 1e12,
+                             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": fl / (sum(ev) / len(ev)) / 1e12 / FP64_PEAK_TFLOPS,
+                             "traffic": None})
+        if not no_cpu:
+            from oracle import oracle as O
+            k = 256
+            t = time.perf_counter()
+            O.batch_rk8_final(w["mu"], w["y0"][:k], w["t0"], w["tf"], w["steps"], n_threads=threads)
+            dt = time.perf_counter() - t
+            out["cpu_baseline"] = {"value": k * (w["steps"] - 1) / dt, "unit": "trajectory-steps/s",
+                                   "cores": threads, "kind": "port",
+                                   "sample": f"first {k} of {n} trajectories"}
+    elif name == "stm_uneven":
+        w = W.stm_uneven()
+        res = {}
+        for variant in ("plain", "persistent", "wide"):
+            b = batch.prepare_final(w["mu"], w["y0"], w["t0"], w["tf"], dim=42)
+            fn = lambda: batch.launch_final(b, variant=variant, max_step=1e4)
+            wall, ev = _timed(fn, steps, warmup)
+            acc = int(b.n_accepted.to(torch.int64).sum().item())
+            res[variant] = dict(value=acc * steps / wall, launch_s=sum(ev) / len(ev),
+                                accepted=acc)
+        best = max(res, key=lambda k: res[k]["value"])
+        acc, ls = res[best]["accepted"], res[best]["launch_s"]
+        out.update(metric="FP64 STM trajectory-steps/s, DOP853 42-d", value=res[best]["value"],
+                   ms_per_step=ls * 1e3,
+                   config={"workload": "stm_uneven: 4096 x 42-d, DOP853 rtol=atol=1e-12, "
+                                       "t_f ~ U(2.5, 3.5)", "kernel": best},
+                   roofline={"bound": "mfma", "achieved": acc * FLOPS_PER_STEP_42 / ls / 1e12,
+                             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": acc * FLOPS_PER_STEP_42 / ls / 1e12 / FP64_PEAK_TFLOPS,
+                             "traffic": None},
+                   variants=res)
+        if not no_cpu:
+            out["cpu_baseline"] = cpu_baseline(w, threads, target_s=8.0)
+    elif name == "manifold_fan":
+        w = W.manifold_fan()
+        n = w["y0"].shape[0]
+        dev = batch._device()
+        y0 = torch.as_tensor(w["y0"], device=dev)
+        mu_d = torch.full((n,), w["mu"], dtype=torch.float64, device=dev)
+        t0_d = torch.zeros(n, dtype=torch.float64, device=dev)
+        tm_d = torch.as_tensor(w["tmax"], device=dev)
+        holder = {}
+
+        def fn():
+            holder["r"] = batch.dop853_event(mu_d, y0, t0_d, tm_d, ev_index=1,
+                                             direction=-1, xtol=w["xtol"], gtol=w["gtol"])
+        wall, ev = _timed(fn, steps, warmup)
+        r = holder["r"]
+        acc = int(r.n_accepted.to(torch.int64).sum().item())
+        hits = int(r.hit.sum().item())
+        ls = sum(ev) / len(ev)
+        fl = acc * 1278 + hits * 4200
+        out.update(metric="FP64 trajectory-steps/s, DOP853 6-d to the first y = 0 crossing",
+                   value=acc * steps / wall, ms_per_step=wall / steps * 1e3,
+                   hits_per_s=hits * steps / wall,
+                   config={"workload": "manifold_fan: 64 seeds x 512 directions (32,768), "
+                                       "DOP853 1e-12, y = 0 with vy < 0, bisection 1e-13, "
+                                       "t_max ~ U(2, 8)", "kernel": "k_dop853_event6"},
+                   roofline={"bound": "mfma", "achieved": fl / ls / 1e12,
+                             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": fl / ls / 1e12 / FP64_PEAK_TFLOPS, "traffic": None})
+        if not no_cpu:
+            from oracle import oracle as O
+            k = 2048
+            idx = np.arange(0, n, n // k)[:k]
+            t = time.perf_counter()
+            _, _, _, na, _ = O.batch_dop853_event(w["mu"], w["y0"][idx], w["tmax"][idx],
+                                                  ev_index=1, direction=-1, xtol=w["xtol"],
+                                                  gtol=w["gtol"], n_threads=threads)
+            dt = time.perf_counter() - t
+            out["cpu_baseline"] = {"value": int(na.sum()) / dt, "unit": "trajectory-steps/s",
+                                   "cores": threads, "kind": "port",
+                                   "sample": f"{k} of {n} trajectories (every {n // k}th)"}
+    else:
+        raise SystemExit(f"unknown workload {name!r}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +235,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--heavy", type=int, default=1000)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--workload", default="persistent_262k",
+                    choices=["persistent_262k", "rk8_grid", "stm_uneven", "manifold_fan"])
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -119,6 +249,11 @@ def main():
     import torch
 
     from hiten_amd import batch, workloads as W
+
+    if args.workload != "persistent_262k":
+        print(json.dumps(other_workload(args.workload, args.steps, args.warmup, args.no_cpu)),
+              flush=True)
+        return
 
     from hiten_amd import _lib
     _lib.check(_lib.load().hh_set_tuning(args.heavy), "hh_set_tuning")
