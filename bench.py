@@ -233,7 +233,9 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--heavy", type=int, default=1000)
+    ap.add_argument("--heavy", type=int, default=2000)
+    ap.add_argument("--heavy-age", type=int, default=128)
+    ap.add_argument("--order", default="span", choices=["span", "interleave", "index"])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--workload", default="persistent_262k",
                     choices=["persistent_262k", "rk8_grid", "stm_uneven", "manifold_fan"])
@@ -256,10 +258,10 @@ def main():
         return
 
     from hiten_amd import _lib
-    _lib.check(_lib.load().hh_set_tuning(args.heavy), "hh_set_tuning")
+    _lib.check(_lib.load().hh_set_tuning(args.heavy, args.heavy_age), "hh_set_tuning")
     w = W.persistent_262k(args.n)
     w["max_steps"] = args.max_steps
-    b = batch.prepare_final(w["mu"], w["y0"], w["t0"], w["tf"], dim=42)
+    b = batch.prepare_final(w["mu"], w["y0"], w["t0"], w["tf"], dim=42, order=args.order)
     kw = dict(rtol=w["rtol"], atol=w["atol"], max_step=1e4, max_steps=args.max_steps,
               block=args.block, grid=args.grid)
 

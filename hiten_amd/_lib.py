@@ -43,7 +43,7 @@ _ULL = C.c_ulonglong
 _SIGNATURES = {
     "hh_last_error": ([], C.c_char_p),
     "hh_abi_version": ([], _I),
-    "hh_set_tuning": ([_I], _I),
+    "hh_set_tuning": ([_I, _I], _I),
     "hh_rk8_fixed": ([_I, _I, _P, _P, _P, _I, _P, _I, _ULL, _P, _P, _P, _I, _P], _I),
     "hh_dop853_final": ([_I, _I, _P, _P, _P, _D, _D, _D, _D, _LL, _I, _ULL, _P, _P, _P, _P, _P,
                          _I, _I, _I, _P, _P, _P], _I),

@@ -167,16 +167,26 @@ def prepare_final(mu, y0, t0, tf, dim: int = 6, order: str = "span") -> FinalBat
 
     ``order`` sets the hand-out order of the persistent kernel: "span" gives
     the longest integration spans |tf - t0| first (longest-processing-time
-    first, so the slowest trajectories do not start last), "index" keeps the
-    row order.
+    first, so the slowest trajectories do not start last); "interleave" also
+    starts the longest eighth first but puts seven shorter spans beside each
+    of them, so that a wave holding a long-running trajectory empties soon;
+    "index" keeps the row order.
     """
     y = _soa(y0, dim)
     n = y.shape[1]
     i32 = dict(dtype=torch.int32, device=y.device)
     t0_t, tf_t = _vec(t0, n, "t0"), _vec(tf, n, "tf")
     perm = None
-    if order == "span":
-        perm = torch.argsort(torch.abs(tf_t - t0_t), descending=True, stable=True).to(torch.int32)
+    if order in ("span", "interleave"):
+        perm = torch.argsort(torch.abs(tf_t - t0_t), descending=True, stable=True)
+        if order == "interleave" and n >= 8:
+            # every aligned group of 8 hand-outs: one of the longest eighth of
+            # the spans, then seven of the rest, both in decreasing span
+            k = n // 8
+            longest, rest = perm[:k], perm[k:]
+            head = torch.cat([longest[:, None], rest[:7 * k].reshape(k, 7)], dim=1).reshape(-1)
+            perm = torch.cat([head, rest[7 * k:]])
+        perm = perm.to(torch.int32)
     elif order not in ("index", None):
         raise ValueError(f"unknown order {order!r}")
     return FinalBatch(dim, n, _vec(mu, n, "mu"), t0_t, tf_t, y, torch.empty_like(y),

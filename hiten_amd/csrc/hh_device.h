@@ -32,6 +32,27 @@
 
 #define HH_TABLE static __device__ const
 #define HH_FN static __device__ __forceinline__
+
+/* Correctly rounded FP64 square root.  For normal x >= 2^-767 (the range in
+ * which the compiler's own expansion applies no scaling) this is that
+ * expansion without its scaling and special-value fixups: the hardware
+ * reciprocal square root refined by the coupled iteration g ~ sqrt(x),
+ * h ~ 1/(2 sqrt(x)) and two residual corrections d = x - g*g (fma), so the
+ * result is the same bits as __builtin_sqrt; other x take __builtin_sqrt.  */
+static __device__ __forceinline__ double hh_dev_sqrt(double x) {
+    if (!(x >= 0x1p-767) || !(x <= 1.7976931348623157e308)) return __builtin_sqrt(x);
+    const double y0 = __builtin_amdgcn_rsq(x);
+    double g = x * y0;
+    double h = y0 * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    double d = __builtin_fma(-g, g, x);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+#define HH_SQRT(x) hh_dev_sqrt(x)
 #include "hh_pow.h"
 
 namespace hh {

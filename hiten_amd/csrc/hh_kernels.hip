@@ -30,7 +30,8 @@ using namespace hh;
 namespace {
 
 thread_local char g_err[512] = "";
-int g_heavy = 1000;   /* persistent 42-d kernel: see hh_set_tuning */
+int g_heavy = 2000;      /* persistent 42-d kernel: see hh_set_tuning */
+int g_heavy_age = 128;
 
 int fail(int code, const char* msg) {
     snprintf(g_err, sizeof g_err, "%s", msg);
@@ -472,8 +473,10 @@ struct Final42Args {
     const int* order;   /* work order of the persistent form, or NULL (index order) */
     int m;              /* grid points per trajectory (grid form)                  */
     const double* teval;  /* explicit output times (grid form), or NULL for linspace */
-    int heavy;            /* persistent form: attempts after which a trajectory is
-                             moved to a wave of its own, 0 = only in the tail */
+    int heavy;            /* persistent form: projected attempts above which a
+                             trajectory is moved to a wave of its own, 0 = only
+                             in the tail */
+    int heavy_age;        /* attempts before the projection is trusted */
 };
 
 __device__ __forceinline__ void load42(const Final42Args& a, int traj, int b, Lane42& p,
@@ -684,7 +687,16 @@ void k_dop853_final42_persist(Final42Args a) {
     for (;;) {
         /* A trajectory that has taken `heavy` attempts stops the refill of its
          * wave, so that it is soon alone there and moves to the wave-wide form */
-        const bool hold = (a.heavy > 0) && __any(traj >= 0 && (s.nacc + s.nrej) >= a.heavy);
+        bool heavy = false;
+        if (a.heavy > 0 && traj >= 0) {
+            /* projected attempts = age * span / elapsed time */
+            const int age = s.nacc + s.nrej;
+            if (age >= a.heavy_age) {
+                const double t0 = a.t0[traj];
+                heavy = (double)age * (s.tf - t0) >= (double)a.heavy * (s.t - t0);
+            }
+        }
+        const bool hold = __any(heavy);
         const bool need = (traj < 0) && !exhausted && !hold;
         const unsigned long long m = __ballot(need && b == 0);
         if (m) {
@@ -881,9 +893,11 @@ const char* hh_last_error(void) { return g_err; }
 
 int hh_abi_version(void) { return HH_ABI_VERSION; }
 
-int hh_set_tuning(int heavy_attempts) {
-    if (heavy_attempts < 0) return fail(HH_ERR_ARGS, "hh_set_tuning: heavy_attempts < 0");
-    g_heavy = heavy_attempts;
+int hh_set_tuning(int heavy_projected, int min_age) {
+    if (heavy_projected < 0 || min_age < 1)
+        return fail(HH_ERR_ARGS, "hh_set_tuning: heavy_projected < 0 or min_age < 1");
+    g_heavy = heavy_projected;
+    g_heavy_age = min_age;
     return HH_OK;
 }
 
@@ -942,6 +956,7 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
     a.m = 0;
     a.teval = nullptr;
     a.heavy = g_heavy;
+    a.heavy_age = g_heavy_age;
     if (block <= 0) block = 256;
     if (block != 64 && block != 128 && block != 256 && block != 512)
         return fail(HH_ERR_ARGS, "hh_dop853_final: block must be 64, 128, 256 or 512");

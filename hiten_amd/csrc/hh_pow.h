@@ -36,6 +36,12 @@
 
 #include "hh_tables.h"
 
+/* The correctly rounded square root; a device includer may supply a faster
+ * correctly rounded one for the argument range it guarantees.              */
+#ifndef HH_SQRT
+#define HH_SQRT(x) __builtin_sqrt(x)
+#endif
+
 #ifdef __cplusplus
 #define HH_U64 unsigned long long
 #else
@@ -80,7 +86,7 @@ HH_FN double hh_pow_pos_special(double x) {
  * the leading terms.  No division is needed.                              */
 HH_FN double hh_pow15(double x) {
     if (!(x > 0.0) || x > 1.7976931348623157e308) return hh_pow_pos_special(x);
-    double s = __builtin_sqrt(x);
+    double s = HH_SQRT(x);
     double e = __builtin_fma(-s, s, x);               /* x - s*s, exact    */
     double p, pe;
     HH_TWO_PROD(x, s, p, pe);
@@ -90,7 +96,7 @@ HH_FN double hh_pow15(double x) {
 
 HH_FN double hh_pow25(double x) {
     if (!(x > 0.0) || x > 1.7976931348623157e308) return hh_pow_pos_special(x);
-    double s = __builtin_sqrt(x);
+    double s = HH_SQRT(x);
     double e = __builtin_fma(-s, s, x);
     double q, qe;
     HH_TWO_PROD(x, x, q, qe);                         /* x^2 = q + qe      */
@@ -108,7 +114,7 @@ HH_FN void hh_pow15_25(double x, double* p15, double* p25) {
         *p25 = *p15;
         return;
     }
-    double s = __builtin_sqrt(x);
+    double s = HH_SQRT(x);
     double e = __builtin_fma(-s, s, x);
     double he = 0.5 * e;
     double p, pe;

@@ -41,6 +41,10 @@ struct Wave42 {
     bool act;                 /* holds a component (b < 7, d < 6)               */
     bool odd;                 /* quotient lane on the second primary's side      */
     int kind;                 /* quotient kind of lanes 0..11                    */
+    int pv_addr;              /* ds_bpermute byte address of the lane holding the
+                                 velocity-part entry this lane's row needs       */
+    double cl;                /* mu (even lanes) or -mu2 (odd lanes): x + cl is
+                                 x + mu or x - mu2 exactly                       */
     unsigned long long sign;  /* bit 63: this lane's derivative is negated       */
 
     __device__ __forceinline__ void setup(int lane_, double mu_, int fwd,
@@ -56,6 +60,17 @@ struct Wave42 {
         mu2 = 1.0 - mu_;
         const int c = act ? comp42(b, d) : 0;
         sign = (act && fwd == -1 && ((flip >> c) & 1ULL)) ? (1ULL << 63) : 0ULL;
+        /* rows 0..2 need entry d+3; row 3 needs entry 4 and row 4 entry 3
+         * (Coriolis terms of the column rows and of the state rows)          */
+        const int src = (d < 3) ? d + 3 : ((d == 3) ? 4 : 3);
+        pv_addr = ((lane & ~7) | src) << 2;
+        cl = odd ? -mu2 : mu_;
+    }
+
+    __device__ __forceinline__ double gather_pv(double v) const {
+        const int lo = __builtin_amdgcn_ds_bpermute(pv_addr, __double2loint(v));
+        const int hi = __builtin_amdgcn_ds_bpermute(pv_addr, __double2hiint(v));
+        return __hiloint2double(hi, lo);
     }
 
     /* derivative of this lane's component; v = this lane's component of the
@@ -64,9 +79,7 @@ struct Wave42 {
         const double p0 = group_get<0>(v);
         const double p1 = group_get<1>(v);
         const double p2 = group_get<2>(v);
-        const double p3 = group_get<3>(v);
-        const double p4 = group_get<4>(v);
-        const double p5 = group_get<5>(v);
+        const double pv = gather_pv(v);
         const double x = rlane(v, 0);
         const double y = rlane(v, 1);
         const double z = rlane(v, 2);
@@ -75,7 +88,7 @@ struct Wave42 {
         /* quotient lane k = kind*2 + odd (Lane42's q1..q6 of both lane kinds):
          *  k: 0 xm/r3  1 xr/R3  2 y/r3  3 y/R3  4 z/r3  5 z/R3
          *     6 mu2/r5 7 mu/R5  8 mu2/r3 9 mu/R3 10 (mu2 xm)/r5 11 (mu xr)/R5 */
-        const double xo = odd ? xr : xm;
+        const double xo = x + cl;
         const double P = (xo * xo + y * y) + z * z;
         double P3, P5;
         hh_pow15_25(P, &P3, &P5);
@@ -89,9 +102,9 @@ struct Wave42 {
         const double Q0 = rlane(q, 0), Q1 = rlane(q, 1), Q2 = rlane(q, 2), Q3 = rlane(q, 3);
         const double Q4 = rlane(q, 4), Q5 = rlane(q, 5), Q6 = rlane(q, 6), Q7 = rlane(q, 7);
         const double Q8 = rlane(q, 8), Q9 = rlane(q, 9), Q10 = rlane(q, 10), Q11 = rlane(q, 11);
-        /* state rows (rtbp.py:233-242) */
-        const double s3 = ((x - mu2 * Q0) - mu * Q1) + 2.0 * p4;
-        const double s4 = ((y - mu2 * Q2) - mu * Q3) - 2.0 * p3;
+        /* state rows (rtbp.py:233-242); pv is vy on lane 3, vx on lane 4 */
+        const double s3 = ((x - mu2 * Q0) - mu * Q1) + 2.0 * pv;
+        const double s4 = ((y - mu2 * Q2) - mu * Q3) - 2.0 * pv;
         const double s5 = (-mu2 * Q4) - mu * Q5;
         /* Hessian entries (rtbp.py:124-141) */
         const double a53 = Q6 * 3.0;
@@ -110,9 +123,7 @@ struct Wave42 {
         const double cC = (d == 3) ? uxz : ((d == 4) ? uyz : uzz);
         const double acc = ((0.0 + cA * p0) + cB * p1) + cC * p2;
         const double E = (d == 3) ? 2.0 : -2.0;
-        const double pE = (d == 3) ? p4 : p3;
-        const double row = (d == 5) ? acc : acc + E * pE;
-        const double pv = (d == 0) ? p3 : ((d == 1) ? p4 : p5);
+        const double row = (d == 5) ? acc : acc + E * pv;
         const double sv = (d == 3) ? s3 : ((d == 4) ? s4 : s5);
         const double lo3 = st ? pv : (0.0 + pv);
         const double hi3 = st ? sv : row;

@@ -45,13 +45,14 @@ const char* hh_last_error(void);
 /* HH_ABI_VERSION of the built library. */
 int hh_abi_version(void);
 
-/* Tuning of the persistent 42-d kernel (process-wide, default 1000): a
- * trajectory that has taken heavy_attempts step attempts in the 8-lanes-per-
- * trajectory layout stops the refill of its wave and, once alone there, is
- * finished with one component per lane; 0 = only when the work queue is
- * empty.  No counterpart in the reference (scheduling only; results are
- * bit-identical either way).                                               */
-int hh_set_tuning(int heavy_attempts);
+/* Tuning of the persistent 42-d kernel (process-wide).  A trajectory that
+ * has taken at least min_age step attempts (default 128) and whose projected
+ * total, attempts * (tf - t0) / (t - t0), reaches heavy_projected (default
+ * 2000) stops the refill of its wave; once alone there it is finished one
+ * component per lane.  heavy_projected = 0 moves trajectories only when the
+ * work queue is empty.  No counterpart in the reference (scheduling only;
+ * results are bit-identical either way).                                   */
+int hh_set_tuning(int heavy_projected, int min_age);
 
 /* Fixed-step 13-stage RK8 on the 6-d CR3BP field (dim 6) or the 42-d
  * variational system (dim 42) over the grid t_vals[0..steps-1] when t_vals

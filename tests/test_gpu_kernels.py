@@ -163,10 +163,10 @@ def heavy_threshold():
     from hiten_amd import _lib
     lib = _lib.load()
 
-    def set_(k):
-        _lib.check(lib.hh_set_tuning(int(k)), "hh_set_tuning")
+    def set_(k, age=1):
+        _lib.check(lib.hh_set_tuning(int(k), int(age)), "hh_set_tuning")
     yield set_
-    set_(1000)
+    set_(2000, 128)
 
 
 @pytest.mark.parametrize("heavy", [0, 3, 40])
