@@ -54,6 +54,23 @@ _SIGNATURES = {
     "hh_eval_rhs": ([_I, _I, _P, _P, _P, _P], _I),
     "hh_eval_math": ([_I, _P, _P, _P, _P], _I),
     "hh_traj_filters": ([_I, _I, _P, _P, _P, _P, _P, _P], _I),
+    "hh_memcpy_async": ([_P, _P, _ULL, _I, _P], _I),
+    "hh_event_create": ([], _P),
+    "hh_event_destroy": ([_P], None),
+    "hh_event_record": ([_P, _P], _I),
+    "hh_event_query": ([_P], _I),
+    "hh_bank_create": ([_I, _I], _P),
+    "hh_bank_destroy": ([_P], None),
+    "hh_bank_size": ([_P], _I),
+    "hh_bank_add": ([_P, _D, _P, _P, _P, _P, _I, _I, _I, _D, _D, _D, _D, _D], _I),
+    "hh_bank_stages": ([_P, _I, _P, _P], _I),
+    "hh_bank_event_inputs": ([_P, _I, _P, _P, _P, _I], _I),
+    "hh_bank_feed_events": ([_P, _I, _P, _P, _P, _P], _I),
+    "hh_bank_stm_inputs": ([_P, _I, _P, _P, _P], _I),
+    "hh_bank_jacobians": ([_P, _I, _P, _P, _P, _P], _I),
+    "hh_bank_set_steps": ([_P, _I, _P, _P], _I),
+    "hh_bank_result": ([_P, _I, _P, _P, _P, _P, _P, _P, _P, _P], _I),
+    "hh_bank_release": ([_P, _I], _I),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -2,7 +2,8 @@
 
     python -m hiten_amd.build
 
-The library is one translation unit compiled by hipcc.  ``-ffp-contract=off``
+The library is the HIP kernels and C-ABI (hh_kernels.hip) plus host-side
+native code (hh_bank.cpp), compiled by hipcc.  ``-ffp-contract=off``
 is required for parity: it stops clang from fusing ``a*b + c`` into an FMA,
 which numba (fastmath=False) never does in the reference.
 """
@@ -18,8 +19,9 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libhiten_hip.so")
-SOURCES = [os.path.join(CSRC, "hh_kernels.hip")]
-DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("hh_device.h", "hh_pow.h", "hh_tables.h")] + [
+SOURCES = [os.path.join(CSRC, "hh_kernels.hip"), os.path.join(CSRC, "hh_bank.cpp")]
+DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("hh_device.h", "hh_wave.h", "hh_pow.h",
+                                                  "hh_tables.h")] + [
     os.path.join(ROOT, "include", "hiten_hip.h")]
 
 ARCH = os.environ.get("HH_OFFLOAD_ARCH", "gfx950")

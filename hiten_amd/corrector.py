@@ -24,6 +24,7 @@ result of the residual's search is reused (bit-identical).
 
 from __future__ import annotations
 
+import ctypes as C
 import math
 from dataclasses import dataclass, field
 from typing import Callable, Optional, Sequence
@@ -169,30 +170,6 @@ class _Engine:
         return phi
 
 
-def _spow(a: np.ndarray, e: float) -> np.ndarray:
-    """Elementwise a**e with NumPy's scalar power (the C library pow), which
-    is what the reference's per-orbit scalar code evaluates; NumPy's array
-    power may round differently."""
-    return np.array([v ** e for v in np.asarray(a, dtype=np.float64)], dtype=np.float64)
-
-
-def halo_quadratic_terms(mu: np.ndarray, X_ev: np.ndarray, Phi: np.ndarray) -> np.ndarray:
-    """halo_quadratic_term for a stack of orbits: (n, 2, 2), element for
-    element the operations of services/orbits.py:917-947."""
-    x, y, z, vx, vy, vz = (X_ev[:, i] for i in range(6))
-    mu2 = 1 - mu
-    rho_1 = 1 / _spow(_spow(x + mu, 2) + _spow(y, 2) + _spow(z, 2), 1.5)
-    rho_2 = 1 / _spow(_spow(x - mu2, 2) + _spow(y, 2) + _spow(z, 2), 1.5)
-    omega_x = -(mu2 * (x + mu) * rho_1) - (mu * (x - mu2) * rho_2) + x
-    DDx = 2 * vy + omega_x
-    DDz = -(mu2 * z * rho_1) - (mu * z * rho_2)
-    small = np.abs(vy) < 1e-9
-    vy = np.where(small, np.where(vy != 0, np.sign(vy) * 1e-9, 1e-9), vy)
-    DD = np.stack([DDx, DDz], axis=1)                      # (n, 2)
-    P = Phi[:, Y, :][:, [X, VY]]                           # (n, 2)
-    return (DD[:, :, None] * P[:, None, :]) / vy[:, None, None]
-
-
 def solve_delta_stack(J: np.ndarray, r: np.ndarray, cond_threshold: float = 1e8) -> np.ndarray:
     """solve_delta_dense for a stack of square systems; the stacked LAPACK
     calls are the per-matrix ones.  A stack that raises falls back to the
@@ -208,235 +185,174 @@ def solve_delta_stack(J: np.ndarray, r: np.ndarray, cond_threshold: float = 1e8)
         return np.stack([solve_delta_dense(a, b, cond_threshold) for a, b in zip(J, r)])
 
 
+_ERRORS = {
+    1: "no section crossing for the current state",
+    3: "Armijo line search failed to find a productive step.",
+}
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
 class _Bank:
-    """Corrections advanced together, one row each: _NewtonBackend.run as a
-    vectorized state machine.
+    """Corrections advanced together, one row each, in the library's native
+    state machine (hh_bank_*): _NewtonBackend.run with its Armijo line search.
 
     Stages: RESID waits for the event search at x (the residual), JAC for
-    the STM to the event time, ARMIJO for the event searches of every trial
-    step alpha = 1, r, r^2, ... >= min_alpha.  The line search is evaluated
-    speculatively -- all trials in one batch -- and armijo.py:111-164 is
-    replayed over them in order, so the accepted step, the fallback to the
-    best point and the failure are the sequential algorithm's.  The residual
-    of the accepted trial is the residual at the next iterate (the same state
-    through the same deterministic search) and is reused.  Every arithmetic
-    step is the reference's, element for element.  Rows of one bank share the
+    the STM to the event time, then the dense solve (NumPy, stacked), then
+    ARMIJO for the event searches of every trial step alpha = 1, r, r^2, ...
+    >= min_alpha.  The line search is evaluated speculatively -- all trials in
+    one batch -- and armijo.py:111-164 is replayed over them in order, so the
+    accepted step, the fallback to the best point and the failure are the
+    sequential algorithm's; the residual of the accepted trial is the
+    residual at the next iterate and is reused.  Rows of one bank share the
     numbers of residual and control indices.
     """
 
-    RESID, JAC, ARMIJO, DONE = 0, 1, 2, 3
+    RESID, JAC, STEP, ARMIJO, DONE = 0, 1, 2, 3, 4
 
     def __init__(self, nr: int, nc: int):
+        from . import _lib
+        self._l = _lib.load(require_gpu=False)
+        self._h = self._l.hh_bank_create(int(nr), int(nc))
+        if not self._h:
+            raise ValueError(f"a bank holds 1..6 residuals and controls, not ({nr}, {nc})")
         self.nr, self.nc = nr, nc
-        self.n = 0
-        self.cap = 0
-        self.jm = 1
-        self.problems: list = []
-        self.results: list = []
-        self._grow(64, 16)
+        self.problems: list = []          # by row (None once released)
+        self.mu = np.zeros(0)             # by row
+        self.fwd = np.zeros(0, dtype=np.int64)
 
-    def _grow(self, cap: int, jm: int) -> None:
-        def g(name, shape, dtype, fill):
-            old = getattr(self, name, None)
-            a = np.full((cap,) + shape, fill, dtype=dtype)
-            if old is not None:
-                a[tuple(slice(0, k) for k in old.shape)] = old
-            setattr(self, name, a)
-        g("mu", (), float, np.nan)
-        g("base", (6,), float, np.nan)
-        g("ctrl", (self.nc,), int, 0)
-        g("res", (self.nr,), int, 0)
-        g("tgt", (self.nr,), float, np.nan)
-        g("halo", (), bool, False)
-        g("fwd", (), int, 1)
-        g("max_att", (), int, 0)
-        g("tol", (), float, np.nan)
-        g("max_delta", (), float, np.inf)
-        g("c", (), float, np.nan)
-        g("x", (self.nc,), float, np.nan)
-        g("k", (), int, 0)
-        g("n_ev", (), int, 0)
-        g("n_stm", (), int, 0)
-        g("stage", (), int, self.DONE)
-        g("r", (self.nr,), float, np.nan)
-        g("rn", (), float, np.nan)
-        g("t_ev", (), float, np.nan)
-        g("x_ev", (6,), float, np.nan)
-        g("nal", (), int, 0)
-        g("alpha", (jm,), float, np.nan)
-        g("trial", (jm, self.nc), float, np.nan)
-        self.cap, self.jm = cap, jm
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            self._l.hh_bank_destroy(h)
+
+    @property
+    def n(self) -> int:
+        return len(self.problems)
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            raise RuntimeError(f"{what} failed (code {rc})")
 
     def add(self, p: CorrectionProblem) -> int:
-        al, a = [], 1.0
-        while a >= p.min_alpha:
-            al.append(a)
-            a = a * p.alpha_reduction
-        if self.n == self.cap or len(al) > self.jm:
-            self._grow(max(self.cap, 2 * self.cap if self.n == self.cap else self.cap),
-                       max(self.jm, len(al)))
-        i = self.n
-        self.n += 1
-        self.problems.append(p)
-        self.results.append(None)
-        self.mu[i] = p.mu
-        self.base[i] = np.asarray(p.x0, float)
-        self.ctrl[i] = list(p.control_indices)
-        self.res[i] = list(p.residual_indices)
-        self.tgt[i] = np.asarray(p.target, float)
-        self.halo[i] = p.extra_jacobian == "halo"
-        self.fwd[i] = p.forward
-        self.max_att[i] = p.max_attempts
-        self.tol[i] = p.tol
-        self.max_delta[i] = np.inf if p.max_delta is None else p.max_delta
-        self.c[i] = p.armijo_c
-        self.x[i] = self.base[i][self.ctrl[i]]
-        self.stage[i] = self.RESID
-        self.nal[i] = len(al)
-        self.alpha[i, :len(al)] = al
-        return i
+        res, ctrl = _i32(list(p.residual_indices)), _i32(list(p.control_indices))
+        x0, tgt = _f64(p.x0), _f64(p.target)
+        md = math.inf if p.max_delta is None else float(p.max_delta)
+        row = self._l.hh_bank_add(self._h, float(p.mu), _p(x0), _p(res), _p(ctrl), _p(tgt),
+                                  int(p.extra_jacobian == "halo"), int(p.forward),
+                                  int(p.max_attempts), float(p.tol), md, float(p.alpha_reduction),
+                                  float(p.min_alpha), float(p.armijo_c))
+        if row < 0:
+            raise ValueError("invalid correction problem for the native bank")
+        if row == len(self.problems):
+            self.problems.append(p)
+            if row >= len(self.mu):
+                cap = max(64, 2 * len(self.mu))
+                self.mu = np.concatenate([self.mu, np.zeros(cap - len(self.mu))])
+                self.fwd = np.concatenate([self.fwd, np.zeros(cap - len(self.fwd), np.int64)])
+        else:
+            self.problems[row] = p
+        self.mu[row] = float(p.mu)
+        self.fwd[row] = int(p.forward)
+        return row
 
-    # -- requests ------------------------------------------------------------
-    def full(self, rows: np.ndarray, params: np.ndarray) -> np.ndarray:
-        F = self.base[rows].copy()
-        np.put_along_axis(F, self.ctrl[rows], params, axis=1)
-        return F
+    def release(self, row: int) -> None:
+        """Give a finished row (result already read) back for reuse."""
+        self._check(self._l.hh_bank_release(self._h, int(row)), "hh_bank_release")
+        self.problems[row] = None
 
-    def _layout(self, rows: np.ndarray):
-        """(R, jm) mask of the event requests of each row (RESID: column 0,
-        ARMIJO: its trials) and the (R, jm, nc) parameters."""
-        resid = self.stage[rows] == self.RESID
-        cols = np.arange(self.jm)[None, :]
-        valid = np.where(resid[:, None], cols == 0, cols < self.nal[rows][:, None])
-        params = self.trial[rows].copy()
-        params[resid, 0] = self.x[rows[resid]]
-        return resid, valid, params
+    def stages(self, rows) -> np.ndarray:
+        rows = _i32(rows)
+        out = np.empty(len(rows), dtype=np.int32)
+        self._check(self._l.hh_bank_stages(self._h, len(rows), _p(rows), _p(out)), "hh_bank_stages")
+        return out
 
-    def event_inputs(self, rows: np.ndarray):
-        """States whose event searches the rows wait for (grouped by row, in
-        row order) and per-row counts."""
-        _, valid, params = self._layout(rows)
-        rid = np.broadcast_to(rows[:, None], valid.shape)[valid]
-        return self.full(rid, params[valid]), valid.sum(axis=1)
+    def event_inputs(self, rows):
+        """States whose event searches the rows wait for (grouped by row) and
+        per-row counts."""
+        rows = _i32(rows)
+        counts = np.empty(len(rows), dtype=np.int32)
+        total = self._l.hh_bank_event_inputs(self._h, len(rows), _p(rows), _p(counts), None, 0)
+        if total < 0:
+            raise RuntimeError("hh_bank_event_inputs failed")
+        states = np.empty((total, 6))
+        got = self._l.hh_bank_event_inputs(self._h, len(rows), _p(rows), _p(counts),
+                                           _p(states), total)
+        if got != total:
+            raise RuntimeError("hh_bank_event_inputs failed")
+        return states, counts
 
-    def stm_inputs(self, rows: np.ndarray):
-        return self.full(rows, self.x[rows]), self.t_ev[rows]
+    def feed_events(self, rows, t, xe, ok) -> None:
+        rows = _i32(rows)
+        self._check(self._l.hh_bank_feed_events(self._h, len(rows), _p(rows), _p(_f64(t)),
+                                                _p(_f64(xe)), _p(_i32(ok))), "hh_bank_feed_events")
 
-    # -- transitions ---------------------------------------------------------
-    def _finish(self, rows, ok, errs, t=None) -> None:
-        F = self.full(rows, self.x[rows])
-        for j, i in enumerate(rows):
-            self.results[i] = CorrectionResult(
-                bool(ok[j]), F[j], float(self.rn[i]), int(self.k[i]),
-                half_period=float(t[j]) if t is not None else math.nan, error=errs[j],
-                n_event_searches=int(self.n_ev[i]), n_stm=int(self.n_stm[i]))
-        self.stage[rows] = self.DONE
+    def stm_inputs(self, rows):
+        rows = _i32(rows)
+        states, tev = np.empty((len(rows), 6)), np.empty(len(rows))
+        self._check(self._l.hh_bank_stm_inputs(self._h, len(rows), _p(rows), _p(states), _p(tev)),
+                    "hh_bank_stm_inputs")
+        return states, tev
 
-    def _residual(self, rows, t, xe, ok) -> None:
-        """Top of the Newton loop for rows whose residual at x is known."""
-        self.n_ev[rows] += 1
-        if (~ok).any():
-            bad = rows[~ok]
-            self.rn[bad] = np.nan
-            self._finish(bad, np.zeros(len(bad), bool),
-                         ["no section crossing for the current state"] * len(bad))
-        rows, t, xe = rows[ok], t[ok], xe[ok]
-        if rows.size == 0:
-            return
-        r = np.take_along_axis(xe, self.res[rows], axis=1) - self.tgt[rows]
-        rn = np.max(np.abs(r), axis=1)
-        self.rn[rows] = rn
-        conv = rn < self.tol[rows]
-        if conv.any():
-            self._finish(rows[conv], np.ones(conv.sum(), bool), [""] * int(conv.sum()), t[conv])
-        maxed = ~conv & (self.k[rows] >= self.max_att[rows])
-        if maxed.any():
-            mr = rows[maxed]
-            self._finish(mr, np.zeros(len(mr), bool),
-                         [f"Newton did not converge after {self.max_att[i]} iterations "
-                          f"(|R|={self.rn[i]:.2e})." for i in mr])
-        go = ~conv & ~maxed
-        g = rows[go]
-        self.r[g], self.t_ev[g], self.x_ev[g] = r[go], t[go], xe[go]
-        self.stage[g] = self.JAC
+    def feed_stm(self, rows, Phi) -> None:
+        """Phi(t_event) (n, 6, 6): Jacobians with the Halo term (native),
+        dense solves (NumPy, as backends/base.py:187-245), trial steps."""
+        rows = _i32(rows)
+        n = len(rows)
+        J, r = np.empty((n, self.nr, self.nc)), np.empty((n, self.nr))
+        self._check(self._l.hh_bank_jacobians(self._h, n, _p(rows), _p(_f64(Phi)), _p(J), _p(r)),
+                    "hh_bank_jacobians")
+        d = _f64(solve_delta_stack(J, r))
+        self._check(self._l.hh_bank_set_steps(self._h, n, _p(rows), _p(d)), "hh_bank_set_steps")
 
-    def feed_events(self, rows: np.ndarray, t: np.ndarray, xe: np.ndarray, ok: np.ndarray) -> None:
-        """Results of event_inputs(rows), in the same order."""
-        resid, valid, _ = self._layout(rows)
-        R = len(rows)
-        T = np.full((R, self.jm), np.nan)
-        XE = np.full((R, self.jm, 6), np.nan)
-        OK = np.zeros((R, self.jm), bool)
-        T[valid], XE[valid], OK[valid] = t, xe, ok
-        if resid.any():
-            self._residual(rows[resid], T[resid, 0], XE[resid, 0], OK[resid, 0])
-        a = ~resid
-        if not a.any():
-            return
-        ar, OKa, XEa, Ta, va = rows[a], OK[a], XE[a], T[a], valid[a]
-        res = np.broadcast_to(self.res[ar][:, None, :], (len(ar), self.jm, self.nr))
-        nt = np.max(np.abs(np.take_along_axis(XEa, res, axis=2) - self.tgt[ar][:, None, :]),
-                    axis=2)
-        rn = self.rn[ar][:, None]
-        good = va & OKa & (nt <= (1.0 - self.c[ar][:, None] * self.alpha[ar]) * rn)
-        has = good.any(axis=1)
-        jg = np.argmax(good, axis=1)
-        cols = np.arange(self.jm)[None, :]
-        before = np.where(has[:, None], cols < jg[:, None], True)
-        cand = va & OKa & (nt < rn) & before
-        jb = np.argmin(np.where(cand, nt, np.inf), axis=1)
-        chosen = np.where(has, jg, np.where(cand.any(axis=1), jb, -1))
-        self.n_ev[ar] += np.where(has, jg + 1, self.nal[ar])
-        fail = chosen < 0
-        if fail.any():
-            self._finish(ar[fail], np.zeros(int(fail.sum()), bool),
-                         ["Armijo line search failed to find a productive step."] * int(fail.sum()))
-        m = ~fail
-        if m.any():
-            rr, cc = ar[m], chosen[m]
-            self.x[rr] = self.trial[rr, cc]
-            self.k[rr] += 1
-            self.stage[rr] = self.RESID
-            idx = np.nonzero(m)[0]
-            self._residual(rr, Ta[idx, cc], XEa[idx, cc], OKa[idx, cc])
-
-    def feed_stm(self, rows: np.ndarray, Phi: np.ndarray) -> None:
-        """Phi(t_event) (n, 6, 6) for rows in stage JAC: Jacobian, Newton step,
-        trial steps of the line search."""
-        self.n_stm[rows] += 1
-        Jm = Phi[np.arange(len(rows))[:, None, None], self.res[rows][:, :, None],
-                 self.ctrl[rows][:, None, :]]
-        h = self.halo[rows]
-        if h.any():
-            Jm[h] = Jm[h] - halo_quadratic_terms(self.mu[rows][h], self.x_ev[rows][h], Phi[h])
-        d = solve_delta_stack(Jm, self.r[rows])
-        md = self.max_delta[rows]
-        dn = np.max(np.abs(d), axis=1)
-        cap = ~np.isinf(md) & (dn > md)
-        if cap.any():
-            d[cap] = d[cap] * (md[cap] / dn[cap])[:, None]
-        self.trial[rows] = self.x[rows][:, None, :] + self.alpha[rows][:, :, None] * d[:, None, :]
-        self.stage[rows] = self.ARMIJO
+    def result(self, row: int) -> CorrectionResult:
+        x = np.empty(6)
+        rn, hp = C.c_double(), C.c_double()
+        k, ne, ns, ok, err = (C.c_int() for _ in range(5))
+        self._check(self._l.hh_bank_result(self._h, int(row), _p(x), C.byref(rn), C.byref(k),
+                                           C.byref(ne), C.byref(ns), C.byref(ok), C.byref(err),
+                                           C.byref(hp)), "hh_bank_result")
+        p = self.problems[row]
+        if err.value == 2:
+            msg = (f"Newton did not converge after {p.max_attempts} iterations "
+                   f"(|R|={rn.value:.2e}).")
+        else:
+            msg = _ERRORS.get(err.value, "")
+        return CorrectionResult(bool(ok.value), x, rn.value, k.value, half_period=hp.value,
+                                error=msg, n_event_searches=ne.value, n_stm=ns.value)
 
 
 def _drive(bank: _Bank, eng: "_Engine") -> None:
     """Run every row of a bank to completion, one batched launch per
     request kind per round."""
+    rows_all = np.arange(bank.n)
+    mu, fwd = bank.mu, bank.fwd
     while True:
-        live = np.nonzero(bank.stage[:bank.n] != bank.DONE)[0]
+        st = bank.stages(rows_all)
+        live = rows_all[st != bank.DONE]
         if live.size == 0:
             return
-        ev = live[bank.stage[live] != bank.JAC]
-        st = live[bank.stage[live] == bank.JAC]
-        for f in np.unique(bank.fwd[ev]):
-            rows = ev[bank.fwd[ev] == f]
+        sl = st[st != bank.DONE]
+        ev = live[sl != bank.JAC]
+        jac = live[sl == bank.JAC]
+        for f in np.unique(fwd[ev]):
+            rows = ev[fwd[ev] == f]
             states, counts = bank.event_inputs(rows)
-            mu = np.repeat(bank.mu[rows], counts)
-            t, xe, ok = eng.event_batch(mu, states, int(f))
+            t, xe, ok = eng.event_batch(np.repeat(mu[rows], counts), states, int(f))
             bank.feed_events(rows, t, xe, ok)
-        if st.size:
-            states, tev = bank.stm_inputs(st)
-            bank.feed_stm(st, eng.stm_batch(bank.mu[st], states, tev))
+        if jac.size:
+            states, tev = bank.stm_inputs(jac)
+            bank.feed_stm(jac, eng.stm_batch(mu[jac], states, tev))
 
 
 def correct_batch(problems: Sequence[CorrectionProblem], engine: _Engine | None = None,
@@ -455,7 +371,7 @@ def correct_batch(problems: Sequence[CorrectionProblem], engine: _Engine | None 
         where.append((b, b.add(p)))
     for b in banks.values():
         _drive(b, eng)
-    out = [b.results[i] for b, i in where]
+    out = [b.result(i) for b, i in where]
     if not compute_half_period:
         for r in out:
             r.half_period = math.nan

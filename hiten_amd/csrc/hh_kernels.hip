@@ -1104,4 +1104,39 @@ int hh_traj_filters(int n, int m, const double* mu, const double* states, double
     return check_launch("hh_traj_filters launch");
 }
 
+/* ---- stream-ordered copies and events for host loops ---------------------- */
+
+int hh_memcpy_async(void* dst, const void* src, unsigned long long bytes, int to_device,
+                    void* stream) {
+    if (bytes == 0) return HH_OK;
+    if (!dst || !src) return fail(HH_ERR_ARGS, "hh_memcpy_async: null pointer");
+    const hipError_t e = hipMemcpyAsync(dst, src, (size_t)bytes,
+                                        to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost,
+                                        (hipStream_t)stream);
+    return e == hipSuccess ? HH_OK : hip_fail(e, "hipMemcpyAsync");
+}
+
+void* hh_event_create(void) {
+    hipEvent_t ev = nullptr;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return nullptr;
+    return (void*)ev;
+}
+
+void hh_event_destroy(void* ev) {
+    if (ev) (void)hipEventDestroy((hipEvent_t)ev);
+}
+
+int hh_event_record(void* ev, void* stream) {
+    const hipError_t e = hipEventRecord((hipEvent_t)ev, (hipStream_t)stream);
+    return e == hipSuccess ? HH_OK : hip_fail(e, "hipEventRecord");
+}
+
+int hh_event_query(void* ev) {
+    const hipError_t e = hipEventQuery((hipEvent_t)ev);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    hip_fail(e, "hipEventQuery");
+    return -HH_ERR_HIP;
+}
+
 }  /* extern "C" */

@@ -82,103 +82,172 @@ class Job:
         return self.finished - self.arrival
 
 
-def _pinned(t: torch.Tensor) -> torch.Tensor:
-    """Asynchronous device -> pinned host copy on the current stream."""
-    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
-    h.copy_(t, non_blocking=True)
-    return h
+def _isclose0(v: float) -> bool:
+    """numpy.isclose(0.0, v) for a scalar (atol 1e-8, rtol 1e-5)."""
+    return abs(0.0 - v) <= 1e-8 + 1e-5 * abs(v)
 
 
-def _up(a) -> torch.Tensor:
-    """Asynchronous host -> device copy through pinned memory on the current
-    stream (a pageable copy would wait for the stream's earlier work)."""
-    h = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).pin_memory()
-    return h.to(batch._device(), non_blocking=True)
+class _Slot:
+    """Reusable staging for one batch in flight: a stream, a completion event,
+    pinned host and device buffers that grow to the largest batch seen.  A
+    slot is handed to a new batch only after its previous batch completed, so
+    its buffers are never overwritten while a copy or kernel still uses them.
+    Copies and events go straight to the HIP runtime through the library."""
+
+    def __init__(self, stream: torch.cuda.Stream):
+        self.lib = _lib.load()
+        self.stream = stream
+        self.sh = int(stream.cuda_stream)
+        self.ev = self.lib.hh_event_create()
+        if not self.ev:
+            raise _lib.HipLibraryError("hh_event_create failed")
+        self.busy = False
+        self.nf_in = self.nf_out = self.ni_out = 0
+
+    def __del__(self):
+        ev, self.ev = getattr(self, "ev", None), None
+        if ev:
+            self.lib.hh_event_destroy(ev)
+
+    def record(self) -> None:
+        _lib.check(self.lib.hh_event_record(self.ev, self.sh), "hh_event_record")
+
+    def ready(self) -> bool:
+        r = self.lib.hh_event_query(self.ev)
+        if r < 0:
+            _lib.check(-r, "hh_event_query")
+        return r == 1
+
+    def copy(self, dst: int, src: int, nbytes: int, to_device: bool) -> None:
+        _lib.check(self.lib.hh_memcpy_async(dst, src, nbytes, int(to_device), self.sh),
+                   "hh_memcpy_async")
+
+    @staticmethod
+    def _grow(n_have, n_need, dtype, pinned):
+        n = max(n_need, 2 * n_have)
+        if pinned:
+            return n, torch.empty(n, dtype=dtype, pin_memory=True)
+        return n, torch.empty(n, dtype=dtype, device=batch._device())
+
+    def reserve(self, nf_in: int, nf_out: int, ni_out: int) -> None:
+        if nf_in > self.nf_in:
+            n, self.hin = self._grow(self.nf_in, nf_in, torch.float64, True)
+            _, self.din = self._grow(self.nf_in, nf_in, torch.float64, False)
+            self.hin_np = self.hin.numpy()
+            self.nf_in = n
+        if nf_out > self.nf_out:
+            n, self.hf = self._grow(self.nf_out, nf_out, torch.float64, True)
+            _, self.fout = self._grow(self.nf_out, nf_out, torch.float64, False)
+            self.hf_np = self.hf.numpy()
+            self.nf_out = n
+        if ni_out > self.ni_out:
+            n, self.hi = self._grow(self.ni_out, ni_out, torch.int32, True)
+            _, self.iout = self._grow(self.ni_out, ni_out, torch.int32, False)
+            self.hi_np = self.hi.numpy()
+            self.ni_out = n
 
 
 class _Packed:
     """Inputs of one launch in one pinned upload and its outputs in one device
-    block per dtype: float64 rows ``fin`` are uploaded as a (rows, k) array;
-    ``fout`` float64 and ``iout`` int32 rows are allocated on the device and
-    copied back together once the launch is enqueued."""
+    block per dtype, all in a slot's buffers: float64 rows ``fin`` (rows, k)
+    go up in one copy; ``fout`` float64 and ``iout`` int32 rows come back in
+    one copy each once the launch is enqueued (on the slot's stream)."""
 
-    def __init__(self, fin: np.ndarray, n_fout: int, n_iout: int):
-        self.k = fin.shape[1]
-        self.din = _up(fin)
-        dev = self.din.device
-        self.fout = torch.empty((n_fout, self.k), dtype=torch.float64, device=dev)
-        self.iout = torch.empty((n_iout, self.k), dtype=torch.int32, device=dev)
+    def __init__(self, fin: np.ndarray, n_fout: int, n_iout: int, slot: _Slot):
+        r, k = fin.shape
+        self.k, self.n_fout, self.n_iout = k, n_fout, n_iout
+        slot.reserve(r * k, n_fout * k, n_iout * k)
+        self.slot = slot
+        slot.hin_np[:r * k] = fin.reshape(-1)
+        slot.copy(slot.din.data_ptr(), slot.hin.data_ptr(), r * k * 8, True)
+        self.din_ptr = slot.din.data_ptr()
+        self.fout_ptr = slot.fout.data_ptr()
+        self.iout_ptr = slot.iout.data_ptr()
 
-    @staticmethod
-    def ptr(t: torch.Tensor) -> int:
-        return t.data_ptr()
+    def din(self, row: int) -> int:
+        """device address of input row ``row``"""
+        return self.din_ptr + row * self.k * 8
+
+    def fout(self, row: int) -> int:
+        return self.fout_ptr + row * self.k * 8
+
+    def iout(self, row: int) -> int:
+        return self.iout_ptr + row * self.k * 4
 
     def download(self, frows=None):
-        f = self.fout if frows is None else self.fout[frows]
-        self.hf = _pinned(f)
-        self.hi = _pinned(self.iout)
+        """Copy fout rows [frows] and all iout rows to the slot's pinned
+        buffers; hf/hi are numpy views of them (valid once the event fired)."""
+        a, b = (0, self.n_fout) if frows is None else (frows.start, frows.stop)
+        k, sl = self.k, self.slot
+        nf, ni = (b - a) * k, self.n_iout * k
+        sl.copy(sl.hf.data_ptr(), self.fout(a), nf * 8, False)
+        sl.copy(sl.hi.data_ptr(), self.iout_ptr, ni * 4, False)
+        self.hf = sl.hf_np[:nf].reshape(b - a, k)
+        self.hi = sl.hi_np[:ni].reshape(self.n_iout, k)
 
 
-def _launch_event(pk: _Packed, y_dev: torch.Tensor, stream, *, ev_index: int, ev_offset: float,
-                  direction: int, xtol: float, gtol: float, rtol: float, atol: float) -> None:
-    """hh_dop853_event on packed buffers: din rows [mu, t0, tmax, ...], y_dev
-    (6, k) SoA start states; fout rows [t_hit, y_hit (6), y_last (6)], iout
-    rows [hit, status, n_accepted]."""
+def _launch_event(pk: _Packed, y_row: int, stream_handle: int, *, ev_index: int,
+                  ev_offset: float, direction: int, xtol: float, gtol: float, rtol: float,
+                  atol: float, y_addr: Optional[int] = None) -> None:
+    """hh_dop853_event on packed buffers: din rows [mu, t0, tmax, ...], start
+    states at din row ``y_row`` (6 rows, SoA) or at ``y_addr``; fout rows
+    [t_hit, y_hit (6), y_last (6)], iout rows [hit, status, n_accepted]."""
     lib = _lib.load()
-    P = _Packed.ptr
-    rc = lib.hh_dop853_event(pk.k, P(pk.din[0]), P(pk.din[1]), P(pk.din[2]), float(rtol),
+    y = pk.din(y_row) if y_addr is None else y_addr
+    rc = lib.hh_dop853_event(pk.k, pk.din(0), pk.din(1), pk.din(2), float(rtol),
                              float(atol), math.inf, batch.MIN_STEP, batch.DEFAULT_MAX_STEPS,
                              int(ev_index), float(ev_offset), int(direction), float(xtol),
-                             float(gtol), 1, 0, P(y_dev), P(pk.iout[0]), P(pk.fout[0]),
-                             P(pk.fout[1]), P(pk.fout[7]), P(pk.iout[1]), P(pk.iout[2]), None, 0,
-                             int(stream.cuda_stream))
+                             float(gtol), 1, 0, y, pk.iout(0), pk.fout(0), pk.fout(1), pk.fout(7),
+                             pk.iout(1), pk.iout(2), None, 0, stream_handle)
     _lib.check(rc, "hh_dop853_event")
 
 
-def _launch_final(pk: _Packed, dim: int, y_dev: torch.Tensor, y_out: torch.Tensor, stream, *,
-                  forward: int, flip_indices, variant: int, max_step: float,
+def _launch_final(pk: _Packed, dim: int, rows3, y_addr: int, out_addr: int, stream_handle: int, *,
+                  forward: int, flip_mask: int, variant: int, max_step: float,
                   istatus: int, inacc: int) -> None:
-    """hh_dop853_final on packed buffers: din rows [mu, t0, tf, ...]."""
+    """hh_dop853_final on packed buffers: mu, t0, tf at din rows ``rows3``."""
     lib = _lib.load()
-    P = _Packed.ptr
-    rc = lib.hh_dop853_final(pk.k, dim, P(pk.din[0]), P(pk.din[1]), P(pk.din[2]), 1e-12, 1e-12,
+    r0, r1, r2 = rows3
+    rc = lib.hh_dop853_final(pk.k, dim, pk.din(r0), pk.din(r1), pk.din(r2), 1e-12, 1e-12,
                              float(max_step), batch.MIN_STEP, batch.DEFAULT_MAX_STEPS,
-                             int(forward), batch.flip_mask(dim, flip_indices), P(y_dev), P(y_out),
-                             P(pk.iout[istatus]), P(pk.iout[inacc]), None, int(variant), 0, 0,
-                             None, None, int(stream.cuda_stream))
+                             int(forward), flip_mask, y_addr, out_addr, pk.iout(istatus),
+                             pk.iout(inacc), None, int(variant), 0, 0, None, None, stream_handle)
     _lib.check(rc, "hh_dop853_final")
 
 
-class _Pending:
-    """Device work on one stream whose host copies are complete once
-    ``done`` has been recorded and has fired."""
+_FLIP_STM = batch.flip_mask(42, slice(36, 42))
+_EYE36 = np.eye(6).ravel()
 
-    def __init__(self, stream: torch.cuda.Stream):
-        self.stream = stream
-        self.done = torch.cuda.Event()
+
+class _Pending:
+    """Device work on a slot's stream whose host copies are complete once the
+    slot's event, recorded after them, has fired."""
+
+    def __init__(self, slot: _Slot):
+        self.slot = slot
+        self.stream = slot.stream
+        slot.busy = True
 
     def ready(self) -> bool:
-        return self.done.query()
+        return self.slot.ready()
 
 
 class _FanLaunch(_Pending):
-    def __init__(self, job: Job, stream):
-        super().__init__(stream)
+    def __init__(self, job: Job, slot: _Slot):
+        super().__init__(slot)
         self.job = job
         r: FanRequest = job.request
         k = r.size
         fin = np.empty((9, k))
         fin[0], fin[1], fin[2], fin[3:9] = r.mu, 0.0, r.tmax, r.y0.T
-        with torch.cuda.stream(stream):
-            self.pk = _Packed(fin, 13, 3)
-            _launch_event(self.pk, self.pk.din[3], stream, ev_index=r.ev_index,
-                          ev_offset=r.ev_offset, direction=r.direction, xtol=r.xtol, gtol=r.gtol,
-                          rtol=r.rtol, atol=r.atol)
-            self.pk.download(slice(0, 7))
-            self.done.record(stream)
+        self.pk = _Packed(fin, 13, 3, slot)
+        _launch_event(self.pk, 3, slot.sh, ev_index=r.ev_index, ev_offset=r.ev_offset,
+                      direction=r.direction, xtol=r.xtol, gtol=r.gtol, rtol=r.rtol, atol=r.atol)
+        self.pk.download(slice(0, 7))
+        slot.record()
 
     def finish(self, server: "Server") -> None:
-        f, i = self.pk.hf.numpy(), self.pk.hi.numpy()
+        f, i = self.pk.hf, self.pk.hi
         self.job.result = FanResult(i[0].astype(bool), f[0].copy(), f[1:7].T.copy(),
                                     i[2].copy())
         server.steps += int(i[2].sum())
@@ -191,8 +260,8 @@ class _EventBatch(_Pending):
     then alignment and the fallback window for the states without a hit
     (poincare/singlehit/backend.py:237-280)."""
 
-    def __init__(self, bank: _Bank, rows: np.ndarray, forward: int, stream):
-        super().__init__(stream)
+    def __init__(self, bank: _Bank, rows: np.ndarray, forward: int, slot: _Slot):
+        super().__init__(slot)
         self.bank, self.rows, self.forward = bank, rows, forward
         self.states, counts = bank.event_inputs(rows)
         self.mu = np.repeat(bank.mu[rows], counts)
@@ -207,31 +276,31 @@ class _EventBatch(_Pending):
     def _launch(self, idx, ts, span):
         self.idx = idx
         self.cur_ts, self.cur_span = ts, span
-        self.aligned = not np.isclose(0.0, ts)
-        self.searched = not np.isclose(0.0, span)
+        self.aligned = not _isclose0(ts)
+        self.searched = not _isclose0(span)
         k = len(idx)
+        # din rows: mu, t0 (0), span, alignment time, state (6)
         fin = np.empty((10, k))
         fin[0], fin[1], fin[2], fin[3], fin[4:10] = self.mu[idx], 0.0, span, ts, self.states[idx].T
-        with torch.cuda.stream(self.stream):
-            # fout rows: t_hit, y_hit (6), y_last (6), aligned state (6);
-            # iout rows: hit, status, n_accepted (event), status, n_accepted (alignment)
-            pk = self.pk = _Packed(fin, 19, 5)
-            y_dev = pk.din[4:10]
-            if self.aligned:
-                pk_al = _AlignView(pk)
-                _launch_final(pk_al, 6, y_dev, pk.fout[13:19], self.stream,
-                              forward=self.forward, flip_indices=None,
-                              variant=_lib.HH_VARIANT_PLAIN, max_step=1e4, istatus=3, inacc=4)
-                y_dev = pk.fout[13:19]
-            if self.searched:
-                _launch_event(pk, y_dev, self.stream, ev_index=Y_INDEX, ev_offset=0.0,
-                              direction=0, xtol=1e-12, gtol=1e-12, rtol=1e-12, atol=1e-12)
-            pk.download(slice(0, 13))
-            self.done.record(self.stream)
+        sl = self.slot
+        # fout rows: t_hit, y_hit (6), y_last (6), aligned state (6);
+        # iout rows: hit, status, n_accepted (event), status, n_accepted (alignment)
+        pk = self.pk = _Packed(fin, 19, 5, sl)
+        y_addr = pk.din(4)
+        if self.aligned:
+            _launch_final(pk, 6, (0, 1, 3), y_addr, pk.fout(13), sl.sh, forward=self.forward,
+                          flip_mask=batch.flip_mask(6, None), variant=_lib.HH_VARIANT_PLAIN,
+                          max_step=1e4, istatus=3, inacc=4)
+            y_addr = pk.fout(13)
+        if self.searched:
+            _launch_event(pk, 0, sl.sh, ev_index=Y_INDEX, ev_offset=0.0, direction=0, xtol=1e-12,
+                          gtol=1e-12, rtol=1e-12, atol=1e-12, y_addr=y_addr)
+        pk.download(slice(0, 13))
+        sl.record()
 
     def finish(self, server: "Server") -> bool:
         """Consume the finished stage; returns True when the batch is complete."""
-        f, i = self.pk.hf.numpy(), self.pk.hi.numpy()
+        f, i = self.pk.hf, self.pk.hi
         if self.aligned:
             server.steps += int(i[4].sum())
         if self.searched:
@@ -248,7 +317,6 @@ class _EventBatch(_Pending):
         miss = np.nonzero(~self.ok)[0]
         if self.stage == 1 and miss.size:
             self.stage = 2
-            self.done = torch.cuda.Event()
             self._launch(miss, self.ts2, self.span2)
             return False
         self.bank.feed_events(self.rows, self.t, self.xe, self.ok)
@@ -256,52 +324,40 @@ class _EventBatch(_Pending):
         return True
 
 
-class _AlignView:
-    """The alignment launch reads mu from row 0, t0 (zeros) from row 1 and the
-    alignment time from row 3 of an event batch's packed input."""
-
-    def __init__(self, pk: _Packed):
-        self.k = pk.k
-        self.din = (pk.din[0], pk.din[1], pk.din[3])
-        self.iout = pk.iout
-
-
 class _StmBatch(_Pending):
     """Phi(t_event) of _compute_stm(var_dynsys, x, t_event, steps=500)
     (corrector/operators.py:298-307) for bank rows waiting for their
     Jacobian, one wave per trajectory."""
 
-    def __init__(self, bank: _Bank, rows: np.ndarray, stream):
-        super().__init__(stream)
+    def __init__(self, bank: _Bank, rows: np.ndarray, slot: _Slot):
+        super().__init__(slot)
         self.bank, self.rows = bank, rows
-        n = len(rows)
         states, self.tev = bank.stm_inputs(rows)
-        self.live = ~np.isclose(0.0, self.tev)
-        self.h = None
+        tev = self.tev
+        self.live = ~(np.abs(0.0 - tev) <= 1e-8 + 1e-5 * np.abs(tev))   # numpy.isclose(0, t)
+        self.pk = None
         if self.live.any():
-            PHI0 = np.zeros((n, 42))
-            PHI0[:, :36] = np.eye(6).ravel()
-            PHI0[:, 36:] = states
-            mu = bank.mu[rows]
             k = int(self.live.sum())
+            # din rows: mu, t0 (0), t_event, PHI0 (42: identity STM, state)
             fin = np.empty((45, k))
-            fin[0], fin[1], fin[2], fin[3:45] = mu[self.live], 0.0, self.tev[self.live], \
-                PHI0[self.live].T
-            with torch.cuda.stream(stream):
-                pk = self.h = _Packed(fin, 42, 2)
-                _launch_final(pk, 42, pk.din[3:45], pk.fout, stream, forward=1,
-                              flip_indices=slice(36, 42), variant=_lib.HH_VARIANT_WIDE,
-                              max_step=1e4, istatus=0, inacc=1)
-                pk.download(slice(0, 36))
-        self.done.record(stream)
+            fin[0], fin[1], fin[2] = bank.mu[rows][self.live], 0.0, tev[self.live]
+            fin[3:39] = _EYE36[:, None]
+            fin[39:45] = states[self.live].T
+            pk = self.pk = _Packed(fin, 42, 2, slot)
+            _launch_final(pk, 42, (0, 1, 2), pk.din(3), pk.fout(0), slot.sh, forward=1,
+                          flip_mask=_FLIP_STM, variant=_lib.HH_VARIANT_WIDE, max_step=1e4,
+                          istatus=0, inacc=1)
+            pk.download(slice(0, 36))
+        slot.record()
 
     def finish(self, server: "Server") -> bool:
-        phi = np.repeat(np.eye(6)[None], len(self.rows), axis=0)
-        if self.h is not None:
-            y, na = self.h.hf.numpy().T, self.h.hi.numpy()[1]
-            phi[self.live] = y.reshape(-1, 6, 6)
-            server.steps += int(na.sum())
-        self.bank.feed_stm(self.rows, phi)
+        n = len(self.rows)
+        phi = np.empty((n, 36))
+        phi[:] = _EYE36
+        if self.pk is not None:
+            phi[self.live] = self.pk.hf.T
+            server.steps += int(self.pk.hi[1].sum())
+        self.bank.feed_stm(self.rows, phi.reshape(n, 6, 6))
         server._advance(self.bank, self.rows)
         return True
 
@@ -309,31 +365,29 @@ class _StmBatch(_Pending):
 class Server:
     """Continuous-batching loop; see the module docstring.
 
-    ``hbm_budget`` bounds the device bytes reserved by admitted jobs; each
-    launch goes to one of ``n_streams`` streams (default priority) that has
-    no batch in flight when there is one.
+    ``hbm_budget`` bounds the device bytes reserved by admitted jobs.  Each
+    batch in flight holds a staging slot (stream, event, reusable pinned and
+    device buffers); slots are spread over ``n_streams`` default-priority
+    streams.
     """
 
     CORRECTION_BYTES = 64 * 1024          # event batch rows (<= 15 per iteration) and STM
 
-    def __init__(self, hbm_budget: int = 32 << 30, n_streams: int = 6):
+    def __init__(self, hbm_budget: int = 32 << 30, n_streams: int = 8):
         self.hbm_budget = int(hbm_budget)
         self.streams = [torch.cuda.Stream() for _ in range(max(2, n_streams))]
-        self._rr = 0
+        self.slots: List[_Slot] = []
         self.reset()
 
-    def _stream(self) -> torch.cuda.Stream:
-        """A stream with no batch in flight if there is one (so that the new
-        batch does not queue behind an unrelated one), else the next in turn."""
-        busy = {id(b.stream) for b in self.inflight}
-        for k in range(len(self.streams)):
-            s = self.streams[(self._rr + k) % len(self.streams)]
-            if id(s) not in busy:
-                self._rr = (self._rr + k + 1) % len(self.streams)
-                return s
-        s = self.streams[self._rr]
-        self._rr = (self._rr + 1) % len(self.streams)
-        return s
+    def _slot(self) -> _Slot:
+        """A slot whose previous batch has completed; a new one (on the next
+        stream in turn) if all are in flight."""
+        for sl in self.slots:
+            if not sl.busy:
+                return sl
+        sl = _Slot(self.streams[len(self.slots) % len(self.streams)])
+        self.slots.append(sl)
+        return sl
 
     def reset(self):
         self.used = 0
@@ -374,10 +428,11 @@ class Server:
     def _advance(self, bank: _Bank, rows: np.ndarray) -> None:
         """Rows whose batch completed: finished jobs out, the rest ready."""
         key = (bank.nr, bank.nc)
-        done = bank.stage[rows] == bank.DONE
+        done = bank.stages(rows) == bank.DONE
         for i in rows[done]:
             job = self.owner.pop((key, int(i)))
-            job.result = bank.results[i]
+            job.result = bank.result(int(i))
+            bank.release(int(i))
             self._complete(job)
         if (~done).any():
             self.ready.setdefault(key, []).extend(rows[~done].tolist())
@@ -389,7 +444,7 @@ class Server:
             self.peak_used = max(self.peak_used, self.used)
             job.admitted = self.now()
             if job.kind == "fan":
-                self.inflight.append(_FanLaunch(job, self._stream()))
+                self.inflight.append(_FanLaunch(job, self._slot()))
             else:
                 p = job.request
                 key = (len(p.residual_indices), len(p.control_indices))
@@ -408,14 +463,14 @@ class Server:
             bank = self.banks[key]
             rows = np.array(rows, dtype=int)
             self.ready[key] = []
-            st = bank.stage[rows]
+            st = bank.stages(rows)
             ev = rows[st != bank.JAC]
             for f in np.unique(bank.fwd[ev]):
                 self.inflight.append(_EventBatch(bank, ev[bank.fwd[ev] == f], int(f),
-                                                 self._stream()))
+                                                 self._slot()))
             jac = rows[st == bank.JAC]
             if jac.size:
-                self.inflight.append(_StmBatch(bank, jac, self._stream()))
+                self.inflight.append(_StmBatch(bank, jac, self._slot()))
 
     def _poll(self) -> bool:
         """Collect finished batches.  Work on one stream completes in launch
@@ -432,9 +487,12 @@ class Server:
             progressed = True
             if isinstance(b, _FanLaunch):
                 b.finish(self)
+                b.slot.busy = False
             elif not b.finish(self):
                 relaunched.append(b)       # its next stage is now last on its stream
                 seen.add(sid)
+            else:
+                b.slot.busy = False
         self.inflight = keep + relaunched
         return progressed
 

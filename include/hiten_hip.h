@@ -156,6 +156,64 @@ int hh_eval_math(int n, const double* x, const double* y, double* out, void* str
 int hh_traj_filters(int n, int m, const double* mu, const double* states, double* min_r1,
                     double* min_r2, double* max_energy_err, void* stream);
 
+/* Stream-ordered plumbing for host loops that keep many small batches in
+ * flight (the serving loop): an asynchronous copy between pinned host and
+ * device memory, and completion events (created without timing).
+ * hh_event_query returns 1 when the event has fired, 0 if not yet, < 0 on
+ * error.  No counterpart in the reference.                                 */
+int hh_memcpy_async(void* dst, const void* src, unsigned long long bytes, int to_device,
+                    void* stream);
+void* hh_event_create(void);
+void hh_event_destroy(void* event);
+int hh_event_record(void* event, void* stream);
+int hh_event_query(void* event);
+
+/* ---- correction bank (host code) ---------------------------------------
+ * Batched single-shooting corrections as a native state machine: the
+ * reference's _NewtonBackend.run (corrector/backends/newton.py:39-143) with
+ * _ArmijoLineSearch (stepping/armijo.py:58-164), the residual/Jacobian
+ * operators (corrector/operators.py:319-452) and the Halo extra term
+ * (services/orbits.py:917-947).  The caller runs the event searches and STM
+ * propagations on the GPU and the dense solve (corrector/backends/base.py:
+ * 187-245) with NumPy, and feeds the results back.  Per row the stages are
+ * event search (residual or all line-search trials) -> STM -> solve ->
+ * event search ...  hh_bank_add returns the row, or -HH_ERR_ARGS.          */
+#define HH_BANK_CONVERGED 0
+#define HH_BANK_NO_CROSSING 1
+#define HH_BANK_MAX_ITERATIONS 2
+#define HH_BANK_LINE_SEARCH_FAILED 3
+#define HH_BANK_RESID 0
+#define HH_BANK_JAC 1
+#define HH_BANK_STEP 2
+#define HH_BANK_ARMIJO 3
+#define HH_BANK_DONE 4
+void* hh_bank_create(int n_residuals, int n_controls);
+void hh_bank_destroy(void* bank);
+int hh_bank_size(void* bank);
+int hh_bank_add(void* bank, double mu, const double* x0, const int* residual_indices,
+                const int* control_indices, const double* target, int halo_term, int forward,
+                int max_attempts, double tol, double max_delta, double alpha_reduction,
+                double min_alpha, double armijo_c);
+int hh_bank_stages(void* bank, int n, const int* rows, int* stage);
+/* states whose event searches the rows wait for, grouped by row ([total][6]);
+ * with states == NULL only counts[n] and the total are returned            */
+int hh_bank_event_inputs(void* bank, int n, const int* rows, int* counts, double* states,
+                         int capacity);
+/* t[total], xe[total][6], ok[total] of those searches                        */
+int hh_bank_feed_events(void* bank, int n, const int* rows, const double* t, const double* xe,
+                        const int* ok);
+int hh_bank_stm_inputs(void* bank, int n, const int* rows, double* states, double* t_event);
+/* phi[n][36] (row-major Phi at the event time) -> J[n][nr][nc], r[n][nr]    */
+int hh_bank_jacobians(void* bank, int n, const int* rows, const double* phi, double* J,
+                      double* r);
+/* Newton steps delta[n][nc] -> capped, trial points, line-search stage      */
+int hh_bank_set_steps(void* bank, int n, const int* rows, const double* delta);
+int hh_bank_result(void* bank, int row, double* x_full, double* residual_norm, int* iterations,
+                   int* n_event_searches, int* n_stm, int* converged, int* error,
+                   double* half_period);
+/* a finished row whose result has been read; hh_bank_add reuses it         */
+int hh_bank_release(void* bank, int row);
+
 #ifdef __cplusplus
 }
 #endif
