@@ -50,7 +50,7 @@ _SIGNATURES = {
     "hh_dop853_grid": ([_I, _I, _P, _P, _P, _I, _P, _D, _D, _D, _D, _LL, _I, _ULL, _P, _P, _P, _P,
                         _P, _I, _P], _I),
     "hh_dop853_event": ([_I, _P, _P, _P, _D, _D, _D, _D, _LL, _I, _D, _I, _D, _D, _I, _ULL, _P,
-                         _P, _P, _P, _P, _P, _P, _P, _I, _P], _I),
+                         _P, _P, _P, _P, _P, _P, _P, _I, _I, _P], _I),
     "hh_eval_rhs": ([_I, _I, _P, _P, _P, _P], _I),
     "hh_eval_math": ([_I, _P, _P, _P, _P], _I),
     "hh_traj_filters": ([_I, _I, _P, _P, _P, _P, _P, _P], _I),

@@ -267,9 +267,14 @@ def dop853_event(mu, y0, t0, tmax, *, ev_index: int = 1, ev_offset: float = 0.0,
                  direction: int = 0, rtol: float = 1e-12, atol: float = 1e-12,
                  max_step: float = math.inf, min_step: float = MIN_STEP,
                  max_steps: int = DEFAULT_MAX_STEPS, xtol: float = 1e-12, gtol: float = 1e-12,
-                 forward: int = 1, flip_indices=None, block: int = 0,
+                 forward: int = 1, flip_indices=None, variant: str = "auto", block: int = 0,
                  stream=None) -> EventResult:
-    """DOP853 until the first crossing of y[ev_index] = ev_offset (rk.py:2678-2803)."""
+    """DOP853 until the first crossing of y[ev_index] = ev_offset (rk.py:2678-2803).
+
+    ``variant`` "plain" integrates one trajectory per lane (throughput);
+    "wide" one per 8-lane group, one component per lane (about 2.5x shorter
+    per trajectory; faster overall up to tens of thousands of trajectories);
+    "auto" picks "wide" up to 65,536 trajectories.  All give the same bits."""
     lib = _lib.load()
     y = _soa(y0, 6)
     n = y.shape[1]
@@ -280,11 +285,14 @@ def dop853_event(mu, y0, t0, tmax, *, ev_index: int = 1, ev_offset: float = 0.0,
     yh = torch.empty_like(y)
     yl = torch.empty_like(y)
     status, nacc, nrej = torch.empty(n, **i32), torch.empty(n, **i32), torch.empty(n, **i32)
+    if variant == "auto":
+        variant = "wide" if n <= 65536 else "plain"
     rc = lib.hh_dop853_event(n, _ptr(mu_t), _ptr(t0_t), _ptr(tm_t), float(rtol), float(atol),
                              float(max_step), float(min_step), int(max_steps), int(ev_index),
                              float(ev_offset), int(direction), float(xtol), float(gtol),
                              int(forward), flip_mask(6, flip_indices), _ptr(y), _ptr(hit),
                              _ptr(th), _ptr(yh), _ptr(yl), _ptr(status), _ptr(nacc), _ptr(nrej),
+                             {"plain": _lib.HH_VARIANT_PLAIN, "wide": _lib.HH_VARIANT_WIDE}[variant],
                              int(block), _lib.stream_handle(stream))
     _lib.check(rc, "hh_dop853_event")
     return EventResult(hit.bool(), th, yh.t(), yl.t(), status, nacc, nrej)

@@ -18,6 +18,7 @@
  */
 #include "hh_device.h"
 #include "hh_wave.h"
+#include "hh_group6.h"
 #include "../../include/hiten_hip.h"
 
 #include <stdio.h>
@@ -452,6 +453,97 @@ __global__ __launch_bounds__(256) void k_dop853_event6(
     if (status) status[i] = st;
     if (nacc) nacc[i] = na;
     if (nrej) nrej[i] = nr;
+}
+
+/* k_dop853_event6 with one trajectory per aligned 8-lane group, one
+ * component per lane (Group6): the latency form for small batches.  The
+ * loop is the one above, statement for statement; the event component is
+ * read from its lane of the group.                                          */
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_dop853_event6_group(
+    int n, const double* __restrict__ mu, const double* __restrict__ t0a,
+    const double* __restrict__ tmaxa, StepParams sp, EventParams ev, int fwd,
+    unsigned long long flip, const double* __restrict__ y0, int* __restrict__ hit,
+    double* __restrict__ t_hit, double* __restrict__ y_hit, double* __restrict__ y_last,
+    int* __restrict__ status, int* __restrict__ nacc, int* __restrict__ nrej) {
+    const long long gtid = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    const int i = (int)(gtid >> 3);
+    if (i >= n) return;
+    Group6 p;
+    p.setup(threadIdx.x & 63, mu[i], fwd, flip);
+    const int d = p.d;
+    double y = p.act ? y0[(size_t)d * n + i] : 0.0;
+    double k[DOP_K];
+#pragma unroll
+    for (int j = 0; j < DOP_K; ++j) k[j] = 0.0;
+    double t = t0a[i];
+    const double tmax = tmaxa[i];
+    k[0] = p.rhs(y);
+    const int ev_addr = p.base + (ev.index << 2);
+    double gp = bperm(y, ev_addr) - ev.offset;
+    double h = p.initial_step(y, k[0], sp.rtol, sp.atol, sp.min_step, sp.max_step);
+    double err_prev = -1.0;
+    int na = 0, nr = 0, st = ST_OK, found = 0;
+    double th = 0.0, yh_out = 0.0, ylast = 0.0;
+    while ((t - tmax) * 1.0 < 0.0) {
+        if (na + nr >= sp.max_steps) {
+            st = ST_MAXSTEPS;
+            break;
+        }
+        h = clamp_step(h, sp.max_step, sp.min_step);
+        h = adjust_step_to_endpoint(t, h, tmax);
+        double yh;
+        const double err = p.attempt(y, k, h, sp.rtol, sp.atol, yh);
+        if (err != err) {
+            st = ST_NONFINITE;
+            break;
+        }
+        double fa, fr;
+        p.step_factors(err, err_prev, fa, fr);
+        if (err <= 1.0) {
+            const double t_new = t + h;
+            const double gn = bperm(yh, ev_addr) - ev.offset;
+            na += 1;
+            if (event_crossed(gp, gn, ev.direction)) {
+                double row[DOP_P], fc[DOP_P];
+                p.dense_rows(y, yh, k, h, row);
+#pragma unroll
+                for (int r = 0; r < DOP_P; ++r) fc[r] = bperm(row[r], ev_addr);
+                const double x = refine_bisect(bperm(y, ev_addr), fc, ev.offset, ev.direction, h,
+                                               ev.xtol, ev.gtol);
+                th = t + x * h;
+                yh_out = dense_from_rows(row, y, x);
+                ylast = yh;
+                found = 1;
+                break;
+            }
+            t = t_new;
+            y = yh;
+            k[0] = k[DOP_S];
+            gp = gn;
+            h = h * fa;
+            err_prev = err;
+        } else {
+            h = clamp_step(h * fr, sp.max_step, sp.min_step);
+            nr += 1;
+        }
+    }
+    if (!found) {
+        th = t;
+        yh_out = y;
+        ylast = y;
+    }
+    if (p.act) {
+        y_hit[(size_t)d * n + i] = yh_out;
+        if (y_last) y_last[(size_t)d * n + i] = ylast;
+    }
+    if (d == 0) {
+        hit[i] = found;
+        t_hit[i] = th;
+        if (status) status[i] = st;
+        if (nacc) nacc[i] = na;
+        if (nrej) nrej[i] = nr;
+    }
 }
 
 /* ---- 42-d, one trajectory per 8-lane group ------------------------------ */
@@ -1053,7 +1145,7 @@ int hh_dop853_event(int n, const double* mu, const double* t0, const double* tma
                     int ev_index, double ev_offset, int direction, double xtol, double gtol,
                     int fwd, unsigned long long flip_mask, const double* y0, int* hit,
                     double* t_hit, double* y_hit, double* y_last, int* status, int* n_acc,
-                    int* n_rej, int block, void* stream) {
+                    int* n_rej, int variant, int block, void* stream) {
     if (n < 0) return fail(HH_ERR_ARGS, "hh_dop853_event: n < 0");
     if (n == 0) return HH_OK;
     if (!mu || !t0 || !tmax || !y0 || !hit || !t_hit || !y_hit)
@@ -1067,9 +1159,21 @@ int hh_dop853_event(int n, const double* mu, const double* t0, const double* tma
     StepParams sp = make_sp(rtol, atol, max_step, min_step, max_steps);
     EventParams ev;
     ev.index = ev_index; ev.offset = ev_offset; ev.direction = direction; ev.xtol = xtol; ev.gtol = gtol;
-    hipLaunchKernelGGL(k_dop853_event6, dim3(blocks_for(n, block)), dim3(block), 0,
-                       (hipStream_t)stream, n, mu, t0, tmax, sp, ev, fwd, flip_mask, y0, hit,
-                       t_hit, y_hit, y_last, status, n_acc, n_rej);
+    hipStream_t st = (hipStream_t)stream;
+    if (variant == HH_VARIANT_WIDE) {
+        const int g = blocks_for((long long)n * 8, block);
+        switch (block) {
+            case 64: hipLaunchKernelGGL(k_dop853_event6_group<64>, dim3(g), dim3(64), 0, st, n, mu, t0, tmax, sp, ev, fwd, flip_mask, y0, hit, t_hit, y_hit, y_last, status, n_acc, n_rej); break;
+            case 128: hipLaunchKernelGGL(k_dop853_event6_group<128>, dim3(g), dim3(128), 0, st, n, mu, t0, tmax, sp, ev, fwd, flip_mask, y0, hit, t_hit, y_hit, y_last, status, n_acc, n_rej); break;
+            case 192: hipLaunchKernelGGL(k_dop853_event6_group<192>, dim3(g), dim3(192), 0, st, n, mu, t0, tmax, sp, ev, fwd, flip_mask, y0, hit, t_hit, y_hit, y_last, status, n_acc, n_rej); break;
+            default: hipLaunchKernelGGL(k_dop853_event6_group<256>, dim3(g), dim3(256), 0, st, n, mu, t0, tmax, sp, ev, fwd, flip_mask, y0, hit, t_hit, y_hit, y_last, status, n_acc, n_rej); break;
+        }
+        return check_launch("hh_dop853_event (group) launch");
+    }
+    if (variant != HH_VARIANT_PLAIN) return fail(HH_ERR_ARGS, "hh_dop853_event: variant must be PLAIN or WIDE");
+    hipLaunchKernelGGL(k_dop853_event6, dim3(blocks_for(n, block)), dim3(block), 0, st, n, mu,
+                       t0, tmax, sp, ev, fwd, flip_mask, y0, hit, t_hit, y_hit, y_last, status,
+                       n_acc, n_rej);
     return check_launch("hh_dop853_event launch");
 }
 

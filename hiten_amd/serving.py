@@ -188,7 +188,7 @@ class _Packed:
 
 def _launch_event(pk: _Packed, y_row: int, stream_handle: int, *, ev_index: int,
                   ev_offset: float, direction: int, xtol: float, gtol: float, rtol: float,
-                  atol: float, y_addr: Optional[int] = None) -> None:
+                  atol: float, y_addr: Optional[int] = None, variant: int = 0) -> None:
     """hh_dop853_event on packed buffers: din rows [mu, t0, tmax, ...], start
     states at din row ``y_row`` (6 rows, SoA) or at ``y_addr``; fout rows
     [t_hit, y_hit (6), y_last (6)], iout rows [hit, status, n_accepted]."""
@@ -198,7 +198,7 @@ def _launch_event(pk: _Packed, y_row: int, stream_handle: int, *, ev_index: int,
                              float(atol), math.inf, batch.MIN_STEP, batch.DEFAULT_MAX_STEPS,
                              int(ev_index), float(ev_offset), int(direction), float(xtol),
                              float(gtol), 1, 0, y, pk.iout(0), pk.fout(0), pk.fout(1), pk.fout(7),
-                             pk.iout(1), pk.iout(2), None, 0, stream_handle)
+                             pk.iout(1), pk.iout(2), None, int(variant), 0, stream_handle)
     _lib.check(rc, "hh_dop853_event")
 
 
@@ -242,7 +242,8 @@ class _FanLaunch(_Pending):
         fin[0], fin[1], fin[2], fin[3:9] = r.mu, 0.0, r.tmax, r.y0.T
         self.pk = _Packed(fin, 13, 3, slot)
         _launch_event(self.pk, 3, slot.sh, ev_index=r.ev_index, ev_offset=r.ev_offset,
-                      direction=r.direction, xtol=r.xtol, gtol=r.gtol, rtol=r.rtol, atol=r.atol)
+                      direction=r.direction, xtol=r.xtol, gtol=r.gtol, rtol=r.rtol, atol=r.atol,
+                      variant=_lib.HH_VARIANT_WIDE)
         self.pk.download(slice(0, 7))
         slot.record()
 
@@ -294,7 +295,8 @@ class _EventBatch(_Pending):
             y_addr = pk.fout(13)
         if self.searched:
             _launch_event(pk, 0, sl.sh, ev_index=Y_INDEX, ev_offset=0.0, direction=0, xtol=1e-12,
-                          gtol=1e-12, rtol=1e-12, atol=1e-12, y_addr=y_addr)
+                          gtol=1e-12, rtol=1e-12, atol=1e-12, y_addr=y_addr,
+                          variant=_lib.HH_VARIANT_WIDE)
         pk.download(slice(0, 13))
         sl.record()
 

@@ -133,7 +133,7 @@ int hh_dop853_event(int n, const double* mu, const double* t0, const double* tma
                     int ev_index, double ev_offset, int direction, double xtol, double gtol,
                     int fwd, unsigned long long flip_mask, const double* y0, int* hit,
                     double* t_hit, double* y_hit, double* y_last, int* status, int* n_acc,
-                    int* n_rej, int block, void* stream);
+                    int* n_rej, int variant, int block, void* stream);
 
 /* Evaluate the CR3BP right-hand side at n states (dim 6: _crtbp_accel,
  * dynamics/rtbp.py:30-74; dim 42: _var_equations, rtbp.py:167-255).

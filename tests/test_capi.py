@@ -49,7 +49,7 @@ def test_argument_errors_are_reported_without_gpu():
     assert b"dim must be 6 or 42" in lib.hh_last_error()
     rc = lib.hh_dop853_event(4, None, None, None, 1e-12, 1e-12, 1e4, 1e-15, 10, 9, 0.0, 0,
                              1e-12, 1e-12, 1, 0, None, None, None, None, None, None, None, None,
-                             0, None)
+                             0, 0, None)
     assert rc == 1
     assert b"null pointer" in lib.hh_last_error() or b"ev_index" in lib.hh_last_error()
 

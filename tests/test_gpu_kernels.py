@@ -131,17 +131,37 @@ def test_dop853_final_42d(gpu, oracle, variant):
     assert_bitwise(got, want, f"dop853 42-d final ({variant})", rtol=1e-10)
 
 
-def test_dop853_event_fan(gpu, oracle):
-    say("hh_dop853_event: 4 seeds x 64 directions, y=0 crossing, direction -1")
+@pytest.mark.parametrize("variant", ["plain", "wide"])
+def test_dop853_event_fan(gpu, oracle, variant):
+    say(f"hh_dop853_event {variant}: 4 seeds x 64 directions, y=0 crossing, direction -1")
     w = W.manifold_fan(n_seeds=4, n_dirs=64)
     r = batch.dop853_event(w["mu"], w["y0"], w["t0"], w["tmax"], ev_index=1, ev_offset=0.0,
-                           direction=-1, xtol=w["xtol"], gtol=w["gtol"])
+                           direction=-1, xtol=w["xtol"], gtol=w["gtol"], variant=variant)
     hit, th, yh, na, nr = oracle.batch_dop853_event(w["mu"], w["y0"], w["tmax"], ev_index=1,
                                                     direction=-1, xtol=w["xtol"], gtol=w["gtol"])
     assert_bitwise(r.hit.cpu().numpy(), hit, "hit flags")
     assert_bitwise(r.t_hit.cpu().numpy(), th, "t_hit", rtol=1e-10)
     assert_bitwise(r.y_hit.cpu().numpy(), yh, "y_hit", rtol=1e-10)
     assert_bitwise(r.n_accepted.cpu().numpy(), na, "accepted")
+
+
+@pytest.mark.parametrize("ev_index,direction", [(0, 1), (2, 0), (1, -1)])
+def test_dop853_event_group_matches_plain(gpu, ev_index, direction):
+    """The one-trajectory-per-group event search gives the per-lane kernel's
+    bits for other event components and directions, backward time, a tight
+    max_steps and a zero span."""
+    say(f"hh_dop853_event group vs plain: ev_index={ev_index} direction={direction}")
+    w = W.manifold_fan(n_seeds=3, n_dirs=32)
+    tmax = w["tmax"].copy()
+    tmax[5] = 0.0
+    for fwd in (1, -1):
+        kw = dict(ev_index=ev_index, ev_offset=0.01 * ev_index, direction=direction,
+                  xtol=1e-13, gtol=1e-13, forward=fwd, max_steps=300 if fwd < 0 else 1_000_000)
+        a = batch.dop853_event(w["mu"], w["y0"], 0.0, tmax, variant="plain", **kw)
+        b = batch.dop853_event(w["mu"], w["y0"], 0.0, tmax, variant="wide", **kw)
+        for name in ("hit", "t_hit", "y_hit", "y_last", "status", "n_accepted", "n_rejected"):
+            assert_bitwise(getattr(b, name).cpu().numpy(), getattr(a, name).cpu().numpy(),
+                           f"{name} fwd={fwd}")
 
 
 def test_dop853_42d_early_stop_status(gpu, oracle):
