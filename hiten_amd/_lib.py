@@ -44,11 +44,11 @@ _SIGNATURES = {
     "hh_last_error": ([], C.c_char_p),
     "hh_abi_version": ([], _I),
     "hh_set_tuning": ([_I, _I], _I),
-    "hh_rk8_fixed": ([_I, _I, _P, _P, _P, _I, _P, _I, _ULL, _P, _P, _P, _I, _P], _I),
+    "hh_rk8_fixed": ([_I, _I, _P, _P, _P, _I, _P, _I, _ULL, _P, _P, _P, _I, _I, _P], _I),
     "hh_dop853_final": ([_I, _I, _P, _P, _P, _D, _D, _D, _D, _LL, _I, _ULL, _P, _P, _P, _P, _P,
                          _I, _I, _I, _P, _P, _P], _I),
     "hh_dop853_grid": ([_I, _I, _P, _P, _P, _I, _P, _D, _D, _D, _D, _LL, _I, _ULL, _P, _P, _P, _P,
-                        _P, _I, _P], _I),
+                        _P, _I, _I, _P], _I),
     "hh_dop853_event": ([_I, _P, _P, _P, _D, _D, _D, _D, _LL, _I, _D, _I, _D, _D, _I, _ULL, _P,
                          _P, _P, _P, _P, _P, _P, _P, _I, _I, _P], _I),
     "hh_eval_rhs": ([_I, _I, _P, _P, _P, _P], _I),

@@ -113,12 +113,16 @@ class EventResult:
 
 
 def rk8_fixed(mu, y0, t0, tf, steps: int, *, dim: int = 6, t_vals=None, forward: int = 1,
-              flip_indices=None, return_trajectory: bool = False, block: int = 0, stream=None):
+              flip_indices=None, return_trajectory: bool = False, variant: str = "auto",
+              block: int = 0, stream=None):
     """Fixed-step RK8 over numpy.linspace(t0, tf, steps), or over the explicit
     grid ``t_vals`` shared by all trajectories (rk.py:531-588).
 
     Returns the final states (n, dim); with ``return_trajectory`` also the
-    grid states as an (n, steps, dim) view.
+    grid states as an (n, steps, dim) view.  For dim 6, ``variant`` "wide"
+    integrates one trajectory per 8-lane group (one component per lane, the
+    latency form), "plain" one per lane; "auto" takes "wide" up to 65,536
+    trajectories.  Both give the same bits.
     """
     lib = _lib.load()
     y = _soa(y0, dim)
@@ -133,9 +137,13 @@ def rk8_fixed(mu, y0, t0, tf, steps: int, *, dim: int = 6, t_vals=None, forward:
         t0_t, tf_t = _vec(t0, n, "t0"), _vec(tf, n, "tf")
     out = torch.empty_like(y)
     traj = torch.empty((steps, dim, n), dtype=torch.float64, device=y.device) if return_trajectory else None
+    if variant == "auto":
+        variant = "wide" if n <= 65536 else "plain"
     rc = lib.hh_rk8_fixed(n, dim, _ptr(mu_t), _ptr(t0_t), _ptr(tf_t), int(steps), _ptr(tv),
                           int(forward), flip_mask(dim, flip_indices), _ptr(y), _ptr(out),
-                          _ptr(traj), int(block), _lib.stream_handle(stream))
+                          _ptr(traj), {"plain": _lib.HH_VARIANT_PLAIN,
+                                       "wide": _lib.HH_VARIANT_WIDE}[variant],
+                          int(block), _lib.stream_handle(stream))
     _lib.check(rc, "hh_rk8_fixed")
     if return_trajectory:
         return out.t(), traj.permute(2, 0, 1)
@@ -198,8 +206,14 @@ def launch_final(b: FinalBatch, *, rtol: float = 1e-12, atol: float = 1e-12,
                  max_step: float = 1e4, min_step: float = MIN_STEP,
                  max_steps: int = DEFAULT_MAX_STEPS, forward: int = 1, flip_indices=None,
                  variant: str = "plain", block: int = 0, grid: int = 0, stream=None) -> None:
-    """Enqueue hh_dop853_final on the stream (asynchronous)."""
+    """Enqueue hh_dop853_final on the stream (asynchronous).
+
+    ``variant``: "plain", "persistent" (dim 42), "wide", or "auto" ("wide"
+    for dim 6 up to 65,536 trajectories, else "plain").
+    """
     lib = _lib.load()
+    if variant == "auto":
+        variant = "wide" if b.dim == 6 and b.n <= 65536 else "plain"
     var = {"plain": _lib.HH_VARIANT_PLAIN, "persistent": _lib.HH_VARIANT_PERSISTENT,
            "wide": _lib.HH_VARIANT_WIDE}[variant]
     rc = lib.hh_dop853_final(b.n, b.dim, _ptr(b.mu), _ptr(b.t0), _ptr(b.tf), float(rtol),
@@ -214,7 +228,7 @@ def launch_final(b: FinalBatch, *, rtol: float = 1e-12, atol: float = 1e-12,
 def dop853_final(mu, y0, t0, tf, *, dim: int = 6, rtol: float = 1e-12, atol: float = 1e-12,
                  max_step: float = 1e4, min_step: float = MIN_STEP,
                  max_steps: int = DEFAULT_MAX_STEPS, forward: int = 1, flip_indices=None,
-                 variant: str = "plain", block: int = 0, grid: int = 0,
+                 variant: str = "auto", block: int = 0, grid: int = 0,
                  stream=None) -> FinalResult:
     """Adaptive DOP853 to tf; the state returned is the dense-output value at tf
     (the last row of _propagate_dynsys(..., steps=2)).
@@ -238,10 +252,12 @@ class GridResult:
 def dop853_grid(mu, y0, t0, tf, m: int, *, dim: int = 6, t_eval=None, rtol: float = 1e-12,
                 atol: float = 1e-12, max_step: float = 1e4, min_step: float = MIN_STEP,
                 max_steps: int = DEFAULT_MAX_STEPS, forward: int = 1, flip_indices=None,
-                block: int = 0, stream=None) -> GridResult:
+                variant: str = "auto", block: int = 0, stream=None) -> GridResult:
     """States at the m points of numpy.linspace(t0, tf, m) -- or at the explicit
     times ``t_eval`` shared by all trajectories -- as the reference's
-    dense-output phase produces them (rk.py:2497-2543)."""
+    dense-output phase produces them (rk.py:2497-2543).  ``variant`` "wide"
+    (dim 6 only) integrates one trajectory per 8-lane group; "auto" takes it
+    for dim 6 up to 65,536 trajectories."""
     lib = _lib.load()
     y = _soa(y0, dim)
     n = y.shape[1]
@@ -254,10 +270,13 @@ def dop853_grid(mu, y0, t0, tf, m: int, *, dim: int = 6, t_eval=None, rtol: floa
     out = torch.empty((n, int(m), dim), dtype=torch.float64, device=y.device)
     i32 = dict(dtype=torch.int32, device=y.device)
     status, nacc, nrej = torch.empty(n, **i32), torch.empty(n, **i32), torch.empty(n, **i32)
+    if variant == "auto":
+        variant = "wide" if dim == 6 and n <= 65536 else "plain"
+    var = {"plain": _lib.HH_VARIANT_PLAIN, "wide": _lib.HH_VARIANT_WIDE}[variant]
     rc = lib.hh_dop853_grid(n, dim, _ptr(mu_t), _ptr(t0_t), _ptr(tf_t), int(m), _ptr(te), float(rtol),
                             float(atol), float(max_step), float(min_step), int(max_steps),
                             int(forward), flip_mask(dim, flip_indices), _ptr(y), _ptr(out),
-                            _ptr(status), _ptr(nacc), _ptr(nrej), int(block),
+                            _ptr(status), _ptr(nacc), _ptr(nrej), var, int(block),
                             _lib.stream_handle(stream))
     _lib.check(rc, "hh_dop853_grid")
     return GridResult(out, status, nacc, nrej)

@@ -213,6 +213,33 @@ struct Group6 {
     }
 };
 
+/* rk8_step (rk.py:154-215) for this lane's component */
+__device__ __forceinline__ void rk8_step_group(const Group6& p, double& y, double h) {
+    double k[RK8_S];
+    k[0] = p.rhs(y);
+#pragma unroll
+    for (int i = 1; i < RK8_S; ++i) {
+        double ys = y;
+#pragma unroll
+        for (int j = 0; j < i; ++j) {
+            const double a = HH_RK8_A[i][j];
+            if (a != 0.0) {
+                const double ha = h * a;
+                ys = ys + ha * k[j];
+            }
+        }
+        k[i] = p.rhs(ys);
+    }
+#pragma unroll
+    for (int j = 0; j < RK8_S; ++j) {
+        const double b = HH_RK8_B[j];
+        if (b != 0.0) {
+            const double hb = h * b;
+            y = y + hb * k[j];
+        }
+    }
+}
+
 /* dense_value of one component from its rows (rk.py:1989-2003) */
 __device__ __forceinline__ double dense_from_rows(const double (&row)[DOP_P], double y_old,
                                                   double x) {

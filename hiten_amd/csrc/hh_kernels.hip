@@ -311,6 +311,37 @@ void k_rk8_fixed42(int n, const double* __restrict__ mu, const double* __restric
     }
 }
 
+/* k_rk8_fixed6 with one trajectory per aligned 8-lane group, one component
+ * per lane (Group6): rk8_run's grid, steps and stores, per component.       */
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_rk8_fixed6_group(
+    int n, const double* __restrict__ mu, const double* __restrict__ t0a,
+    const double* __restrict__ tfa, int steps, const double* __restrict__ t_vals, int fwd,
+    unsigned long long flip, const double* __restrict__ y0, double* __restrict__ yout,
+    double* __restrict__ traj) {
+    const long long gtid = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    const int i = (int)(gtid >> 3);
+    if (i >= n) return;
+    Group6 p;
+    p.setup(threadIdx.x & 63, mu[i], fwd, flip);
+    const int d = p.d;
+    double y = p.act ? y0[(size_t)d * n + i] : 0.0;
+    const double t0 = t0a[i], tf = tfa[i];
+    const double delta = tf - t0;
+    const double step = delta / (double)(steps - 1);
+    const bool zstep = (step == 0.0);
+    const bool constant = t_vals ? isclose(t_vals[0], t_vals[steps - 1]) : isclose(t0, tf);
+    if (traj && p.act) traj[(size_t)d * n + i] = y;
+    double t_prev = grid_time(t_vals, 0, steps, t0, tf, step, delta, zstep);
+    for (int s_idx = 1; s_idx < steps; ++s_idx) {
+        const double t_s = grid_time(t_vals, s_idx, steps, t0, tf, step, delta, zstep);
+        if (!constant) rk8_step_group(p, y, t_s - t_prev);
+        t_prev = t_s;
+        if (traj && p.act) traj[((size_t)s_idx * 6 + d) * n + i] = y;
+    }
+    if (p.act) yout[(size_t)d * n + i] = y;
+}
+
 template <bool GRID>
 __global__ __launch_bounds__(256) void k_dop853_run6(
     int n, const double* __restrict__ mu, const double* __restrict__ t0a,
@@ -351,6 +382,124 @@ __global__ __launch_bounds__(256) void k_dop853_run6(
     if (status) status[i] = s.status;
     if (nacc) nacc[i] = s.nacc;
     if (nrej) nrej[i] = s.nrej;
+}
+
+/* k_dop853_run6 with one trajectory per aligned 8-lane group, one component
+ * per lane (Group6).  traj_init / traj_iterate and the FinalOut / GridOut
+ * policies, statement for statement, with each lane holding its component. */
+template <int BLOCK, bool GRID>
+__global__ __launch_bounds__(BLOCK) void k_dop853_run6_group(
+    int n, const double* __restrict__ mu, const double* __restrict__ t0a,
+    const double* __restrict__ tfa, StepParams sp, int fwd, unsigned long long flip,
+    const double* __restrict__ y0, double* __restrict__ yout, int m,
+    const double* __restrict__ teval, int* __restrict__ status, int* __restrict__ nacc,
+    int* __restrict__ nrej) {
+    const long long gtid = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    const int i = (int)(gtid >> 3);
+    if (i >= n) return;
+    Group6 p;
+    p.setup(threadIdx.x & 63, mu[i], fwd, flip);
+    const int d = p.d;
+    double y = p.act ? y0[(size_t)d * n + i] : 0.0;
+    double k[DOP_K];
+#pragma unroll
+    for (int j = 0; j < DOP_K; ++j) k[j] = 0.0;
+    const double t0 = t0a[i], tf = tfa[i];
+    /* GridOut: output times numpy.linspace(t0, tf, m) or teval */
+    double* gout = GRID ? yout + (size_t)i * m * 6 : nullptr;
+    const double gdelta = tf - t0;
+    const double gstep = GRID ? gdelta / (double)(m - 1) : 0.0;
+    const bool gzstep = (gstep == 0.0);
+    auto tq = [&](int q) -> double {
+        if (teval) return teval[q];
+        if (q == m - 1) return tf;
+        if (gzstep) return ((double)q / (double)(m - 1)) * gdelta + t0;
+        return (double)q * gstep + t0;
+    };
+    auto put = [&](int q, double v) {
+        if (p.act) gout[(size_t)q * 6 + d] = v;
+    };
+    int q = 0;
+    double yfin = y;   /* FinalOut */
+    double t = t0, h, err_prev = -1.0;
+    int na = 0, nr = 0, st = ST_OK;
+    bool done = isclose(t0, tf) || !((t0 - tf) * 1.0 < 0.0);
+    k[0] = p.rhs(y);
+    h = p.initial_step(y, k[0], sp.rtol, sp.atol, sp.min_step, sp.max_step);
+    if (done) {
+        if constexpr (GRID) {
+            for (int j = 0; j < m; ++j) put(j, y);
+            q = m;
+        }
+    }
+    while (!done) {
+        if (na + nr >= sp.max_steps) {
+            st = ST_MAXSTEPS;
+            done = true;
+            if constexpr (GRID) { for (; q < m; ++q) put(q, y); } else { yfin = y; }
+            break;
+        }
+        double hc = clamp_step(h, sp.max_step, sp.min_step);
+        hc = adjust_step_to_endpoint(t, hc, tf);
+        double yh;
+        const double err = p.attempt(y, k, hc, sp.rtol, sp.atol, yh);
+        if (err != err) {
+            st = ST_NONFINITE;
+            done = true;
+            if constexpr (GRID) { for (; q < m; ++q) put(q, y); } else { yfin = y; }
+            break;
+        }
+        double fa, fr;
+        p.step_factors(err, err_prev, fa, fr);
+        if (err <= 1.0) {
+            const double t_new = t + hc;
+            const bool last = !((t_new - tf) * 1.0 < 0.0);
+            const double hs = t_new - t;
+            if constexpr (GRID) {
+                if (q < m && (last || tq(q) < t_new)) {
+                    if (hs == 0.0) {
+                        for (; q < m; ++q) put(q, y);
+                    } else {
+                        double row[DOP_P];
+                        p.dense_rows(y, yh, k, hs, row);
+                        for (; q < m; ++q) {
+                            const double t_q = tq(q);
+                            if (!(last || t_q < t_new)) break;
+                            put(q, dense_from_rows(row, y, (t_q - t) / hs));
+                        }
+                    }
+                }
+            } else {
+                if (last) {
+                    if (hs == 0.0) {
+                        yfin = y;
+                    } else {
+                        double row[DOP_P];
+                        p.dense_rows(y, yh, k, hs, row);
+                        yfin = dense_from_rows(row, y, (tf - t) / hs);
+                    }
+                }
+            }
+            done = last;
+            t = t_new;
+            y = yh;
+            k[0] = k[DOP_S];
+            h = hc * fa;
+            err_prev = err;
+            na += 1;
+        } else {
+            h = clamp_step(hc * fr, sp.max_step, sp.min_step);
+            nr += 1;
+        }
+    }
+    if constexpr (!GRID) {
+        if (p.act) yout[(size_t)d * n + i] = yfin;
+    }
+    if (d == 0) {
+        if (status) status[i] = st;
+        if (nacc) nacc[i] = na;
+        if (nrej) nrej[i] = nr;
+    }
 }
 
 struct EventParams {
@@ -979,6 +1128,21 @@ StepParams make_sp(double rtol, double atol, double max_step, double min_step,
 /* ======================================================================== */
 /* C-ABI                                                                     */
 /* ======================================================================== */
+template <bool GRID>
+static int launch_run6_group(int n, const double* mu, const double* t0, const double* tf,
+                             StepParams sp, int fwd, unsigned long long flip, const double* y0,
+                             double* out, int m, const double* teval, int* status, int* nacc,
+                             int* nrej, int block, hipStream_t st, const char* what) {
+    const int g = blocks_for((long long)n * 8, block);
+    switch (block) {
+        case 64: hipLaunchKernelGGL((k_dop853_run6_group<64, GRID>), dim3(g), dim3(64), 0, st, n, mu, t0, tf, sp, fwd, flip, y0, out, m, teval, status, nacc, nrej); break;
+        case 128: hipLaunchKernelGGL((k_dop853_run6_group<128, GRID>), dim3(g), dim3(128), 0, st, n, mu, t0, tf, sp, fwd, flip, y0, out, m, teval, status, nacc, nrej); break;
+        case 192: hipLaunchKernelGGL((k_dop853_run6_group<192, GRID>), dim3(g), dim3(192), 0, st, n, mu, t0, tf, sp, fwd, flip, y0, out, m, teval, status, nacc, nrej); break;
+        default: hipLaunchKernelGGL((k_dop853_run6_group<256, GRID>), dim3(g), dim3(256), 0, st, n, mu, t0, tf, sp, fwd, flip, y0, out, m, teval, status, nacc, nrej); break;
+    }
+    return check_launch(what);
+}
+
 extern "C" {
 
 const char* hh_last_error(void) { return g_err; }
@@ -995,7 +1159,7 @@ int hh_set_tuning(int heavy_projected, int min_age) {
 
 int hh_rk8_fixed(int n, int dim, const double* mu, const double* t0, const double* tf, int steps,
                  const double* t_vals, int fwd, unsigned long long flip_mask, const double* y0,
-                 double* y_out, double* traj_out, int block, void* stream) {
+                 double* y_out, double* traj_out, int variant, int block, void* stream) {
     if (n < 0) return fail(HH_ERR_ARGS, "hh_rk8_fixed: n < 0");
     if (n == 0) return HH_OK;
     if (dim != 6 && dim != 42) return fail(HH_ERR_ARGS, "hh_rk8_fixed: dim must be 6 or 42");
@@ -1006,7 +1170,18 @@ int hh_rk8_fixed(int n, int dim, const double* mu, const double* t0, const doubl
     if (block > 256 || block % 64) return fail(HH_ERR_ARGS, "hh_rk8_fixed: block must be 64..256, multiple of 64");
     const double* t0p = t0 ? t0 : mu;   /* unused when t_vals is given */
     const double* tfp = tf ? tf : mu;
-    if (dim == 6) {
+    if (dim == 6 && variant == HH_VARIANT_WIDE) {
+        const int g = blocks_for((long long)n * 8, block);
+        hipStream_t st = (hipStream_t)stream;
+        switch (block) {
+            case 64: hipLaunchKernelGGL(k_rk8_fixed6_group<64>, dim3(g), dim3(64), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out); break;
+            case 128: hipLaunchKernelGGL(k_rk8_fixed6_group<128>, dim3(g), dim3(128), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out); break;
+            case 192: hipLaunchKernelGGL(k_rk8_fixed6_group<192>, dim3(g), dim3(192), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out); break;
+            default: hipLaunchKernelGGL(k_rk8_fixed6_group<256>, dim3(g), dim3(256), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out); break;
+        }
+    } else if (variant != HH_VARIANT_PLAIN && variant != HH_VARIANT_WIDE) {
+        return fail(HH_ERR_ARGS, "hh_rk8_fixed: variant must be PLAIN or WIDE");
+    } else if (dim == 6) {
         hipLaunchKernelGGL(k_rk8_fixed6, dim3(blocks_for(n, block)), dim3(block), 0,
                            (hipStream_t)stream, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0,
                            y_out, traj_out);
@@ -1032,9 +1207,14 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
     StepParams sp = make_sp(rtol, atol, max_step, min_step, max_steps);
     hipStream_t st = (hipStream_t)stream;
     if (dim == 6) {
-        if (variant != HH_VARIANT_PLAIN) return fail(HH_ERR_ARGS, "hh_dop853_final: dim 6 supports the plain variant only");
+        if (variant != HH_VARIANT_PLAIN && variant != HH_VARIANT_WIDE)
+            return fail(HH_ERR_ARGS, "hh_dop853_final: dim 6 supports the plain and wide variants");
         if (block <= 0) block = 128;
         if (block > 256 || block % 64) return fail(HH_ERR_ARGS, "hh_dop853_final: block must be 64..256, multiple of 64");
+        if (variant == HH_VARIANT_WIDE)
+            return launch_run6_group<false>(n, mu, t0, tf, sp, fwd, flip_mask, y0, y_out, 0,
+                                            nullptr, status, n_acc, n_rej, block, st,
+                                            "hh_dop853_final(6, wide) launch");
         hipLaunchKernelGGL(k_dop853_run6<false>, dim3(blocks_for(n, block)), dim3(block), 0, st, n,
                            mu, t0, tf, sp, fwd, flip_mask, y0, y_out, 0, (const double*)nullptr,
                            status, n_acc, n_rej);
@@ -1104,8 +1284,8 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
 int hh_dop853_grid(int n, int dim, const double* mu, const double* t0, const double* tf, int m,
                    const double* t_eval, double rtol, double atol, double max_step,
                    double min_step, long long max_steps, int fwd, unsigned long long flip_mask,
-                   const double* y0, double* out, int* status, int* n_acc, int* n_rej, int block,
-                   void* stream) {
+                   const double* y0, double* out, int* status, int* n_acc, int* n_rej,
+                   int variant, int block, void* stream) {
     if (n < 0) return fail(HH_ERR_ARGS, "hh_dop853_grid: n < 0");
     if (n == 0) return HH_OK;
     if (dim != 6 && dim != 42) return fail(HH_ERR_ARGS, "hh_dop853_grid: dim must be 6 or 42");
@@ -1115,9 +1295,15 @@ int hh_dop853_grid(int n, int dim, const double* mu, const double* t0, const dou
         return fail(HH_ERR_ARGS, "hh_dop853_grid: invalid tolerances/step bounds");
     StepParams sp = make_sp(rtol, atol, max_step, min_step, max_steps);
     hipStream_t st = (hipStream_t)stream;
+    if (variant != HH_VARIANT_PLAIN && !(variant == HH_VARIANT_WIDE && dim == 6))
+        return fail(HH_ERR_ARGS, "hh_dop853_grid: variant must be PLAIN, or WIDE for dim 6");
     if (dim == 6) {
         if (block <= 0) block = 128;
         if (block > 256 || block % 64) return fail(HH_ERR_ARGS, "hh_dop853_grid: block must be 64..256, multiple of 64");
+        if (variant == HH_VARIANT_WIDE)
+            return launch_run6_group<true>(n, mu, t0, tf, sp, fwd, flip_mask, y0, out, m, t_eval,
+                                           status, n_acc, n_rej, block, st,
+                                           "hh_dop853_grid(6, wide) launch");
         hipLaunchKernelGGL(k_dop853_run6<true>, dim3(blocks_for(n, block)), dim3(block), 0, st, n,
                            mu, t0, tf, sp, fwd, flip_mask, y0, out, m, t_eval, status, n_acc,
                            n_rej);

@@ -223,20 +223,24 @@ def _compute_monodromy(dynsys, x0, period):
 
 def propagate_batch(mu, states0, t0, tf, *, forward: int = 1, steps: int = 2,
                     method: str = "adaptive", flip_indices=None, rtol: float = 1e-12,
-                    atol: float = 1e-12, max_step: float = 1e4, dim: int = 6, variant: str = "plain",
+                    atol: float = 1e-12, max_step: float = 1e4, dim: int = 6, variant: str = "auto",
                     max_steps: int = batch.DEFAULT_MAX_STEPS):
     """_propagate_dynsys for many trajectories in one launch.
 
     ``steps == 2`` returns the final states (n, dim) (the dense value at tf);
     otherwise the (n, steps, dim) grid.  ``t0``/``tf``/``mu`` may be scalars or
     per-trajectory arrays.  Result arrays stay on the GPU (torch tensors).
+    ``variant`` picks the kernel layout ("auto", "plain", "wide", and
+    "persistent" for the 42-d final value); all give the same bits.
     """
+    fixed_variant = variant if variant in ("plain", "wide") else "auto"
     if method == "fixed":
         if steps == 2:
             return batch.rk8_fixed(mu, states0, t0, tf, steps, dim=dim, forward=forward,
-                                   flip_indices=flip_indices)
+                                   flip_indices=flip_indices, variant=fixed_variant)
         _, traj = batch.rk8_fixed(mu, states0, t0, tf, steps, dim=dim, forward=forward,
-                                  flip_indices=flip_indices, return_trajectory=True)
+                                  flip_indices=flip_indices, return_trajectory=True,
+                                  variant=fixed_variant)
         return traj
     if steps == 2:
         return batch.dop853_final(mu, states0, t0, tf, dim=dim, rtol=rtol, atol=atol,
@@ -244,11 +248,11 @@ def propagate_batch(mu, states0, t0, tf, *, forward: int = 1, steps: int = 2,
                                   variant=variant, max_steps=max_steps)
     return batch.dop853_grid(mu, states0, t0, tf, steps, dim=dim, rtol=rtol, atol=atol,
                              max_step=max_step, forward=forward, flip_indices=flip_indices,
-                             max_steps=max_steps)
+                             max_steps=max_steps, variant=fixed_variant)
 
 
 def compute_stm_batch(mu, x0s, tf, *, forward: int = 1, steps: int = 2, rtol: float = 1e-12,
-                      atol: float = 1e-12, max_step: float = 1e4, variant: str = "plain"):
+                      atol: float = 1e-12, max_step: float = 1e4, variant: str = "auto"):
     """_compute_stm for many initial states: returns (phi_T (n, 6, 6), x_T (n, 6))
     with ``steps == 2``, else the full (n, steps, 42) PHI grid."""
     x0s = np.atleast_2d(np.asarray(x0s, dtype=np.float64))

@@ -290,7 +290,7 @@ class _EventBatch(_Pending):
         y_addr = pk.din(4)
         if self.aligned:
             _launch_final(pk, 6, (0, 1, 3), y_addr, pk.fout(13), sl.sh, forward=self.forward,
-                          flip_mask=batch.flip_mask(6, None), variant=_lib.HH_VARIANT_PLAIN,
+                          flip_mask=batch.flip_mask(6, None), variant=_lib.HH_VARIANT_WIDE,
                           max_step=1e4, istatus=3, inacc=4)
             y_addr = pk.fout(13)
         if self.searched:

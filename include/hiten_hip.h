@@ -64,10 +64,14 @@ int hh_set_tuning(int heavy_projected, int min_age);
  *   mu, t0, tf : [n] (t0, tf may be NULL when t_vals is given)
  *   y0, y_out  : [dim][n]
  *   traj_out   : [steps][dim][n] every grid state, or NULL
+ *   variant    : HH_VARIANT_PLAIN (one trajectory per lane, or per 8-lane
+ *                group for dim 42) or HH_VARIANT_WIDE (dim 6: one
+ *                trajectory per 8-lane group, one component per lane: the
+ *                latency form for small batches); same results either way
  *   block      : threads per block (64..256, 0 = default)              */
 int hh_rk8_fixed(int n, int dim, const double* mu, const double* t0, const double* tf, int steps,
                  const double* t_vals, int fwd, unsigned long long flip_mask, const double* y0,
-                 double* y_out, double* traj_out, int block, void* stream);
+                 double* y_out, double* traj_out, int variant, int block, void* stream);
 
 /* Adaptive DOP853 from t0[i] to tf[i]; y_out is the dense-output value at
  * tf[i], i.e. the last row of _propagate_dynsys(..., steps=2,
@@ -84,8 +88,9 @@ int hh_rk8_fixed(int n, int dim, const double* mu, const double* t0, const doubl
  *                left alone in its wave, or past the heavy threshold of
  *                hh_set_tuning, finishes one component per lane) or
  *                HH_VARIANT_WIDE (dim 42: one trajectory per wave, one
- *                component per lane, from the start: the latency form for
- *                batches smaller than the GPU's wave slots)
+ *                component per lane, from the start; dim 6: one trajectory
+ *                per 8-lane group, one component per lane: the latency
+ *                forms for batches smaller than the GPU's wave slots)
  *   block      : threads per block (dim 6: 64..256; dim 42: 64/128/256/512)
  *   grid       : persistent blocks (0 = as many as fit at once)
  *   work_counter : one device int, required for the persistent variant;
@@ -111,12 +116,14 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
  * (dynamics/base.py:346-458) for the CR3BP systems.  A span that
  * numpy.isclose calls zero repeats y0 (dynamics/base.py:421-424).
  *   out : [n][m][dim] (trajectory-major, the reference's (steps, dim)
- *         array per trajectory)                                           */
+ *         array per trajectory)
+ *   variant : HH_VARIANT_PLAIN, or HH_VARIANT_WIDE for dim 6 (one
+ *         trajectory per 8-lane group, one component per lane)           */
 int hh_dop853_grid(int n, int dim, const double* mu, const double* t0, const double* tf, int m,
                    const double* t_eval, double rtol, double atol, double max_step,
                    double min_step, long long max_steps, int fwd, unsigned long long flip_mask,
-                   const double* y0, double* out, int* status, int* n_acc, int* n_rej, int block,
-                   void* stream);
+                   const double* y0, double* out, int* status, int* n_acc, int* n_rej,
+                   int variant, int block, void* stream);
 
 /* Adaptive DOP853 (6-d) from t0[i] until the first crossing of the plane
  * y[ev_index] = ev_offset in `direction` (-1, 0, +1), refined by bisection

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 import torch
 
-from hiten_amd import batch, workloads as W
+from hiten_amd import _lib, batch, workloads as W
 
 pytestmark = pytest.mark.gpu
 
@@ -79,11 +79,12 @@ def test_rhs_matches_oracle(gpu, oracle, dim):
     assert_bitwise(got, want, f"rhs{dim}")
 
 
-def test_rk8_fixed_matches_oracle(gpu, oracle):
-    say("hh_rk8_fixed: 128 trajectories x 2000 grid points to 2*pi")
+@pytest.mark.parametrize("variant", ["plain", "wide"])
+def test_rk8_fixed_matches_oracle(gpu, oracle, variant):
+    say(f"hh_rk8_fixed {variant}: 128 trajectories x 2000 grid points to 2*pi")
     w = W.rk8_grid(n=128)
     got_final, got_traj = batch.rk8_fixed(w["mu"], w["y0"], w["t0"], w["tf"], w["steps"],
-                                          return_trajectory=True)
+                                          return_trajectory=True, variant=variant)
     got_final = got_final.cpu().numpy()
     got_traj = got_traj.cpu().numpy()
     want = oracle.batch_rk8_final(w["mu"], w["y0"], w["t0"], w["tf"], w["steps"])
@@ -95,21 +96,24 @@ def test_rk8_fixed_matches_oracle(gpu, oracle):
         assert_bitwise(got_traj[i], st, f"rk8 grid traj {i}")
 
 
-def test_rk8_backward_flip(gpu, oracle):
-    say("hh_rk8_fixed: backward time with all components flipped")
+@pytest.mark.parametrize("variant", ["plain", "wide"])
+def test_rk8_backward_flip(gpu, oracle, variant):
+    say(f"hh_rk8_fixed {variant}: backward time with all components flipped")
     w = W.rk8_grid(n=64)
-    got = batch.rk8_fixed(w["mu"], w["y0"], 0.0, 1.5, 300, forward=-1).cpu().numpy()
+    got = batch.rk8_fixed(w["mu"], w["y0"], 0.0, 1.5, 300, forward=-1,
+                          variant=variant).cpu().numpy()
     tv = np.linspace(0.0, 1.5, 300)
     want = np.stack([oracle.rk_fixed(oracle.System("cr3bp6", mu=w["mu"], fwd=-1), w["y0"][i], tv)[-1]
                      for i in range(64)])
     assert_bitwise(got, want, "rk8 backward")
 
 
-def test_dop853_final_6d(gpu, oracle):
-    say("hh_dop853_final dim=6: 512 trajectories")
+@pytest.mark.parametrize("variant", ["plain", "wide"])
+def test_dop853_final_6d(gpu, oracle, variant):
+    say(f"hh_dop853_final dim=6 {variant}: 512 trajectories")
     w = W.stm_uneven(n=512)
     y0 = w["y0"][:, 36:]
-    r = batch.dop853_final(w["mu"], y0, w["t0"], w["tf"], dim=6, max_step=1e4)
+    r = batch.dop853_final(w["mu"], y0, w["t0"], w["tf"], dim=6, max_step=1e4, variant=variant)
     got = r.y.cpu().numpy()
     want, na, nr, st = oracle.batch_dop853_final(w["mu"], y0, w["t0"], w["tf"], max_step=1e4)
     assert_bitwise(r.n_accepted.cpu().numpy(), na, "accepted steps")
@@ -234,15 +238,16 @@ def test_dop853_42d_wave_wide_backward_flip(gpu, oracle, heavy_threshold, varian
         assert_bitwise(got[i], want[-1], f"backward traj {i}", rtol=1e-10)
 
 
-@pytest.mark.parametrize("dim,forward,flip", [(6, 1, None), (6, -1, None), (42, 1, None),
-                                              (42, -1, slice(36, 42))])
-def test_dop853_grid_matches_oracle(gpu, oracle, dim, forward, flip):
-    say(f"hh_dop853_grid dim={dim} forward={forward}: 48 trajectories x 257 points")
+@pytest.mark.parametrize("dim,forward,flip,variant", [
+    (6, 1, None, "plain"), (6, -1, None, "plain"), (6, 1, None, "wide"),
+    (6, -1, slice(0, 3), "wide"), (42, 1, None, "plain"), (42, -1, slice(36, 42), "plain")])
+def test_dop853_grid_matches_oracle(gpu, oracle, dim, forward, flip, variant):
+    say(f"hh_dop853_grid dim={dim} forward={forward} {variant}: 48 trajectories x 257 points")
     w = W.stm_uneven(n=48)
     y0 = w["y0"] if dim == 42 else w["y0"][:, 36:]
     m = 257
     r = batch.dop853_grid(w["mu"], y0, 0.0, w["tf"], m, dim=dim, forward=forward,
-                          flip_indices=flip)
+                          flip_indices=flip, variant=variant)
     got = r.states.cpu().numpy()
     kind = "cr3bp42" if dim == 42 else "cr3bp6"
     for i in range(0, 48, 5):
@@ -253,18 +258,39 @@ def test_dop853_grid_matches_oracle(gpu, oracle, dim, forward, flip):
         assert_bitwise(got[i], want, f"grid traj {i}", rtol=1e-10)
 
 
-def test_dop853_grid_zero_span_and_reversed(gpu, oracle):
-    say("hh_dop853_grid: zero span and reversed span repeat y0")
+@pytest.mark.parametrize("variant", ["plain", "wide"])
+def test_dop853_grid_zero_span_and_reversed(gpu, oracle, variant):
+    say(f"hh_dop853_grid {variant}: zero span and reversed span repeat y0")
     w = W.stm_uneven(n=4)
     y0 = w["y0"][:, 36:]
     tf = np.array([1e-12, 0.0, -0.5, 2.0])
-    r = batch.dop853_grid(w["mu"], y0, 0.0, tf, 9, dim=6)
+    r = batch.dop853_grid(w["mu"], y0, 0.0, tf, 9, dim=6, variant=variant)
     got = r.states.cpu().numpy()
     for i in range(3):
         assert np.array_equal(got[i], np.repeat(y0[i][None], 9, axis=0))
     want, _ = oracle.dop853(oracle.System("cr3bp6", mu=w["mu"]), y0[3], np.linspace(0, 2.0, 9),
                             rtol=1e-12, atol=1e-12, max_step=1e4)
     assert_bitwise(got[3], want, "regular span")
+    fin = batch.dop853_final(w["mu"], y0, 0.0, tf, dim=6, variant=variant).y.cpu().numpy()
+    for i in range(3):
+        assert np.array_equal(fin[i], y0[i])
+    assert_bitwise(fin[3], want[-1], "regular span final", rtol=1e-10)
+
+
+@pytest.mark.parametrize("variant", ["plain", "wide"])
+def test_dop853_6d_max_steps_stop(gpu, oracle, variant):
+    say(f"hh_dop853_final/grid dim=6 {variant}: max_steps stop keeps the last state")
+    w = W.stm_uneven(n=8)
+    y0 = w["y0"][:, 36:]
+    r = batch.dop853_final(w["mu"], y0, 0.0, w["tf"], dim=6, max_steps=5, variant=variant)
+    assert (r.status.cpu().numpy() == _lib.HH_STATUS_MAXSTEPS).all()
+    assert (r.n_accepted.cpu().numpy() + r.n_rejected.cpu().numpy() == 5).all()
+    g = batch.dop853_grid(w["mu"], y0, 0.0, w["tf"], 17, dim=6, max_steps=5, variant=variant)
+    assert (g.status.cpu().numpy() == _lib.HH_STATUS_MAXSTEPS).all()
+    st = g.states.cpu().numpy()
+    fin = r.y.cpu().numpy()
+    for i in range(8):
+        assert np.array_equal(st[i, -1], fin[i])
 
 
 def test_rk8_fixed_42d_and_explicit_grid(gpu, oracle):
@@ -279,11 +305,13 @@ def test_rk8_fixed_42d_and_explicit_grid(gpu, oracle):
         assert_bitwise(traj[i], want, f"rk8 42-d traj {i}")
 
 
-def test_dop853_grid_explicit_t_eval(gpu, oracle):
-    say("hh_dop853_grid with an explicit t_eval array")
+@pytest.mark.parametrize("variant", ["plain", "wide"])
+def test_dop853_grid_explicit_t_eval(gpu, oracle, variant):
+    say(f"hh_dop853_grid {variant} with an explicit t_eval array")
     w = W.stm_uneven(n=16)
     te = np.sort(np.concatenate([[0.0, 2.2], np.random.default_rng(1).uniform(0.0, 2.2, 40)]))
-    r = batch.dop853_grid(w["mu"], w["y0"][:, 36:], None, None, 0, dim=6, t_eval=te)
+    r = batch.dop853_grid(w["mu"], w["y0"][:, 36:], None, None, 0, dim=6, t_eval=te,
+                          variant=variant)
     got = r.states.cpu().numpy()
     for i in range(0, 16, 3):
         want, _ = oracle.dop853(oracle.System("cr3bp6", mu=w["mu"]), w["y0"][i, 36:], te,
