@@ -32,14 +32,16 @@ FLOPS_PER_STEP_42 = 8622          # algorithmic FLOPs of one accepted 42-d DOP85
 FP64_PEAK_TFLOPS = 78.6           # MI355X FP64 vector/matrix peak
 
 
-PROFILE = os.path.join(ROOT, "profiles", "k_dop853_final42_persist__persistent_262k.md")
+def profile_path(kernel: str) -> str:
+    return os.path.join(ROOT, "profiles", f"{kernel}__persistent_262k.md")
 
 
-def profiled_traffic():
-    """HBM bytes per launch of the persistent kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated)."""
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of ``kernel`` on the headline workload from its
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel
+    calibrated), or None without one."""
     try:
-        with open(PROFILE) as f:
+        with open(profile_path(kernel)) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch"):
                     return int(line.split("=")[1])
@@ -236,7 +238,7 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--plain-steps", type=int, default=1)
@@ -246,6 +248,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--heavy", type=int, default=2000)
     ap.add_argument("--heavy-age", type=int, default=128)
+    ap.add_argument("--pair-heavy", type=int, default=0)
+    ap.add_argument("--variant", default="auto",
+                    choices=["auto", "persistent", "persistent_pair"],
+                    help="auto: autotune variant and knobs on the batch before timing")
+    ap.add_argument("--tune-trials", type=int, default=12)
+    ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--order", default="span", choices=["span", "interleave", "index"])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--workload", default="persistent_262k",
@@ -268,30 +276,52 @@ def main():
               flush=True)
         return
 
-    from hiten_amd import _lib
-    _lib.check(_lib.load().hh_set_tuning(args.heavy, args.heavy_age), "hh_set_tuning")
+    from hiten_amd import _lib, autotune
     w = W.persistent_262k(args.n)
     w["max_steps"] = args.max_steps
     b = batch.prepare_final(w["mu"], w["y0"], w["t0"], w["tf"], dim=42, order=args.order)
-    kw = dict(rtol=w["rtol"], atol=w["atol"], max_step=1e4, max_steps=args.max_steps,
-              block=args.block, grid=args.grid)
+    step_kw = dict(rtol=w["rtol"], atol=w["atol"], max_step=1e4, max_steps=args.max_steps)
 
-    def run(variant, reps, warm):
+    # kernel variant and scheduling knobs: autotuned on this batch (untimed),
+    # or as given on the command line
+    tune_info = None
+    if args.variant == "auto":
+        res = autotune.tune_final(b, reps=1, max_trials=args.tune_trials, use_cache=False,
+                                  **step_kw)
+        cfg = res.best
+        tune_info = {"trials": len(res.trials),
+                     "table": [dict(t.config.__dict__, ms=round(t.seconds * 1e3, 2))
+                               for t in res.trials]}
+    else:
+        cfg = autotune.Config(variant=args.variant, heavy_projected=args.heavy,
+                              min_age=args.heavy_age, pair_projected=args.pair_heavy)
+        autotune.apply(cfg)
+    kernel = {"persistent": "k_dop853_final42_persist",
+              "persistent_pair": "k_dop853_final42_resume"}[cfg.variant]
+    kw = dict(step_kw, block=args.block or cfg.block, grid=args.grid)
+
+    def run(variant, reps, warm, graph=None):
+        lkw = step_kw if variant == "plain" else kw
         for _ in range(warm):
-            batch.launch_final(b, variant=variant, **kw)
+            batch.launch_final(b, variant=variant, **lkw)
         torch.cuda.synchronize()
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                for _ in range(reps)]
         t = time.perf_counter()
         for e0, e1 in evs:
             e0.record()
-            batch.launch_final(b, variant=variant, **kw)
+            if graph is not None:
+                graph.replay()
+            else:
+                batch.launch_final(b, variant=variant, **lkw)
             e1.record()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t
         return wall, [e0.elapsed_time(e1) / 1e3 for e0, e1 in evs]
 
-    wall, launches = run("persistent", args.steps, args.warmup)
+    # the timed passes replay the launch sequence from a HIP graph
+    graph = None if args.no_graph else batch.capture_final(b, variant=cfg.variant, **kw)
+    wall, launches = run(cfg.variant, args.steps, args.warmup, graph)
     y_persist = b.y.clone()
     nacc = b.n_accepted.to(torch.int64)
     steps_per_pass = int(nacc.sum().item())
@@ -323,16 +353,19 @@ def main():
                         "mu per trajectory in {0.012150585609624, 3.0035e-6, 2.528e-5}, "
                         "states about each L1, t_f log-uniform [0.5, 20]",
             "n_trajectories": args.n,
-            "kernel": "k_dop853_final42_persist",
+            "kernel": kernel,
             "max_steps": args.max_steps,
+            "tuning": dict(cfg.__dict__, block=kw["block"], grid=args.grid or "resident"),
+            "hip_graph": graph is not None,
         },
+        "autotune": tune_info,
         "roofline": {
             "bound": "mfma",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": profiled_traffic(),
+            "traffic": profiled_traffic(kernel),
         },
         "accepted_steps_per_pass": steps_per_pass,
         "rejected_steps_per_pass": rejected,

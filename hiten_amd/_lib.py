@@ -22,8 +22,10 @@ HH_OK = 0
 HH_VARIANT_PLAIN = 0
 HH_VARIANT_PERSISTENT = 1
 HH_VARIANT_WIDE = 2
+HH_VARIANT_PAIR = 3
+HH_VARIANT_PERSISTENT_PAIR = 4
 HH_STATUS_OK, HH_STATUS_MAXSTEPS, HH_STATUS_NONFINITE = 0, 1, 2
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 
 class HipUnavailable(RuntimeError):
@@ -32,6 +34,16 @@ class HipUnavailable(RuntimeError):
 
 class HipLibraryError(RuntimeError):
     """An entry point of libhiten_hip.so returned an error code."""
+
+
+class Tuning(C.Structure):
+    """hh_tuning_t: scheduling knobs of the persistent 42-d kernels."""
+    _fields_ = [("heavy_projected", C.c_int), ("min_age", C.c_int),
+                ("pair_projected", C.c_int), ("refill_min", C.c_int),
+                ("waves_per_simd", C.c_int)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 _P = C.c_void_p
@@ -43,10 +55,12 @@ _ULL = C.c_ulonglong
 _SIGNATURES = {
     "hh_last_error": ([], C.c_char_p),
     "hh_abi_version": ([], _I),
-    "hh_set_tuning": ([_I, _I], _I),
+    "hh_set_tuning": ([C.POINTER(Tuning)], _I),
+    "hh_get_tuning": ([C.POINTER(Tuning)], _I),
     "hh_rk8_fixed": ([_I, _I, _P, _P, _P, _I, _P, _I, _ULL, _P, _P, _P, _I, _I, _P], _I),
     "hh_dop853_final": ([_I, _I, _P, _P, _P, _D, _D, _D, _D, _LL, _I, _ULL, _P, _P, _P, _P, _P,
-                         _I, _I, _I, _P, _P, _P], _I),
+                         _I, _I, _I, _P, _LL, _P, _P], _I),
+    "hh_dop853_final_workspace": ([_I, _I, _I], _LL),
     "hh_dop853_grid": ([_I, _I, _P, _P, _P, _I, _P, _D, _D, _D, _D, _LL, _I, _ULL, _P, _P, _P, _P,
                         _P, _I, _I, _P], _I),
     "hh_dop853_event": ([_I, _P, _P, _P, _D, _D, _D, _D, _LL, _I, _D, _I, _D, _D, _I, _ULL, _P,
@@ -102,6 +116,26 @@ def load(require_gpu: bool = True):
     if require_gpu and not torch.cuda.is_available():
         raise HipUnavailable("no HIP device is visible to this process")
     return _lib
+
+
+def get_tuning() -> dict:
+    """Current hh_tuning_t as a dict."""
+    t = Tuning()
+    check(load(require_gpu=False).hh_get_tuning(C.byref(t)), "hh_get_tuning")
+    return t.as_dict()
+
+
+def set_tuning(**kw) -> dict:
+    """Change some hh_tuning_t fields (the others keep their values); returns
+    the previous settings."""
+    lib = load(require_gpu=False)
+    old = get_tuning()
+    unknown = set(kw) - set(old)
+    if unknown:
+        raise ValueError(f"unknown tuning fields {sorted(unknown)}")
+    new = dict(old, **{k: int(v) for k, v in kw.items()})
+    check(lib.hh_set_tuning(C.byref(Tuning(**new))), "hh_set_tuning")
+    return old
 
 
 def check(rc: int, what: str) -> None:

@@ -163,11 +163,20 @@ class FinalBatch:
     status: torch.Tensor
     n_accepted: torch.Tensor
     n_rejected: torch.Tensor
-    counter: torch.Tensor
     order: Optional[torch.Tensor] = None
+    workspace: Optional[torch.Tensor] = None   # grown on demand by launch_final
 
     def result(self) -> FinalResult:
         return FinalResult(self.y.t(), self.status, self.n_accepted, self.n_rejected)
+
+    def workspace_for(self, variant_code: int):
+        """(pointer, bytes) of a workspace large enough for ``variant_code``."""
+        need = int(_lib.load().hh_dop853_final_workspace(self.n, self.dim, variant_code))
+        if need == 0:
+            return None, 0
+        if self.workspace is None or self.workspace.numel() < need:
+            self.workspace = torch.empty(need, dtype=torch.uint8, device=self.y.device)
+        return _ptr(self.workspace), int(self.workspace.numel())
 
 
 def prepare_final(mu, y0, t0, tf, dim: int = 6, order: str = "span") -> FinalBatch:
@@ -198,8 +207,7 @@ def prepare_final(mu, y0, t0, tf, dim: int = 6, order: str = "span") -> FinalBat
     elif order not in ("index", None):
         raise ValueError(f"unknown order {order!r}")
     return FinalBatch(dim, n, _vec(mu, n, "mu"), t0_t, tf_t, y, torch.empty_like(y),
-                      torch.empty(n, **i32), torch.empty(n, **i32), torch.empty(n, **i32),
-                      torch.zeros(1, **i32), perm)
+                      torch.empty(n, **i32), torch.empty(n, **i32), torch.empty(n, **i32), perm)
 
 
 def launch_final(b: FinalBatch, *, rtol: float = 1e-12, atol: float = 1e-12,
@@ -208,21 +216,59 @@ def launch_final(b: FinalBatch, *, rtol: float = 1e-12, atol: float = 1e-12,
                  variant: str = "plain", block: int = 0, grid: int = 0, stream=None) -> None:
     """Enqueue hh_dop853_final on the stream (asynchronous).
 
-    ``variant``: "plain", "persistent" (dim 42), "wide", or "auto" ("wide"
-    for dim 6 up to 65,536 trajectories, else "plain").
+    ``variant``: "plain", "persistent" (dim 42), "persistent_pair" (dim 42),
+    "wide", "pair" (dim 42), or "auto" ("wide" for dim 6 up to 65,536
+    trajectories, else "plain").
     """
     lib = _lib.load()
     if variant == "auto":
         variant = "wide" if b.dim == 6 and b.n <= 65536 else "plain"
     var = {"plain": _lib.HH_VARIANT_PLAIN, "persistent": _lib.HH_VARIANT_PERSISTENT,
-           "wide": _lib.HH_VARIANT_WIDE}[variant]
+           "wide": _lib.HH_VARIANT_WIDE, "pair": _lib.HH_VARIANT_PAIR,
+           "persistent_pair": _lib.HH_VARIANT_PERSISTENT_PAIR}[variant]
+    ws, ws_bytes = b.workspace_for(var)
     rc = lib.hh_dop853_final(b.n, b.dim, _ptr(b.mu), _ptr(b.t0), _ptr(b.tf), float(rtol),
                              float(atol), float(max_step), float(min_step), int(max_steps),
                              int(forward), flip_mask(b.dim, flip_indices), _ptr(b.y0), _ptr(b.y),
                              _ptr(b.status), _ptr(b.n_accepted), _ptr(b.n_rejected), var,
-                             int(block), int(grid), _ptr(b.counter), _ptr(b.order),
+                             int(block), int(grid), ws, ws_bytes, _ptr(b.order),
                              _lib.stream_handle(stream))
     _lib.check(rc, "hh_dop853_final")
+
+
+class FinalGraph:
+    """The launch sequence of one hh_dop853_final call on FinalBatch ``b``
+    (the workspace reset kernels and the integration kernel) captured in a
+    HIP graph: ``replay()`` re-runs it with one graph launch on the current
+    stream, on the batch's current inputs.  The scheduling knobs in effect at
+    capture time are the graph's.  The capture contains kernels only (the
+    library resets its counters with a fill kernel, not a memset)."""
+
+    def __init__(self, b: "FinalBatch", **launch_kw):
+        import torch
+        self.b = b
+        self.launch_kw = dict(launch_kw)
+        self.launch_kw.pop("stream", None)
+        # one eager call first: allocates the workspace and caches the
+        # library's device queries outside the capture
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            launch_final(b, **self.launch_kw)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            launch_final(b, **self.launch_kw)
+        torch.cuda.synchronize()
+
+    def replay(self) -> None:
+        self.graph.replay()
+
+
+def capture_final(b: "FinalBatch", **launch_kw) -> FinalGraph:
+    """Capture ``launch_final(b, **launch_kw)`` in a HIP graph."""
+    return FinalGraph(b, **launch_kw)
 
 
 def dop853_final(mu, y0, t0, tf, *, dim: int = 6, rtol: float = 1e-12, atol: float = 1e-12,
@@ -233,7 +279,8 @@ def dop853_final(mu, y0, t0, tf, *, dim: int = 6, rtol: float = 1e-12, atol: flo
     """Adaptive DOP853 to tf; the state returned is the dense-output value at tf
     (the last row of _propagate_dynsys(..., steps=2)).
     """
-    b = prepare_final(mu, y0, t0, tf, dim, order="span" if variant == "persistent" else None)
+    b = prepare_final(mu, y0, t0, tf, dim,
+                      order="span" if variant.startswith("persistent") else None)
     launch_final(b, rtol=rtol, atol=atol, max_step=max_step, min_step=min_step,
                  max_steps=max_steps, forward=forward, flip_indices=flip_indices, variant=variant,
                  block=block, grid=grid, stream=stream)

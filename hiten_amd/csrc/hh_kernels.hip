@@ -18,12 +18,14 @@
  */
 #include "hh_device.h"
 #include "hh_wave.h"
+#include "hh_pair.h"
 #include "hh_group6.h"
 #include "../../include/hiten_hip.h"
 
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <type_traits>
 
 using namespace hh;
@@ -31,8 +33,8 @@ using namespace hh;
 namespace {
 
 thread_local char g_err[512] = "";
-int g_heavy = 2000;      /* persistent 42-d kernel: see hh_set_tuning */
-int g_heavy_age = 128;
+/* persistent 42-d kernels: see hh_set_tuning */
+hh_tuning_t g_tuning = {2000, 128, 0, 1, 2};
 
 int fail(int code, const char* msg) {
     snprintf(g_err, sizeof g_err, "%s", msg);
@@ -718,6 +720,11 @@ struct Final42Args {
                              trajectory is moved to a wave of its own, 0 = only
                              in the tail */
     int heavy_age;        /* attempts before the projection is trusted */
+    int pair_heavy;       /* pair form: projected attempts from which a heavy
+                             trajectory takes both waves of its workgroup
+                             (0 = never; below it, one wave) */
+    int refill_min;       /* pair form: idle groups a wave gathers before it
+                             refills (1 = refill every idle group at once)   */
 };
 
 __device__ __forceinline__ void load42(const Final42Args& a, int traj, int b, Lane42& p,
@@ -788,7 +795,8 @@ void k_dop853_final42_plain(Final42Args a) {
 /* traj_iterate + FinalOut for one trajectory in the wave-wide layout, from
  * the current (y, k[0], t, h, err_prev, counters) to its end; stores the
  * results as store42 does.                                                  */
-__device__ __forceinline__ void wide_finish(const Final42Args& a, const Wave42& W, int traj,
+template <class WL>
+__device__ __forceinline__ void wide_finish(const Final42Args& a, const WL& W, int traj,
                                             double y, double (&k)[DOP_K], double t, double tf,
                                             double h, double err_prev, int nacc, int nrej) {
     const StepParams& sp = a.sp;
@@ -830,8 +838,8 @@ __device__ __forceinline__ void wide_finish(const Final42Args& a, const Wave42& 
             nrej += 1;
         }
     }
-    if (W.act) a.yout[(size_t)comp42(W.lane >> 3, W.d) * a.n + traj] = out;
-    if (W.lane == 0) {
+    if (W.holds()) a.yout[(size_t)W.comp() * a.n + traj] = out;
+    if (W.leader()) {
         if (a.status) a.status[traj] = status;
         if (a.nacc) a.nacc[traj] = nacc;
         if (a.nrej) a.nrej[traj] = nrej;
@@ -904,6 +912,48 @@ __global__ __launch_bounds__(BLOCK) void k_dop853_final42_wide(Final42Args a) {
     const double d1 = __builtin_sqrt(s1) / sn;
     const double h = select_initial_step(d0, d1, sp.min_step, sp.max_step);
     wide_finish(a, W, traj, y0, k, t0, tf, h, -1.0, 0, 0);
+}
+
+/* One trajectory per pair of waves (128-thread workgroup) from the start:
+ * the state wave and the STM wave of Pair42 side by side on two SIMDs.     */
+__global__ __launch_bounds__(128) void k_dop853_final42_pair(Final42Args a) {
+    __shared__ PairLds lds;
+    const int traj = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    /* wave-uniform in a scalar register, so that every branch on the role is
+     * a uniform one and each wave executes exactly its own barriers */
+    const int role = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (traj >= a.n) return;
+    Pair42 P;
+    P.setup(lane, role, a.mu[traj], a.fwd, a.flip, &lds);
+    const double y0 = P.holds() ? a.y0[(size_t)P.comp() * a.n + traj] : 0.0;
+    const double t0 = a.t0[traj], tf = a.tf[traj];
+    if (isclose(t0, tf) || !((t0 - tf) * 1.0 < 0.0)) {
+        /* zero or reversed span: y0 unchanged (dynamics/base.py:421-424) */
+        if (P.holds()) a.yout[(size_t)P.comp() * a.n + traj] = y0;
+        if (P.leader()) {
+            if (a.status) a.status[traj] = ST_OK;
+            if (a.nacc) a.nacc[traj] = 0;
+            if (a.nrej) a.nrej[traj] = 0;
+        }
+        return;
+    }
+    double k[DOP_K];
+#pragma unroll
+    for (int j = 0; j < DOP_K; ++j) k[j] = 0.0;
+    k[0] = P.rhs(y0);
+    /* initial_step with the block-tree sums of Lane42 */
+    const StepParams& sp = a.sp;
+    const double sc = sp.atol + sp.rtol * __builtin_fabs(y0);
+    const double ua = y0 / sc;
+    const double ub = k[0] / sc;
+    double s0, s1;
+    P.pair_tree(P.holds() ? ua * ua : 0.0, P.holds() ? ub * ub : 0.0, s0, s1);
+    const double sn = __builtin_sqrt(42.0);
+    const double d0 = __builtin_sqrt(s0) / sn;
+    const double d1 = __builtin_sqrt(s1) / sn;
+    const double h = select_initial_step(d0, d1, sp.min_step, sp.max_step);
+    wide_finish(a, P, traj, y0, k, t0, tf, h, -1.0, 0, 0);
 }
 
 /* Work-stealing form.  Each wave keeps its 8 groups busy: after every step
@@ -981,6 +1031,320 @@ void k_dop853_final42_persist(Final42Args a) {
                 store42(a, traj, b, s, o.y);
                 traj = -1;
             }
+        }
+    }
+}
+
+/* ---- persistent form with a resume queue and pair tails --------------- */
+
+/* Workspace of the pair-tail persistent form (caller memory, reset by the
+ * library on the stream before each launch):
+ *   ctr[0] fresh-work counter        ctr[1] resume slots claimed (tail)
+ *   ctr[2] resume slots taken (head) ctr[3] trajectories not yet finished
+ *   ctr[4] spills that found no slot (an error; never happens for
+ *          cap >= 15 n + 64: every pairing or wave-wide run finishes one
+ *          trajectory and spills at most 15)
+ *   rq[cap]        trajectory ids of the resume queue, -1 until written
+ *   spill[89][n]   resumable state of a spilled trajectory: y (42, entry
+ *                  6*b + d of Lane42 block b), k0 (42), t, h, err_prev, nacc,
+ *                  nrej.  (t, h, err_prev, counters and the FSAL stage are
+ *                  the whole state of the step loop, so a trajectory resumed
+ *                  elsewhere continues bit for bit.)                      */
+constexpr int RQ_CTR = 64;          /* ints reserved for the counters      */
+constexpr int SPILL_W = 89;
+
+struct ResumeArgs {
+    int* ctr;
+    int* rq;
+    double* spill;
+    int cap;
+};
+
+__host__ __device__ inline long long resume_cap(long long n) { return 15 * n + 64; }
+__host__ __device__ inline long long ws_align(long long bytes) { return (bytes + 255) / 256 * 256; }
+
+/* The trajectory a pair takes over, in Wave42's block/entry order */
+struct PairHandoff {
+    double y[8][8];
+    double k[8][8];
+    double t, tf, h, err_prev, mu;
+    int nacc, nrej, traj;
+};
+
+/* Write the resumable state of every group with `me` set to the spill area
+ * and publish its id in the resume queue (one slot claim per wave).       */
+__device__ __forceinline__ void spill42(const Final42Args& a, const ResumeArgs& r, bool me,
+                                        int traj, int b, int grp, int lane,
+                                        const Traj<Lane42>& s) {
+    const unsigned long long m = __ballot(me && b == 0);
+    if (!m) return;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(&r.ctr[1], __popcll(m));
+    base = __shfl(base, 0);
+    const int slot = base + __popcll(m & ((1ULL << (grp * 8)) - 1ULL));
+    const size_t n = (size_t)a.n;
+    if (me) {
+        if (b < 7) {
+#pragma unroll
+            for (int d = 0; d < 6; ++d) {
+                r.spill[(size_t)(b * 6 + d) * n + traj] = s.y[d];
+                r.spill[(size_t)(42 + b * 6 + d) * n + traj] = s.k[0][d];
+            }
+        }
+        if (b == 0) {
+            r.spill[84 * n + traj] = s.t;
+            r.spill[85 * n + traj] = s.h;
+            r.spill[86 * n + traj] = s.err_prev;
+            r.spill[87 * n + traj] = (double)s.nacc;
+            r.spill[88 * n + traj] = (double)s.nrej;
+        }
+    }
+    __threadfence();
+    if (me && b == 0) {
+        if (slot < r.cap) {
+            __hip_atomic_store(&r.rq[slot], traj, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            atomicAdd(&r.ctr[4], 1);
+            atomicSub(&r.ctr[3], 1);
+        }
+    }
+}
+
+/* Continue a spilled trajectory in this lane's group (inverse of spill42) */
+__device__ __forceinline__ void resume42(const Final42Args& a, const ResumeArgs& r, int traj,
+                                         int b, Lane42& p, Traj<Lane42>& s) {
+    const size_t n = (size_t)a.n;
+    p.mu = a.mu[traj];
+    p.mu2 = 1.0 - p.mu;
+    p.flipbits = lane_flip_bits(a.fwd, a.flip, b, true);
+    p.flip_on = (a.fwd == -1) && a.flip != 0;
+    const int bb = (b < 7) ? b : 0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const double vy = r.spill[(size_t)(bb * 6 + d) * n + traj];
+        const double vk = r.spill[(size_t)(42 + bb * 6 + d) * n + traj];
+        s.y[d] = (b < 7) ? vy : 0.0;
+        s.k[0][d] = (b < 7) ? vk : 0.0;
+    }
+    s.t = r.spill[84 * n + traj];
+    s.h = r.spill[85 * n + traj];
+    s.err_prev = r.spill[86 * n + traj];
+    s.nacc = (int)r.spill[87 * n + traj];
+    s.nrej = (int)r.spill[88 * n + traj];
+    s.tf = a.tf[traj];
+    s.status = ST_OK;
+    s.done = false;
+}
+
+/* Finish the handed-over trajectory with both waves of the workgroup */
+__device__ __forceinline__ void run_pair(const Final42Args& a, const ResumeArgs& r,
+                                         const PairHandoff& ho, PairLds* plds, int wid,
+                                         int lane) {
+    Pair42 P;
+    P.setup(lane, wid, ho.mu, a.fwd, a.flip, plds);
+    const int bb = lane >> 3, dd = lane & 7;
+    const double y = P.holds() ? ho.y[bb][dd] : 0.0;
+    double k[DOP_K];
+#pragma unroll
+    for (int j = 0; j < DOP_K; ++j) k[j] = 0.0;
+    k[0] = P.holds() ? ho.k[bb][dd] : 0.0;
+    __builtin_amdgcn_s_setprio(3);
+    wide_finish(a, P, ho.traj, y, k, ho.t, ho.tf, ho.h, ho.err_prev, ho.nacc, ho.nrej);
+    __builtin_amdgcn_s_setprio(0);
+    if (P.leader()) atomicSub(&r.ctr[3], 1);
+}
+
+/* Work-stealing form with two waves per workgroup.  Each wave keeps its 8
+ * Lane42 groups busy from the resume queue first, then from the fresh
+ * counter.  A trajectory flagged heavy (projected attempts, as in the form
+ * above) -- or the last one of a wave once no work is left -- moves on at
+ * once: its wave spills every other trajectory it holds to the resume queue,
+ * where any wave picks them up, and finishes it in the Wave42 layout.  Past
+ * the second threshold it is finished by both waves of its workgroup in the
+ * Pair42 layout instead: its wave claims the workgroup's pair (LDS
+ * compare-and-swap), both waves spill what they hold and meet at a barrier;
+ * a wave that finds its partner's claim joins it at its next iteration.
+ * Waves leave once every trajectory has finished; each trajectory is bounded
+ * by max_steps and every pairing or wave-wide run finishes one, so every
+ * wave leaves.  The only waits are the barriers of a pair (both waves reach
+ * them) and the wait for a claimed resume slot, whose producer writes it
+ * right after claiming it.                                                 */
+template <int BLOCK, int WPE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+void k_dop853_final42_resume(Final42Args a, ResumeArgs r) {
+    static_assert(BLOCK == 128, "the pair form takes two waves per workgroup");
+    __shared__ PairLds plds;
+    __shared__ PairHandoff ho;
+    __shared__ int pair_state;
+    const int lane = threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int b = lane & 7;
+    const int grp = lane >> 3;
+    if (threadIdx.x == 0) pair_state = -1;
+    __syncthreads();
+    Lane42 p;
+    p.blk = b;
+    Traj<Lane42> s;
+    FinalOut o;
+    int traj = -1;
+    bool fresh_done = false;  /* the fresh counter has passed n                */
+    bool queue_dry = false;   /* the last look at the resume queue found it empty */
+    s.done = true;
+    for (;;) {
+        /* heavy trajectories (projected attempts = age * span / elapsed time):
+         * past a.pair_heavy they take both waves of the workgroup, past
+         * a.heavy -- or as the last one of a wave once no work is left -- the
+         * wave they are in                                                   */
+        bool heavy = false, huge = false;
+        if (traj >= 0) {
+            const int age = s.nacc + s.nrej;
+            if (age >= a.heavy_age) {
+                const double t0 = a.t0[traj];
+                const double lhs = (double)age * (s.tf - t0);
+                heavy = a.heavy > 0 && lhs >= (double)a.heavy * (s.t - t0);
+                huge = a.pair_heavy > 0 && lhs >= (double)a.pair_heavy * (s.t - t0);
+            }
+        }
+        const unsigned long long live = __ballot(traj >= 0 && b == 0);
+        if (__popcll(live) == 1 && fresh_done && queue_dry) heavy = heavy || (traj >= 0);
+        const unsigned long long hm = __ballot(huge && b == 0);
+        /* pair decisions are kept in scalar registers: every branch around a
+         * barrier is then a uniform one, taken or skipped by the whole wave */
+        int owner_src = -1;
+        int join = 0;
+        if (hm) {
+            int old = 0;
+            if (lane == 0) old = atomicCAS(&pair_state, -1, wid);
+            old = __builtin_amdgcn_readfirstlane(__shfl(old, 0));
+            if (old == -1) owner_src = __ffsll((long long)hm) - 1;
+            else join = 1;
+        } else {
+            join = __builtin_amdgcn_readfirstlane(
+                __hip_atomic_load(&pair_state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= 0
+                    ? 1 : 0);
+        }
+        owner_src = __builtin_amdgcn_readfirstlane(owner_src);
+        if (owner_src >= 0 || join) {
+            const bool moved = (owner_src >= 0) && (grp == (owner_src >> 3));
+            spill42(a, r, traj >= 0 && !moved, traj, b, grp, lane, s);
+            if (moved) {
+                if (b < 7) {
+#pragma unroll
+                    for (int d = 0; d < 6; ++d) {
+                        ho.y[b][d] = s.y[d];
+                        ho.k[b][d] = s.k[0][d];
+                    }
+                }
+                if (b == 0) {
+                    ho.t = s.t;
+                    ho.tf = s.tf;
+                    ho.h = s.h;
+                    ho.err_prev = s.err_prev;
+                    ho.mu = p.mu;
+                    ho.nacc = s.nacc;
+                    ho.nrej = s.nrej;
+                    ho.traj = traj;
+                }
+            }
+            traj = -1;
+            __syncthreads();
+            run_pair(a, r, ho, &plds, wid, lane);
+            __syncthreads();
+            if (threadIdx.x == 0) pair_state = -1;
+            __syncthreads();
+            continue;
+        }
+        const unsigned long long wm = __ballot(heavy && b == 0);
+        if (wm) {
+            /* one wave: the other trajectories of the wave go to the resume
+             * queue and this one is finished one component per lane       */
+            const int src = __ffsll((long long)wm) - 1;
+            const bool moved = grp == (src >> 3);
+            spill42(a, r, traj >= 0 && !moved, traj, b, grp, lane, s);
+            if (!moved) traj = -1;
+            run_wide(a, lane, src, traj, p, s);
+            if (lane == 0) atomicSub(&r.ctr[3], 1);
+            traj = -1;
+            continue;
+        }
+        /* refill: the resume queue first, then fresh work; a wave that still
+         * has live groups waits until refill_min of its groups are idle    */
+        int finished = 0;
+        const unsigned long long need = __ballot(traj < 0 && b == 0);
+        if (need && (__popcll(need) >= a.refill_min || __popcll(live) == 0)) {
+            int got = 0, rbase = 0, dry = 0;
+            if (lane == 0) {
+                const int tail = min(__hip_atomic_load(&r.ctr[1], __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT), r.cap);
+                const int head = __hip_atomic_load(&r.ctr[2], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                dry = tail <= head;
+                if (!dry) {
+                    const int take = min(tail - head, __popcll(need));
+                    if (atomicCAS(&r.ctr[2], head, head + take) == head) {
+                        got = take;
+                        rbase = head;
+                    }
+                }
+            }
+            got = __shfl(got, 0);
+            rbase = __shfl(rbase, 0);
+            queue_dry = __shfl(dry, 0) != 0;
+            const int rank = __popcll(need & ((1ULL << (grp * 8)) - 1ULL));
+            if (traj < 0 && rank < got) {
+                int id = -1;
+                if (b == 0) {
+                    /* the producer claimed this slot and writes it next */
+                    while ((id = __hip_atomic_load(&r.rq[rbase + rank], __ATOMIC_ACQUIRE,
+                                                   __HIP_MEMORY_SCOPE_AGENT)) < 0)
+                        __builtin_amdgcn_s_sleep(2);
+                }
+                traj = __shfl(id, grp * 8);
+                resume42(a, r, traj, b, p, s);
+                o.init(s.y);
+            }
+            const bool want = (traj < 0) && !fresh_done;
+            const unsigned long long m = __ballot(want && b == 0);
+            if (m) {
+                const int cnt = __popcll(m);
+                const int leader = __ffsll((long long)m) - 1;
+                int base = 0;
+                if (lane == leader) base = atomicAdd(&r.ctr[0], cnt);
+                base = __shfl(base, leader);
+                bool past = false;
+                if (want) {
+                    const int idx = base + __popcll(m & ((1ULL << (grp * 8)) - 1ULL));
+                    if (idx < a.n) {
+                        traj = a.order ? a.order[idx] : idx;
+                        load42(a, traj, b, p, s);
+                        o.init(s.y);
+                        if (s.done) {
+                            /* zero or reversed span: finished at load time */
+                            store42(a, traj, b, s, o.y);
+                            traj = -1;
+                            finished = 1;
+                        }
+                    } else {
+                        past = true;
+                    }
+                }
+                fresh_done = __any(past);
+            }
+        }
+        if (traj >= 0) {
+            traj_iterate(p, s, a.sp, o);
+            if (s.done) {
+                store42(a, traj, b, s, o.y);
+                traj = -1;
+                finished = 1;
+            }
+        }
+        const unsigned long long fm = __ballot(finished && b == 0);
+        if (fm && lane == 0) atomicSub(&r.ctr[3], __popcll(fm));
+        if (!__any(traj >= 0)) {
+            if (__hip_atomic_load(&r.ctr[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= 0)
+                break;
+            __builtin_amdgcn_s_sleep(8);
         }
     }
 }
@@ -1149,11 +1513,19 @@ const char* hh_last_error(void) { return g_err; }
 
 int hh_abi_version(void) { return HH_ABI_VERSION; }
 
-int hh_set_tuning(int heavy_projected, int min_age) {
-    if (heavy_projected < 0 || min_age < 1)
-        return fail(HH_ERR_ARGS, "hh_set_tuning: heavy_projected < 0 or min_age < 1");
-    g_heavy = heavy_projected;
-    g_heavy_age = min_age;
+int hh_set_tuning(const hh_tuning_t* t) {
+    if (!t) return fail(HH_ERR_ARGS, "hh_set_tuning: null pointer");
+    if (t->heavy_projected < 0 || t->min_age < 1 || t->pair_projected < 0 ||
+        t->refill_min < 1 || t->refill_min > 8 || t->waves_per_simd < 1 || t->waves_per_simd > 2)
+        return fail(HH_ERR_ARGS, "hh_set_tuning: heavy_projected >= 0, min_age >= 1, "
+                                 "pair_projected >= 0, refill_min in 1..8, waves_per_simd in 1..2");
+    g_tuning = *t;
+    return HH_OK;
+}
+
+int hh_get_tuning(hh_tuning_t* t) {
+    if (!t) return fail(HH_ERR_ARGS, "hh_get_tuning: null pointer");
+    *t = g_tuning;
     return HH_OK;
 }
 
@@ -1193,11 +1565,53 @@ int hh_rk8_fixed(int n, int dim, const double* mu, const double* t0, const doubl
     return check_launch("hh_rk8_fixed launch");
 }
 
+/* Resident workgroups of a kernel on the current device (CU count times the
+ * occupancy per CU, capped at cap_per_cu when cap_per_cu > 0), queried once
+ * per device and kernel: later calls -- e.g. inside a stream capture -- make
+ * no HIP query.                                                            */
+static int resident_grid(const void* kernel, int block, int cap_per_cu, int* grid_out) {
+    struct Entry { int dev; const void* fn; int block; int cus; int per_cu; };
+    static Entry table[64];
+    static int used = 0;
+    static std::mutex mu;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    std::lock_guard<std::mutex> lock(mu);
+    for (int i = 0; i < used; ++i) {
+        if (table[i].dev == dev && table[i].fn == kernel && table[i].block == block) {
+            int per = table[i].per_cu > 0 ? table[i].per_cu : 1;
+            if (cap_per_cu > 0 && per > cap_per_cu) per = cap_per_cu;
+            *grid_out = table[i].cus * per;
+            return HH_OK;
+        }
+    }
+    int cus = 0, per_cu = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return hip_fail(e, "hipDeviceGetAttribute");
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0);
+    if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (used < 64) table[used++] = Entry{dev, kernel, block, cus, per_cu};
+    int per = per_cu > 0 ? per_cu : 1;
+    if (cap_per_cu > 0 && per > cap_per_cu) per = cap_per_cu;
+    *grid_out = cus * per;
+    return HH_OK;
+}
+
+long long hh_dop853_final_workspace(int n, int dim, int variant) {
+    if (n <= 0 || dim != 42) return 0;
+    if (variant == HH_VARIANT_PERSISTENT) return 256;
+    if (variant == HH_VARIANT_PERSISTENT_PAIR)
+        return ws_align((RQ_CTR + resume_cap(n)) * 4) + (long long)SPILL_W * n * 8;
+    return 0;
+}
+
 int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const double* tf,
                     double rtol, double atol, double max_step, double min_step,
                     long long max_steps, int fwd, unsigned long long flip_mask, const double* y0,
                     double* y_out, int* status, int* n_acc, int* n_rej, int variant, int block,
-                    int grid, int* work_counter, const int* order, void* stream) {
+                    int grid, void* workspace, long long workspace_bytes, const int* order,
+                    void* stream) {
     if (n < 0) return fail(HH_ERR_ARGS, "hh_dop853_final: n < 0");
     if (n == 0) return HH_OK;
     if (dim != 6 && dim != 42) return fail(HH_ERR_ARGS, "hh_dop853_final: dim must be 6 or 42");
@@ -1223,12 +1637,14 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
     Final42Args a;
     a.n = n; a.mu = mu; a.t0 = t0; a.tf = tf; a.sp = sp; a.fwd = fwd; a.flip = flip_mask;
     a.y0 = y0; a.yout = y_out; a.status = status; a.nacc = n_acc; a.nrej = n_rej;
-    a.counter = work_counter;
+    a.counter = (int*)workspace;
     a.order = order;
     a.m = 0;
     a.teval = nullptr;
-    a.heavy = g_heavy;
-    a.heavy_age = g_heavy_age;
+    a.heavy = g_tuning.heavy_projected;
+    a.heavy_age = g_tuning.min_age;
+    a.pair_heavy = g_tuning.pair_projected;
+    a.refill_min = g_tuning.refill_min;
     if (block <= 0) block = 256;
     if (block != 64 && block != 128 && block != 256 && block != 512)
         return fail(HH_ERR_ARGS, "hh_dop853_final: block must be 64, 128, 256 or 512");
@@ -1242,6 +1658,10 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
         }
         return check_launch("hh_dop853_final(42, plain) launch");
     }
+    if (variant == HH_VARIANT_PAIR) {
+        hipLaunchKernelGGL(k_dop853_final42_pair, dim3(n), dim3(128), 0, st, a);
+        return check_launch("hh_dop853_final(42, pair) launch");
+    }
     if (variant == HH_VARIANT_WIDE) {
         const int g = blocks_for((long long)n * 64, block);
         switch (block) {
@@ -1252,26 +1672,50 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
         }
         return check_launch("hh_dop853_final(42, wide) launch");
     }
-    if (variant != HH_VARIANT_PERSISTENT) return fail(HH_ERR_ARGS, "hh_dop853_final: unknown variant");
-    if (!work_counter) return fail(HH_ERR_ARGS, "hh_dop853_final: persistent variant needs work_counter");
-    if (grid <= 0) {
-        int dev = 0, cus = 0, per_cu = 0;
-        hipError_t e = hipGetDevice(&dev);
-        if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
-        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (e != hipSuccess) return hip_fail(e, "hipDeviceGetAttribute");
-        switch (block) {
-            case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dop853_final42_persist<64>, 64, 0); break;
-            case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dop853_final42_persist<128>, 128, 0); break;
-            case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dop853_final42_persist<256>, 256, 0); break;
-            default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dop853_final42_persist<512>, 512, 0); break;
+    if (variant != HH_VARIANT_PERSISTENT && variant != HH_VARIANT_PERSISTENT_PAIR)
+        return fail(HH_ERR_ARGS, "hh_dop853_final: unknown variant");
+    const long long need = hh_dop853_final_workspace(n, dim, variant);
+    if (!workspace || workspace_bytes < need)
+        return fail(HH_ERR_ARGS, "hh_dop853_final: persistent variants need a workspace of "
+                                 "hh_dop853_final_workspace() bytes");
+    if (variant == HH_VARIANT_PERSISTENT_PAIR) {
+        ResumeArgs r;
+        r.ctr = (int*)workspace;
+        r.rq = r.ctr + RQ_CTR;
+        r.cap = (int)resume_cap(n);
+        r.spill = (double*)((char*)workspace + ws_align((long long)(RQ_CTR + r.cap) * 4));
+        const int wpe = g_tuning.waves_per_simd;
+        int g = grid;
+        if (g <= 0) {
+            /* 4 SIMDs per CU: wpe waves per SIMD are 2 * wpe two-wave workgroups */
+            const int rc = resident_grid(
+                wpe == 1 ? (const void*)k_dop853_final42_resume<128, 1>
+                         : (const void*)k_dop853_final42_resume<128, 2>,
+                128, 2 * wpe, &g);
+            if (rc != HH_OK) return rc;
         }
-        if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-        grid = cus * (per_cu > 0 ? per_cu : 1);
+        const long long max_grid = ((long long)n * 8 + 127) / 128;
+        if (g > max_grid) g = (int)max_grid;
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, r.ctr, 0, RQ_CTR);
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, r.ctr + 3, n, 1);
+        hipLaunchKernelGGL(k_fill_int, dim3((r.cap + 255) / 256), dim3(256), 0, st, r.rq, -1, r.cap);
+        if (wpe == 1)
+            hipLaunchKernelGGL((k_dop853_final42_resume<128, 1>), dim3(g), dim3(128), 0, st, a, r);
+        else
+            hipLaunchKernelGGL((k_dop853_final42_resume<128, 2>), dim3(g), dim3(128), 0, st, a, r);
+        return check_launch("hh_dop853_final(42, persistent pair) launch");
+    }
+    if (grid <= 0) {
+        const void* fn = block == 64 ? (const void*)k_dop853_final42_persist<64>
+                       : block == 128 ? (const void*)k_dop853_final42_persist<128>
+                       : block == 256 ? (const void*)k_dop853_final42_persist<256>
+                                      : (const void*)k_dop853_final42_persist<512>;
+        const int rc = resident_grid(fn, block, 0, &grid);
+        if (rc != HH_OK) return rc;
     }
     const long long max_grid = ((long long)n * 8 + block - 1) / block;
     if (grid > max_grid) grid = (int)max_grid;
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, work_counter, 0, 1);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, a.counter, 0, 1);
     switch (block) {
         case 64: hipLaunchKernelGGL(k_dop853_final42_persist<64>, dim3(grid), dim3(64), 0, st, a); break;
         case 128: hipLaunchKernelGGL(k_dop853_final42_persist<128>, dim3(grid), dim3(128), 0, st, a); break;

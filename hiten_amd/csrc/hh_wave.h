@@ -67,6 +67,11 @@ struct Wave42 {
         cl = odd ? -mu2 : mu_;
     }
 
+    /* accessors shared with the pair layout */
+    __device__ __forceinline__ bool holds() const { return act; }
+    __device__ __forceinline__ bool leader() const { return lane == 0; }
+    __device__ __forceinline__ int comp() const { return comp42(lane >> 3, d); }
+
     __device__ __forceinline__ double gather_pv(double v) const {
         const int lo = __builtin_amdgcn_ds_bpermute(pv_addr, __double2loint(v));
         const int hi = __builtin_amdgcn_ds_bpermute(pv_addr, __double2hiint(v));

@@ -211,7 +211,8 @@ def _launch_final(pk: _Packed, dim: int, rows3, y_addr: int, out_addr: int, stre
     rc = lib.hh_dop853_final(pk.k, dim, pk.din(r0), pk.din(r1), pk.din(r2), 1e-12, 1e-12,
                              float(max_step), batch.MIN_STEP, batch.DEFAULT_MAX_STEPS,
                              int(forward), flip_mask, y_addr, out_addr, pk.iout(istatus),
-                             pk.iout(inacc), None, int(variant), 0, 0, None, None, stream_handle)
+                             pk.iout(inacc), None, int(variant), 0, 0, None, 0, None,
+                             stream_handle)
     _lib.check(rc, "hh_dop853_final")
 
 

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define HH_ABI_VERSION 1
+#define HH_ABI_VERSION 2
 
 #define HH_OK 0
 #define HH_ERR_ARGS 1
@@ -38,6 +38,8 @@ extern "C" {
 #define HH_VARIANT_PLAIN 0
 #define HH_VARIANT_PERSISTENT 1
 #define HH_VARIANT_WIDE 2
+#define HH_VARIANT_PAIR 3
+#define HH_VARIANT_PERSISTENT_PAIR 4
 
 /* Message describing the last error returned on this thread. */
 const char* hh_last_error(void);
@@ -45,14 +47,35 @@ const char* hh_last_error(void);
 /* HH_ABI_VERSION of the built library. */
 int hh_abi_version(void);
 
-/* Tuning of the persistent 42-d kernel (process-wide).  A trajectory that
- * has taken at least min_age step attempts (default 128) and whose projected
- * total, attempts * (tf - t0) / (t - t0), reaches heavy_projected (default
- * 2000) stops the refill of its wave; once alone there it is finished one
- * component per lane.  heavy_projected = 0 moves trajectories only when the
- * work queue is empty.  No counterpart in the reference (scheduling only;
- * results are bit-identical either way).                                   */
-int hh_set_tuning(int heavy_projected, int min_age);
+/* Tuning of the persistent 42-d kernels (process-wide; read at launch).
+ * No counterpart in the reference: scheduling only, results are
+ * bit-identical for every setting.
+ *   heavy_projected : a trajectory that has taken at least min_age step
+ *                     attempts and whose projected total, attempts *
+ *                     (tf - t0) / (t - t0), reaches it is finished one
+ *                     component per lane: HH_VARIANT_PERSISTENT stops the
+ *                     refill of its wave and switches once it is alone
+ *                     there; HH_VARIANT_PERSISTENT_PAIR moves the wave's
+ *                     other trajectories to its resume queue and switches at
+ *                     once.  0 = only when no work is left.  Default 2000.
+ *   min_age         : default 128
+ *   pair_projected  : HH_VARIANT_PERSISTENT_PAIR: from this projected total
+ *                     both waves of the workgroup take the trajectory (pair
+ *                     layout); 0 = never (default)
+ *   refill_min      : HH_VARIANT_PERSISTENT_PAIR: idle groups (1..8) a wave
+ *                     with live groups gathers before it refills; default 1
+ *   waves_per_simd  : HH_VARIANT_PERSISTENT_PAIR: 1 or 2 (default): the
+ *                     register budget per lane (512 or 256 VGPRs) and the
+ *                     resident grid                                        */
+typedef struct {
+    int heavy_projected;
+    int min_age;
+    int pair_projected;
+    int refill_min;
+    int waves_per_simd;
+} hh_tuning_t;
+int hh_set_tuning(const hh_tuning_t* t);
+int hh_get_tuning(hh_tuning_t* t);
 
 /* Fixed-step 13-stage RK8 on the 6-d CR3BP field (dim 6) or the 42-d
  * variational system (dim 42) over the grid t_vals[0..steps-1] when t_vals
@@ -84,26 +107,43 @@ int hh_rk8_fixed(int n, int dim, const double* mu, const double* t0, const doubl
  *   status, n_acc, n_rej : [n] int, each may be NULL
  *   variant    : HH_VARIANT_PLAIN (one launch, one trajectory per lane or
  *                8-lane group), HH_VARIANT_PERSISTENT (dim 42: work-
- *                stealing groups refilled from *work_counter; a trajectory
+ *                stealing groups refilled from a counter; a trajectory
  *                left alone in its wave, or past the heavy threshold of
- *                hh_set_tuning, finishes one component per lane) or
+ *                hh_set_tuning, finishes one component per lane),
+ *                HH_VARIANT_PERSISTENT_PAIR (dim 42: work stealing with two
+ *                waves per workgroup and a resume queue; a trajectory past
+ *                the heavy threshold, or the last one of a wave once no work
+ *                is left, is finished by both waves of its workgroup in the
+ *                pair layout while the other trajectories of the workgroup
+ *                move to the resume queue), or
  *                HH_VARIANT_WIDE (dim 42: one trajectory per wave, one
  *                component per lane, from the start; dim 6: one trajectory
  *                per 8-lane group, one component per lane: the latency
- *                forms for batches smaller than the GPU's wave slots)
+ *                forms for batches smaller than the GPU's wave slots) or
+ *                HH_VARIANT_PAIR (dim 42: one trajectory per 128-thread
+ *                workgroup, the state and the STM integrated by two waves
+ *                side by side; block is ignored)
  *   block      : threads per block (dim 6: 64..256; dim 42: 64/128/256/512)
  *   grid       : persistent blocks (0 = as many as fit at once)
- *   work_counter : one device int, required for the persistent variant;
- *                reset by the library on the stream before the launch
+ *   workspace  : device memory of at least hh_dop853_final_workspace(n,
+ *                dim, variant) bytes (work counters, resume queue, spilled
+ *                states), required by the persistent variants and reset by
+ *                the library on the stream before the launch; may be NULL
+ *                for the others.  Calls that share a workspace must not
+ *                overlap.
  *   order      : [n] permutation of 0..n-1 giving the order in which the
- *                persistent variant hands out trajectories (e.g. longest
- *                t_f first), or NULL for index order; ignored by plain     */
+ *                persistent variants hand out trajectories (e.g. longest
+ *                t_f first), or NULL for index order; ignored by the others */
 int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const double* tf,
                     double rtol, double atol, double max_step, double min_step,
                     long long max_steps, int fwd, unsigned long long flip_mask,
                     const double* y0, double* y_out, int* status, int* n_acc, int* n_rej,
-                    int variant, int block, int grid, int* work_counter, const int* order,
-                    void* stream);
+                    int variant, int block, int grid, void* workspace,
+                    long long workspace_bytes, const int* order, void* stream);
+
+/* Bytes of workspace hh_dop853_final needs for (n, dim, variant): 0 for the
+ * variants that need none.  No counterpart in the reference.              */
+long long hh_dop853_final_workspace(int n, int dim, int variant);
 
 /* Adaptive DOP853 from t0[i] to tf[i] with dense output at the m points of
  * numpy.linspace(t0[i], tf[i], m), or at t_eval[0..m-1] when t_eval is given

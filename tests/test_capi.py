@@ -44,9 +44,14 @@ def test_argument_errors_are_reported_without_gpu():
     from hiten_amd import _lib
     lib = _lib.load(require_gpu=False)
     rc = lib.hh_dop853_final(4, 7, None, None, None, 1e-12, 1e-12, 1e4, 1e-15, 10, 1, 0, None,
-                             None, None, None, None, 0, 0, 0, None, None, None)
+                             None, None, None, None, 0, 0, 0, None, 0, None, None)
     assert rc == 1
     assert b"dim must be 6 or 42" in lib.hh_last_error()
+    # workspace sizes: none for the one-launch forms, queue + spill area for the pair form
+    assert lib.hh_dop853_final_workspace(1000, 42, _lib.HH_VARIANT_PLAIN) == 0
+    assert lib.hh_dop853_final_workspace(1000, 42, _lib.HH_VARIANT_PERSISTENT) > 0
+    assert lib.hh_dop853_final_workspace(1000, 42, _lib.HH_VARIANT_PERSISTENT_PAIR) >= \
+        (15 * 1000 + 64) * 4 + 89 * 1000 * 8
     rc = lib.hh_dop853_event(4, None, None, None, 1e-12, 1e-12, 1e4, 1e-15, 10, 9, 0.0, 0,
                              1e-12, 1e-12, 1, 0, None, None, None, None, None, None, None, None,
                              0, 0, None)

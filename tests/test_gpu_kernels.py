@@ -121,7 +121,7 @@ def test_dop853_final_6d(gpu, oracle, variant):
     assert_bitwise(got, want, "dop853 6-d final", rtol=1e-10)
 
 
-@pytest.mark.parametrize("variant", ["plain", "persistent", "wide"])
+@pytest.mark.parametrize("variant", ["plain", "persistent", "persistent_pair", "wide", "pair"])
 def test_dop853_final_42d(gpu, oracle, variant):
     say(f"hh_dop853_final dim=42 {variant}: 256 trajectories")
     w = W.stm_uneven(n=256)
@@ -168,12 +168,13 @@ def test_dop853_event_group_matches_plain(gpu, ev_index, direction):
                            f"{name} fwd={fwd}")
 
 
-def test_dop853_42d_early_stop_status(gpu, oracle):
-    say("hh_dop853_final dim=42: trajectories that end in a non-finite error norm")
+@pytest.mark.parametrize("variant", ["persistent", "persistent_pair"])
+def test_dop853_42d_early_stop_status(gpu, oracle, variant):
+    say(f"hh_dop853_final dim=42 {variant}: trajectories that end in a non-finite error norm")
     w = W.persistent_262k()
     idx = np.array([183507, 248628, 145063, 223092, 0, 1, 2, 3])
     args = (w["mu"][idx], w["y0"][idx], w["t0"][idx], w["tf"][idx])
-    r = batch.dop853_final(*args, dim=42, max_step=1e4, max_steps=200_000, variant="persistent")
+    r = batch.dop853_final(*args, dim=42, max_step=1e4, max_steps=200_000, variant=variant)
     want, na, nr, st = oracle.batch_dop853_final(*args, max_step=1e4, max_steps=200_000)
     assert_bitwise(r.status.cpu().numpy(), st, "status")
     assert_bitwise(r.n_accepted.cpu().numpy(), na, "accepted")
@@ -183,22 +184,22 @@ def test_dop853_42d_early_stop_status(gpu, oracle):
 
 @pytest.fixture
 def heavy_threshold():
-    """Set the persistent kernel's heavy-trajectory threshold; restore 1000."""
-    from hiten_amd import _lib
-    lib = _lib.load()
+    """Set the persistent kernels' scheduling knobs; restore them afterwards."""
+    saved = _lib.get_tuning()
 
-    def set_(k, age=1):
-        _lib.check(lib.hh_set_tuning(int(k), int(age)), "hh_set_tuning")
+    def set_(k, age=1, pair=0, **kw):
+        _lib.set_tuning(heavy_projected=k, min_age=age, pair_projected=pair, **kw)
     yield set_
-    set_(2000, 128)
+    _lib.set_tuning(**saved)
 
 
+@pytest.mark.parametrize("variant", ["persistent", "persistent_pair"])
 @pytest.mark.parametrize("heavy", [0, 3, 40])
-def test_dop853_42d_wave_wide_forward(gpu, oracle, heavy_threshold, heavy):
-    """Trajectories finished one component per lane (after `heavy` attempts,
-    or in the tail for 0) match the oracle bit for bit, including the
-    non-finite early stops."""
-    say(f"hh_dop853_final dim=42 persistent, heavy={heavy}: 200 trajectories + 4 early stops")
+def test_dop853_42d_wave_wide_forward(gpu, oracle, heavy_threshold, heavy, variant):
+    """Trajectories finished one component per lane or by a pair of waves
+    (after `heavy` attempts, or in the tail for 0) match the oracle bit for
+    bit, including the non-finite early stops."""
+    say(f"hh_dop853_final dim=42 {variant}, heavy={heavy}: 200 trajectories + 4 early stops")
     heavy_threshold(heavy)
     w = W.stm_uneven(n=200)
     w2 = W.persistent_262k()
@@ -208,7 +209,7 @@ def test_dop853_42d_wave_wide_forward(gpu, oracle, heavy_threshold, heavy):
     t0 = np.concatenate([w["t0"], w2["t0"][idx]])
     tf = np.concatenate([w["tf"], w2["tf"][idx]])
     r = batch.dop853_final(mu, y0, t0, tf, dim=42, max_step=1e4, max_steps=200_000,
-                           variant="persistent")
+                           variant=variant)
     want, na, nr, st = oracle.batch_dop853_final(mu, y0, t0, tf, max_step=1e4, max_steps=200_000)
     assert_bitwise(r.status.cpu().numpy(), st, "status")
     assert_bitwise(r.n_accepted.cpu().numpy(), na, "accepted")
@@ -216,7 +217,7 @@ def test_dop853_42d_wave_wide_forward(gpu, oracle, heavy_threshold, heavy):
     assert_bitwise(r.y.cpu().numpy(), want, f"42-d final, heavy={heavy}", rtol=1e-10)
 
 
-@pytest.mark.parametrize("variant", ["persistent", "wide"])
+@pytest.mark.parametrize("variant", ["persistent", "persistent_pair", "wide", "pair"])
 def test_dop853_42d_wave_wide_backward_flip(gpu, oracle, heavy_threshold, variant):
     """Backward _DirectedSystem with the STM-part flip (compute_stm's
     slice(36, 42)) through the one-component-per-lane form."""
@@ -236,6 +237,27 @@ def test_dop853_42d_wave_wide_backward_flip(gpu, oracle, heavy_threshold, varian
                                    atol=1e-12, max_step=1e4)
         assert int(r.n_accepted[i]) == info["n_accepted"]
         assert_bitwise(got[i], want[-1], f"backward traj {i}", rtol=1e-10)
+
+
+@pytest.mark.parametrize("heavy,pair,max_steps", [(3, 0, 200_000), (3, 3, 200_000), (3, 3, 60),
+                                                  (3, 60, 200_000), (0, 0, 200_000)])
+def test_dop853_42d_persistent_pair_resume_queue(gpu, heavy_threshold, heavy, pair, max_steps):
+    """Many moves: every trajectory flagged after 3 attempts sends the other
+    trajectories of its wave (or, past the pair threshold, of its workgroup)
+    through the resume queue, some of them several times; results, counters
+    and statuses equal the plain kernel's (itself checked against the
+    oracle), also when max_steps stops them."""
+    say(f"hh_dop853_final dim=42 persistent_pair heavy={heavy} pair={pair} "
+        f"max_steps={max_steps}: 4096")
+    heavy_threshold(heavy, 1, pair)
+    w = W.stm_uneven(n=4096)
+    kw = dict(dim=42, max_step=1e4, max_steps=max_steps)
+    a = batch.dop853_final(w["mu"], w["y0"], w["t0"], w["tf"], variant="plain", **kw)
+    b = batch.dop853_final(w["mu"], w["y0"], w["t0"], w["tf"], variant="persistent_pair", **kw)
+    assert_bitwise(b.status.cpu().numpy(), a.status.cpu().numpy(), "status")
+    assert_bitwise(b.n_accepted.cpu().numpy(), a.n_accepted.cpu().numpy(), "accepted")
+    assert_bitwise(b.n_rejected.cpu().numpy(), a.n_rejected.cpu().numpy(), "rejected")
+    assert_bitwise(b.y.cpu().numpy(), a.y.cpu().numpy(), "final state")
 
 
 @pytest.mark.parametrize("dim,forward,flip,variant", [
