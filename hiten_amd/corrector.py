@@ -161,9 +161,11 @@ class _Engine:
         phi = np.repeat(np.eye(6)[None], n, axis=0)
         live = ~const
         if live.any():
-            # one wave per trajectory while the batch is small (latency), else
-            # the work-stealing form; both give the same bits
-            variant = "wide" if int(live.sum()) <= 2048 else "persistent"
+            # two waves per trajectory while the batch is small (latency), one
+            # wave per trajectory up to 2048, else the work-stealing form; all
+            # give the same bits
+            k = int(live.sum())
+            variant = "pair" if k <= 1024 else ("wide" if k <= 2048 else "persistent_pair")
             r = batch.dop853_final(mu[live], PHI0[live], 0.0, t_event[live], dim=42,
                                    flip_indices=slice(36, 42), max_step=1e4, variant=variant)
             phi[live] = r.y[:, :36].reshape(-1, 6, 6).cpu().numpy()

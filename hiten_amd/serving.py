@@ -24,6 +24,7 @@ job gives the same bits alone and in the mix.
 from __future__ import annotations
 
 import collections
+import gc
 import math
 import time
 from dataclasses import dataclass, field
@@ -347,8 +348,11 @@ class _StmBatch(_Pending):
             fin[3:39] = _EYE36[:, None]
             fin[39:45] = states[self.live].T
             pk = self.pk = _Packed(fin, 42, 2, slot)
+            # two waves per trajectory (state and STM side by side) while the
+            # batch leaves the GPU mostly idle, else one wave per trajectory
+            var = _lib.HH_VARIANT_PAIR if k <= 1024 else _lib.HH_VARIANT_WIDE
             _launch_final(pk, 42, (0, 1, 2), pk.din(3), pk.fout(0), slot.sh, forward=1,
-                          flip_mask=_FLIP_STM, variant=_lib.HH_VARIANT_WIDE, max_step=1e4,
+                          flip_mask=_FLIP_STM, variant=var, max_step=1e4,
                           istatus=0, inacc=1)
             pk.download(slice(0, 36))
         slot.record()
@@ -515,18 +519,28 @@ class Server:
         """Serve requests (dicts with arrival time ``t``, as
         workloads.serving_stream draws them) in real time; returns the jobs."""
         reqs = sorted(stream_of_requests, key=lambda r: r["t"])
+        # no cyclic garbage collection inside the loop: a full collection
+        # over the loop's objects stalls every request in flight (one is run
+        # before the clock starts)
+        gc_was_on = gc.isenabled()
+        gc.collect()
+        gc.disable()
         self.reset()
         i = 0
-        while i < len(reqs) or self.busy():
-            now = self.now()
-            if now > time_limit:
-                raise TimeoutError(f"serving loop exceeded {time_limit} s")
-            while i < len(reqs) and reqs[i]["t"] <= now:
-                self.submit(*to_request(reqs[i]), arrival=reqs[i]["t"])
-                i += 1
-            if not self.step():
-                time.sleep(20e-6)
-        torch.cuda.synchronize()
+        try:
+            while i < len(reqs) or self.busy():
+                now = self.now()
+                if now > time_limit:
+                    raise TimeoutError(f"serving loop exceeded {time_limit} s")
+                while i < len(reqs) and reqs[i]["t"] <= now:
+                    self.submit(*to_request(reqs[i]), arrival=reqs[i]["t"])
+                    i += 1
+                if not self.step():
+                    time.sleep(20e-6)
+            torch.cuda.synchronize()
+        finally:
+            if gc_was_on:
+                gc.enable()
         return self.jobs
 
 
