@@ -165,8 +165,9 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
     elif name == "stm_uneven":
         w = W.stm_uneven()
         res = {}
-        for variant in ("plain", "persistent", "wide"):
-            b = batch.prepare_final(w["mu"], w["y0"], w["t0"], w["tf"], dim=42)
+        for variant in ("plain", "persistent", "persistent_pair", "wide", "pair"):
+            b = batch.prepare_final(w["mu"], w["y0"], w["t0"], w["tf"], dim=42,
+                                    order="span" if variant.startswith("persistent") else None)
             fn = lambda: batch.launch_final(b, variant=variant, max_step=1e4)
             wall, ev = _timed(fn, steps, warmup)
             acc = int(b.n_accepted.to(torch.int64).sum().item())
@@ -253,6 +254,8 @@ def main():
                     choices=["auto", "persistent", "persistent_pair"],
                     help="auto: autotune variant and knobs on the batch before timing")
     ap.add_argument("--tune-trials", type=int, default=12)
+    ap.add_argument("--refill-min", type=int, default=1)
+    ap.add_argument("--waves-per-simd", type=int, default=2)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--order", default="span", choices=["span", "interleave", "index"])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
@@ -294,7 +297,8 @@ def main():
                                for t in res.trials]}
     else:
         cfg = autotune.Config(variant=args.variant, heavy_projected=args.heavy,
-                              min_age=args.heavy_age, pair_projected=args.pair_heavy)
+                              min_age=args.heavy_age, pair_projected=args.pair_heavy,
+                              refill_min=args.refill_min, waves_per_simd=args.waves_per_simd)
         autotune.apply(cfg)
     kernel = {"persistent": "k_dop853_final42_persist",
               "persistent_pair": "k_dop853_final42_resume"}[cfg.variant]
