@@ -40,7 +40,7 @@ class Tuning(C.Structure):
     """hh_tuning_t: scheduling knobs of the persistent 42-d kernels."""
     _fields_ = [("heavy_projected", C.c_int), ("min_age", C.c_int),
                 ("pair_projected", C.c_int), ("refill_min", C.c_int),
-                ("waves_per_simd", C.c_int)]
+                ("waves_per_simd", C.c_int), ("exclusive_projected", C.c_int)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

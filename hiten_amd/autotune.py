@@ -10,7 +10,8 @@ of the caller's own batch on the GPU:
   (512 VGPRs at one wave per SIMD, 256 at two);
 - ``refill_min``: how many idle groups a wave gathers before it refills;
 - ``heavy_projected`` / ``min_age`` / ``pair_projected``: when a long-running
-  trajectory leaves the 8-per-wave layout for one wave or a pair of waves.
+  trajectory leaves the 8-per-wave layout for one wave or a pair of waves;
+- ``exclusive_projected``: when such a run takes its SIMD for itself.
 
 Every knob changes scheduling only, so each trial's states, step counts and
 statuses are compared bit for bit with the first trial's; a difference raises.
@@ -37,12 +38,15 @@ SPACE: Dict[str, Sequence] = {
     "waves_per_simd": (2, 1),
     "refill_min": (1, 2, 4),
     "pair_projected": (0, 6000),
+    "exclusive_projected": (0, 3000, 2000),
 }
 
-_TUNING_KEYS = ("heavy_projected", "min_age", "pair_projected", "refill_min", "waves_per_simd")
+_TUNING_KEYS = ("heavy_projected", "min_age", "pair_projected", "refill_min", "waves_per_simd",
+                "exclusive_projected")
 _BLOCK = {"persistent": 256, "persistent_pair": 128}
 # knobs a variant does not read (their trials are skipped)
-_UNUSED = {"persistent": {"pair_projected", "refill_min", "waves_per_simd"}}
+_UNUSED = {"persistent": {"pair_projected", "refill_min", "waves_per_simd",
+                          "exclusive_projected"}}
 
 
 @dataclass(frozen=True)
@@ -53,6 +57,7 @@ class Config:
     pair_projected: int = 0
     refill_min: int = 1
     waves_per_simd: int = 2
+    exclusive_projected: int = 0
 
     @property
     def block(self) -> int:

@@ -34,7 +34,7 @@ namespace {
 
 thread_local char g_err[512] = "";
 /* persistent 42-d kernels: see hh_set_tuning */
-hh_tuning_t g_tuning = {2000, 128, 0, 1, 2};
+hh_tuning_t g_tuning = {2000, 128, 0, 1, 2, 0};
 
 int fail(int code, const char* msg) {
     snprintf(g_err, sizeof g_err, "%s", msg);
@@ -795,14 +795,22 @@ void k_dop853_final42_plain(Final42Args a) {
 /* traj_iterate + FinalOut for one trajectory in the wave-wide layout, from
  * the current (y, k[0], t, h, err_prev, counters) to its end; stores the
  * results as store42 does.                                                  */
-template <class WL>
+/* Per-attempt hook of wide_finish: scheduling only, never touches results */
+struct NoHook {
+    __device__ __forceinline__ void attempt(int, double) {}
+    __device__ __forceinline__ void end() {}
+};
+
+template <class WL, class Hook = NoHook>
 __device__ __forceinline__ void wide_finish(const Final42Args& a, const WL& W, int traj,
                                             double y, double (&k)[DOP_K], double t, double tf,
-                                            double h, double err_prev, int nacc, int nrej) {
+                                            double h, double err_prev, int nacc, int nrej,
+                                            Hook hook = Hook()) {
     const StepParams& sp = a.sp;
     double out = y;
     int status = ST_OK;
     for (;;) {
+        hook.attempt(nacc + nrej, t);
         if (nacc + nrej >= sp.max_steps) {
             status = ST_MAXSTEPS;
             out = y;
@@ -838,6 +846,7 @@ __device__ __forceinline__ void wide_finish(const Final42Args& a, const WL& W, i
             nrej += 1;
         }
     }
+    hook.end();
     if (W.holds()) a.yout[(size_t)W.comp() * a.n + traj] = out;
     if (W.leader()) {
         if (a.status) a.status[traj] = status;
@@ -850,8 +859,10 @@ __device__ __forceinline__ void wide_finish(const Final42Args& a, const WL& W, i
  * wave-wide layout: lane 8*b + d takes entry d of block b from lane src + b.
  * Raises the wave's issue priority meanwhile: such a trajectory is the
  * longest-running work on its SIMD.                                         */
+template <class Hook = NoHook>
 __device__ __forceinline__ void run_wide(const Final42Args& a, int lane, int src, int traj_lane,
-                                         const Lane42& p, const Traj<Lane42>& s) {
+                                         const Lane42& p, const Traj<Lane42>& s,
+                                         Hook hook = Hook()) {
     Wave42 W;
     W.setup(lane, rlane(p.mu, src), a.fwd, a.flip);
     const int from = src + (lane >> 3);
@@ -871,7 +882,8 @@ __device__ __forceinline__ void run_wide(const Final42Args& a, int lane, int src
     __builtin_amdgcn_s_setprio(3);
     wide_finish(a, W, __builtin_amdgcn_readlane(traj_lane, src), y, k, rlane(s.t, src),
                 rlane(s.tf, src), rlane(s.h, src), rlane(s.err_prev, src),
-                __builtin_amdgcn_readlane(s.nacc, src), __builtin_amdgcn_readlane(s.nrej, src));
+                __builtin_amdgcn_readlane(s.nacc, src), __builtin_amdgcn_readlane(s.nrej, src),
+                hook);
     __builtin_amdgcn_s_setprio(0);
 }
 
@@ -1058,6 +1070,44 @@ struct ResumeArgs {
     int* rq;
     double* spill;
     int cap;
+    int* simd_tab;   /* SIMD_TAB ints: nonzero = that SIMD is reserved         */
+    int exclusive;   /* projected attempts from which a wave-wide or pair run
+                        reserves its SIMD (0 = never)                          */
+};
+
+/* This wave's SIMD as an index < SIMD_TAB, from the hardware id registers
+ * (XCC, shader engine, array, CU, SIMD).  Only used to find the other waves
+ * that share a SIMD; a wrong layout would cost time, never results.        */
+constexpr int SIMD_TAB = 65536;
+__device__ __forceinline__ int simd_key() {
+    const unsigned hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);    /* HW_ID, 32 bits  */
+    const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | 20);   /* XCC_ID, 4 bits  */
+    const unsigned simd = (hw >> 4) & 3u, cu = (hw >> 8) & 15u, sh = (hw >> 12) & 1u,
+                   se = (hw >> 13) & 7u;
+    return (int)(((xcc & 15u) << 12) | (se << 9) | (sh << 8) | (cu << 4) | simd);
+}
+
+/* Reserves the wave's SIMD for a long run once the trajectory's projected
+ * attempts, age * span / elapsed, reach r.exclusive (checked every 32
+ * attempts): the other waves of the persistent kernel on that SIMD spill
+ * their work and sleep until end() releases it.                            */
+struct SimdReserve {
+    int* tab;
+    int key, threshold, lane;
+    double t0, span;
+    bool held;
+    __device__ __forceinline__ void attempt(int age, double t) {
+        if (held || !tab || threshold <= 0 || (age & 31) != 0 || age == 0) return;
+        if ((double)age * span >= (double)threshold * (t - t0)) {
+            int old = 1;
+            if (lane == 0) old = atomicCAS(&tab[key], 0, 1);
+            held = __builtin_amdgcn_readfirstlane(__shfl(old, 0)) == 0;
+        }
+    }
+    __device__ __forceinline__ void end() {
+        if (held && lane == 0) atomicExch(&tab[key], 0);
+        held = false;
+    }
 };
 
 __host__ __device__ inline long long resume_cap(long long n) { return 15 * n + 64; }
@@ -1139,7 +1189,7 @@ __device__ __forceinline__ void resume42(const Final42Args& a, const ResumeArgs&
 /* Finish the handed-over trajectory with both waves of the workgroup */
 __device__ __forceinline__ void run_pair(const Final42Args& a, const ResumeArgs& r,
                                          const PairHandoff& ho, PairLds* plds, int wid,
-                                         int lane) {
+                                         int lane, int key) {
     Pair42 P;
     P.setup(lane, wid, ho.mu, a.fwd, a.flip, plds);
     const int bb = lane >> 3, dd = lane & 7;
@@ -1148,8 +1198,10 @@ __device__ __forceinline__ void run_pair(const Final42Args& a, const ResumeArgs&
 #pragma unroll
     for (int j = 0; j < DOP_K; ++j) k[j] = 0.0;
     k[0] = P.holds() ? ho.k[bb][dd] : 0.0;
+    SimdReserve hook{r.simd_tab, key, r.exclusive, lane, a.t0[ho.traj],
+                     ho.tf - a.t0[ho.traj], false};
     __builtin_amdgcn_s_setprio(3);
-    wide_finish(a, P, ho.traj, y, k, ho.t, ho.tf, ho.h, ho.err_prev, ho.nacc, ho.nrej);
+    wide_finish(a, P, ho.traj, y, k, ho.t, ho.tf, ho.h, ho.err_prev, ho.nacc, ho.nrej, hook);
     __builtin_amdgcn_s_setprio(0);
     if (P.leader()) atomicSub(&r.ctr[3], 1);
 }
@@ -1182,6 +1234,7 @@ void k_dop853_final42_resume(Final42Args a, ResumeArgs r) {
     const int grp = lane >> 3;
     if (threadIdx.x == 0) pair_state = -1;
     __syncthreads();
+    const int key = simd_key();
     Lane42 p;
     p.blk = b;
     Traj<Lane42> s;
@@ -1248,7 +1301,7 @@ void k_dop853_final42_resume(Final42Args a, ResumeArgs r) {
             }
             traj = -1;
             __syncthreads();
-            run_pair(a, r, ho, &plds, wid, lane);
+            run_pair(a, r, ho, &plds, wid, lane, key);
             __syncthreads();
             if (threadIdx.x == 0) pair_state = -1;
             __syncthreads();
@@ -1262,9 +1315,28 @@ void k_dop853_final42_resume(Final42Args a, ResumeArgs r) {
             const bool moved = grp == (src >> 3);
             spill42(a, r, traj >= 0 && !moved, traj, b, grp, lane, s);
             if (!moved) traj = -1;
-            run_wide(a, lane, src, traj, p, s);
+            const int tr = __builtin_amdgcn_readlane(traj, src);
+            const double t0 = a.t0[tr];
+            run_wide(a, lane, src, traj, p, s,
+                     SimdReserve{r.simd_tab, key, r.exclusive, lane, t0, a.tf[tr] - t0, false});
             if (lane == 0) atomicSub(&r.ctr[3], 1);
             traj = -1;
+            continue;
+        }
+        /* a long run holds this SIMD: hand this wave's work to the others and
+         * sleep until it is released (or a pair claim or the end wakes it) */
+        if (r.exclusive > 0 &&
+            __builtin_amdgcn_readfirstlane(__hip_atomic_load(&r.simd_tab[key], __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT)) != 0) {
+            spill42(a, r, traj >= 0, traj, b, grp, lane, s);
+            traj = -1;
+            while (__builtin_amdgcn_readfirstlane(
+                       __hip_atomic_load(&r.simd_tab[key], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT)) != 0 &&
+                   __hip_atomic_load(&r.ctr[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0 &&
+                   __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                       &pair_state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) < 0)
+                __builtin_amdgcn_s_sleep(64);
             continue;
         }
         /* refill: the resume queue first, then fresh work; a wave that still
@@ -1516,9 +1588,11 @@ int hh_abi_version(void) { return HH_ABI_VERSION; }
 int hh_set_tuning(const hh_tuning_t* t) {
     if (!t) return fail(HH_ERR_ARGS, "hh_set_tuning: null pointer");
     if (t->heavy_projected < 0 || t->min_age < 1 || t->pair_projected < 0 ||
-        t->refill_min < 1 || t->refill_min > 8 || t->waves_per_simd < 1 || t->waves_per_simd > 2)
+        t->refill_min < 1 || t->refill_min > 8 || t->waves_per_simd < 1 ||
+        t->waves_per_simd > 2 || t->exclusive_projected < 0)
         return fail(HH_ERR_ARGS, "hh_set_tuning: heavy_projected >= 0, min_age >= 1, "
-                                 "pair_projected >= 0, refill_min in 1..8, waves_per_simd in 1..2");
+                                 "pair_projected >= 0, refill_min in 1..8, waves_per_simd in "
+                                 "1..2, exclusive_projected >= 0");
     g_tuning = *t;
     return HH_OK;
 }
@@ -1602,7 +1676,8 @@ long long hh_dop853_final_workspace(int n, int dim, int variant) {
     if (n <= 0 || dim != 42) return 0;
     if (variant == HH_VARIANT_PERSISTENT) return 256;
     if (variant == HH_VARIANT_PERSISTENT_PAIR)
-        return ws_align((RQ_CTR + resume_cap(n)) * 4) + (long long)SPILL_W * n * 8;
+        return ws_align((RQ_CTR + resume_cap(n)) * 4) + ws_align((long long)SPILL_W * n * 8) +
+               (long long)SIMD_TAB * 4;
     return 0;
 }
 
@@ -1684,6 +1759,8 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
         r.rq = r.ctr + RQ_CTR;
         r.cap = (int)resume_cap(n);
         r.spill = (double*)((char*)workspace + ws_align((long long)(RQ_CTR + r.cap) * 4));
+        r.simd_tab = (int*)((char*)r.spill + ws_align((long long)SPILL_W * n * 8));
+        r.exclusive = g_tuning.exclusive_projected;
         const int wpe = g_tuning.waves_per_simd;
         int g = grid;
         if (g <= 0) {
@@ -1699,6 +1776,7 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, r.ctr, 0, RQ_CTR);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, r.ctr + 3, n, 1);
         hipLaunchKernelGGL(k_fill_int, dim3((r.cap + 255) / 256), dim3(256), 0, st, r.rq, -1, r.cap);
+        hipLaunchKernelGGL(k_fill_int, dim3(SIMD_TAB / 256), dim3(256), 0, st, r.simd_tab, 0, SIMD_TAB);
         if (wpe == 1)
             hipLaunchKernelGGL((k_dop853_final42_resume<128, 1>), dim3(g), dim3(128), 0, st, a, r);
         else

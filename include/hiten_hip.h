@@ -66,13 +66,18 @@ int hh_abi_version(void);
  *                     with live groups gathers before it refills; default 1
  *   waves_per_simd  : HH_VARIANT_PERSISTENT_PAIR: 1 or 2 (default): the
  *                     register budget per lane (512 or 256 VGPRs) and the
- *                     resident grid                                        */
+ *                     resident grid
+ *   exclusive_projected : HH_VARIANT_PERSISTENT_PAIR: from this projected
+ *                     total a wave-wide or pair run reserves its SIMD; the
+ *                     other waves on it hand their work to the resume queue
+ *                     and sleep until it ends.  0 = never (default)       */
 typedef struct {
     int heavy_projected;
     int min_age;
     int pair_projected;
     int refill_min;
     int waves_per_simd;
+    int exclusive_projected;
 } hh_tuning_t;
 int hh_set_tuning(const hh_tuning_t* t);
 int hh_get_tuning(hh_tuning_t* t);

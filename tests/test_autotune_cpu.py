@@ -45,7 +45,7 @@ def test_config_maps_to_tuning_and_block():
     c = A.Config("persistent", 1500, 64, 0, 1, 2)
     assert c.block == 256 and A.Config().block == 128
     assert c.tuning() == {"heavy_projected": 1500, "min_age": 64, "pair_projected": 0,
-                          "refill_min": 1, "waves_per_simd": 2}
+                          "refill_min": 1, "waves_per_simd": 2, "exclusive_projected": 0}
 
 
 def test_tuning_is_validated_by_the_library():

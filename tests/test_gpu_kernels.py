@@ -239,17 +239,19 @@ def test_dop853_42d_wave_wide_backward_flip(gpu, oracle, heavy_threshold, varian
         assert_bitwise(got[i], want[-1], f"backward traj {i}", rtol=1e-10)
 
 
-@pytest.mark.parametrize("heavy,pair,max_steps", [(3, 0, 200_000), (3, 3, 200_000), (3, 3, 60),
-                                                  (3, 60, 200_000), (0, 0, 200_000)])
-def test_dop853_42d_persistent_pair_resume_queue(gpu, heavy_threshold, heavy, pair, max_steps):
+@pytest.mark.parametrize("heavy,pair,max_steps,exclusive", [
+    (3, 0, 200_000, 0), (3, 3, 200_000, 0), (3, 3, 60, 0), (3, 60, 200_000, 0),
+    (0, 0, 200_000, 0), (3, 0, 200_000, 3), (3, 60, 200_000, 3)])
+def test_dop853_42d_persistent_pair_resume_queue(gpu, heavy_threshold, heavy, pair, max_steps,
+                                                 exclusive):
     """Many moves: every trajectory flagged after 3 attempts sends the other
     trajectories of its wave (or, past the pair threshold, of its workgroup)
     through the resume queue, some of them several times; results, counters
     and statuses equal the plain kernel's (itself checked against the
     oracle), also when max_steps stops them."""
     say(f"hh_dop853_final dim=42 persistent_pair heavy={heavy} pair={pair} "
-        f"max_steps={max_steps}: 4096")
-    heavy_threshold(heavy, 1, pair)
+        f"max_steps={max_steps} exclusive={exclusive}: 4096")
+    heavy_threshold(heavy, 1, pair, exclusive_projected=exclusive)
     w = W.stm_uneven(n=4096)
     kw = dict(dim=42, max_step=1e4, max_steps=max_steps)
     a = batch.dop853_final(w["mu"], w["y0"], w["t0"], w["tf"], variant="plain", **kw)
