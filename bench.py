@@ -186,6 +186,41 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
                    variants=res)
         if not no_cpu:
             out["cpu_baseline"] = cpu_baseline(w, threads, target_s=8.0)
+    elif name == "synodic":
+        from hiten_amd import synodic
+        n, m, tf = 4096, 4713, 6.0
+        w = W.stm_uneven(n=n)
+        grid = batch.dop853_grid(w["mu"], w["y0"][:, 36:], 0.0, tf, m, dim=6, max_step=1e4)
+        states = grid.states
+        times = torch.linspace(0.0, tf, m, dtype=torch.float64, device=states.device)
+        kw = dict(normal=synodic.axis_normal("y"), plane_coords=("y", "vy"),
+                  interp_kind="cubic", segment_refine=20, newton_max_iter=25)
+        res = synodic.detect_arrays(times, states, **kw)
+        wall, ev = _timed(res.relaunch, steps, warmup)
+        hits = int(res.count.to(torch.int64).sum().item())
+        ls = sum(ev) / len(ev)
+        nbytes = n * m * 48 + m * 8
+        out.update(metric="trajectory samples scanned/s, section detection (cubic, "
+                          "segment_refine 20)", value=n * m * steps / wall,
+                   unit="samples/s", ms_per_step=wall / steps * 1e3, hits_per_pass=hits,
+                   config={"workload": f"synodic: {n} x {m} samples of 6-d DOP853 arcs to "
+                                       f"t = {tf}, plane y = 0, cubic Hermite + Newton 25, "
+                                       "segment_refine 20", "kernel": "k_synodic_detect"},
+                   roofline={"bound": "hbm", "achieved": nbytes / ls / 1e9, "peak": 8000.0,
+                             "unit": "GB/s", "frac": nbytes / ls / 1e9 / 8000.0,
+                             "traffic": None})
+        if not no_cpu:
+            from oracle import synodic as OS
+            host = states[:2].cpu().numpy()
+            tt = times.cpu().numpy()
+            t = time.perf_counter()
+            for i in range(2):
+                OS.detect(tt, host[i], normal=kw["normal"], plane_idx=(1, 4),
+                          interp_kind="cubic", segment_refine=20, newton_max_iter=25)
+            dt = time.perf_counter() - t
+            out["cpu_baseline"] = {"value": 2 * m / dt, "unit": "samples/s", "cores": 1,
+                                   "kind": "port",
+                                   "sample": f"2 of {n} trajectories, pure-Python restatement"}
     elif name == "manifold_fan":
         w = W.manifold_fan()
         n = w["y0"].shape[0]
@@ -260,7 +295,8 @@ def main():
     ap.add_argument("--order", default="span", choices=["span", "interleave", "index"])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--workload", default="persistent_262k",
-                    choices=["persistent_262k", "rk8_grid", "stm_uneven", "manifold_fan"])
+                    choices=["persistent_262k", "rk8_grid", "stm_uneven", "manifold_fan",
+                             "synodic"])
     args = ap.parse_args()
 
     if args.gpus != 1:

@@ -21,7 +21,8 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libhiten_hip.so")
 SOURCES = [os.path.join(CSRC, "hh_kernels.hip"), os.path.join(CSRC, "hh_bank.cpp")]
 DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("hh_device.h", "hh_wave.h", "hh_pow.h",
-                                                  "hh_tables.h")] + [
+                                                  "hh_tables.h", "hh_pair.h", "hh_group6.h",
+                                                  "hh_synodic.h")] + [
     os.path.join(ROOT, "include", "hiten_hip.h")]
 
 ARCH = os.environ.get("HH_OFFLOAD_ARCH", "gfx950")

@@ -20,6 +20,7 @@
 #include "hh_wave.h"
 #include "hh_pair.h"
 #include "hh_group6.h"
+#include "hh_synodic.h"
 #include "../../include/hiten_hip.h"
 
 #include <stdio.h>
@@ -1902,6 +1903,54 @@ int hh_eval_math(int n, const double* x, const double* y, double* out, void* str
     hipLaunchKernelGGL(k_eval_math, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream,
                        n, x, y, out);
     return check_launch("hh_eval_math launch");
+}
+
+int hh_synodic_detect(int n, int m, const double* times, int per_traj_times,
+                      const int* lengths, const double* states, const double* normal,
+                      double offset, int direction, int plane_i, int plane_j, int cubic,
+                      int segment_refine, double tol_on_surface, double dedup_time_tol,
+                      double dedup_point_tol, int max_hits, int newton_max_iter, int cap,
+                      int* count, double* hit_t, double* hit_x, double* hit_p, void* stream) {
+    if (n < 0 || m < 0) return fail(HH_ERR_ARGS, "hh_synodic_detect: n < 0 or m < 0");
+    if (n == 0) return HH_OK;
+    if (!times || !states || !normal || !count)
+        return fail(HH_ERR_ARGS, "hh_synodic_detect: null pointer");
+    if (cap < 0 || (cap > 0 && (!hit_t || !hit_x || !hit_p)))
+        return fail(HH_ERR_ARGS, "hh_synodic_detect: cap < 0 or null hit arrays");
+    if (direction < -1 || direction > 1)
+        return fail(HH_ERR_ARGS, "hh_synodic_detect: direction must be -1, 0 or 1");
+    if (plane_i < 0 || plane_i > 5 || plane_j < 0 || plane_j > 5)
+        return fail(HH_ERR_ARGS, "hh_synodic_detect: plane indices must be 0..5");
+    if (segment_refine < 0 || newton_max_iter < 0 || max_hits < 0)
+        return fail(HH_ERR_ARGS, "hh_synodic_detect: negative segment_refine, "
+                                 "newton_max_iter or max_hits");
+    SynArgs a;
+    a.n = n;
+    a.m = m;
+    a.times = times;
+    a.per_traj_times = per_traj_times ? 1 : 0;
+    a.lengths = lengths;
+    a.states = states;
+    for (int d = 0; d < 6; ++d) a.nrm[d] = normal[d];
+    a.offset = offset;
+    a.direction = direction;
+    a.pi = plane_i;
+    a.pj = plane_j;
+    a.cubic = cubic ? 1 : 0;
+    a.refine = segment_refine;
+    a.newton_max_iter = newton_max_iter;
+    a.max_hits = max_hits;
+    a.cap = cap;
+    a.tol_on = tol_on_surface;
+    a.dedup_t = dedup_time_tol;
+    a.dedup_p = dedup_point_tol;
+    a.count = count;
+    a.hit_t = hit_t;
+    a.hit_x = hit_x;
+    a.hit_p = hit_p;
+    hipLaunchKernelGGL(k_synodic_detect<256>, dim3(blocks_for((long long)n * 64, 256)), dim3(256),
+                       0, (hipStream_t)stream, a);
+    return check_launch("hh_synodic_detect launch");
 }
 
 int hh_traj_filters(int n, int m, const double* mu, const double* states, double* min_r1,

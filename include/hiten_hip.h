@@ -199,6 +199,28 @@ int hh_eval_rhs(int n, int dim, const double* mu, const double* y, double* out, 
  * obtains from libm: np.sqrt, '/', and '**' in rtbp.py / utils.py).       */
 int hh_eval_math(int n, const double* x, const double* y, double* out, void* stream);
 
+/* Crossings of an affine section n . x = offset by stored trajectories
+ * (synodic Poincare maps): trajectory i is samples 0 .. lengths[i]-1 (all m
+ * when lengths is NULL) of states[i][*][6] at times[*] (shared) or
+ * times[i][*] (per_traj_times).  Detection, refinement (linear or cubic
+ * Hermite + Newton, optional segment_refine sub-intervals), direction filter
+ * and deduplication as the reference; hits in time order.
+ * Replaces: _SynodicDetectionBackend.detect_on_trajectory / run
+ * (src/hiten/algorithms/poincare/synodic/backend.py:644-887).
+ *   normal     : host pointer to 6 doubles
+ *   direction  : 0 (both), 1 (n . x - offset rising), -1 (falling)
+ *   plane_i/j  : state components of each hit's 2-d point
+ *   max_hits   : hits kept per trajectory at most (0 = no limit)
+ *   cap        : hit slots per trajectory in hit_t [n][cap], hit_x
+ *                [n][cap][6], hit_p [n][cap][2]; count[i] is the number of
+ *                hits kept, which may exceed cap (then only cap are stored) */
+int hh_synodic_detect(int n, int m, const double* times, int per_traj_times,
+                      const int* lengths, const double* states, const double* normal,
+                      double offset, int direction, int plane_i, int plane_j, int cubic,
+                      int segment_refine, double tol_on_surface, double dedup_time_tol,
+                      double dedup_point_tol, int max_hits, int newton_max_iter, int cap,
+                      int* count, double* hit_t, double* hit_x, double* hit_p, void* stream);
+
 /* Filters of a dense-output state grid states[n][m][6] (row-major per
  * trajectory), as the manifold computation applies them to every arc:
  * min_r1[i], min_r2[i] = minimum distance to the primaries, NaN if any
