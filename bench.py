@@ -92,9 +92,10 @@ def serving_run(seconds: float, rate: float = 2000.0):
     t = time.perf_counter()
     jobs = server.run(reqs, time_limit=600.0)
     wall = time.perf_counter() - t
+    server.close()
     out = S.summarize(jobs, wall, server.steps)
     out.update(requests=len(reqs), rate_per_s=rate, seconds_of_arrivals=seconds,
-               max_queue=server.max_queue)
+               max_queue=server.max_queue, corrector=server.corrector)
     return out
 
 

@@ -71,6 +71,10 @@ _SIGNATURES = {
     "hh_eval_math": ([_I, _P, _P, _P, _P], _I),
     "hh_traj_filters": ([_I, _I, _P, _P, _P, _P, _P, _P], _I),
     "hh_memcpy_async": ([_P, _P, _ULL, _I, _P], _I),
+    "hh_stream_create_dedicated": ([], _P),
+    "hh_stream_destroy": ([_P], _I),
+    "hh_host_alloc": ([_ULL, C.POINTER(_P)], _I),
+    "hh_host_free": ([_P], _I),
     "hh_event_create": ([], _P),
     "hh_event_destroy": ([_P], None),
     "hh_event_record": ([_P, _P], _I),
@@ -87,6 +91,9 @@ _SIGNATURES = {
     "hh_bank_set_steps": ([_P, _I, _P, _P], _I),
     "hh_bank_result": ([_P, _I, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "hh_bank_release": ([_P, _I], _I),
+    "hh_correct": ([_I, _P, _P, _D, _D, _D, _D, _P], _I),
+    "hh_correct_sizes": ([_P, _P], _I),
+    "hh_eval_solve2": ([_I, _P, _P, _P, _P, _P], _I),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -187,6 +187,7 @@ def solve_delta_stack(J: np.ndarray, r: np.ndarray, cond_threshold: float = 1e8)
         return np.stack([solve_delta_dense(a, b, cond_threshold) for a, b in zip(J, r)])
 
 
+HH_BANK_MAX_ITERATIONS = 2
 _ERRORS = {
     1: "no section crossing for the current state",
     3: "Armijo line search failed to find a productive step.",
@@ -325,7 +326,7 @@ class _Bank:
                                            C.byref(ne), C.byref(ns), C.byref(ok), C.byref(err),
                                            C.byref(hp)), "hh_bank_result")
         p = self.problems[row]
-        if err.value == 2:
+        if err.value == HH_BANK_MAX_ITERATIONS:
             msg = (f"Newton did not converge after {p.max_attempts} iterations "
                    f"(|R|={rn.value:.2e}).")
         else:
@@ -380,9 +381,122 @@ def correct_batch(problems: Sequence[CorrectionProblem], engine: _Engine | None 
     return out
 
 
-def correct(problem: CorrectionProblem, **kw) -> CorrectionResult:
-    """Correct one orbit; raises ConvergenceError like _NewtonBackend.run."""
-    r = correct_batch([problem], **kw)[0]
+# ---- device-resident corrector (hh_correct) --------------------------------
+
+#: hh_correction_t / hh_correction_result_t (include/hiten_hip.h), C layout
+PROBLEM_DTYPE = np.dtype([
+    ("mu", "f8"), ("x0", "f8", 6), ("target", "f8", 6), ("tol", "f8"), ("max_delta", "f8"),
+    ("alpha_reduction", "f8"), ("min_alpha", "f8"), ("armijo_c", "f8"),
+    ("residual_indices", "i4", 6), ("control_indices", "i4", 6), ("n_residuals", "i4"),
+    ("n_controls", "i4"), ("halo", "i4"), ("forward", "i4"), ("max_attempts", "i4"),
+    ("reserved", "i4")], align=True)
+RESULT_DTYPE = np.dtype([
+    ("x", "f8", 6), ("residual_norm", "f8"), ("half_period", "f8"), ("iterations", "i4"),
+    ("n_event_searches", "i4"), ("n_stm", "i4"), ("converged", "i4"), ("error", "i4"),
+    ("n_steps", "i4"), ("done", "i4"), ("reserved", "i4")], align=True)
+HH_CORRECT_INVALID = 4
+_JMAX = 64
+
+
+def line_search_steps(p: CorrectionProblem) -> int:
+    """Number of trial steps alpha = 1, r, r^2, ... >= min_alpha."""
+    a, n = 1.0, 0
+    while a >= p.min_alpha and n <= _JMAX:
+        n += 1
+        a = a * p.alpha_reduction
+    return n
+
+
+def device_supported(p: CorrectionProblem) -> bool:
+    """Whether hh_correct takes the problem: a square 2x2 system (Halo,
+    Lyapunov) with at most 64 line-search steps."""
+    return (len(p.residual_indices) == 2 and len(p.control_indices) == 2
+            and all(0 <= int(i) <= 5 for i in (*p.residual_indices, *p.control_indices))
+            and line_search_steps(p) <= _JMAX)
+
+
+def pack_problems(problems: Sequence[CorrectionProblem]) -> np.ndarray:
+    """hh_correction_t records of the problems."""
+    out = np.zeros(len(problems), dtype=PROBLEM_DTYPE)
+    for i, p in enumerate(problems):
+        if not device_supported(p):
+            raise ValueError(f"problem {i}: the device corrector takes 2x2 problems with at "
+                             f"most {_JMAX} line-search steps")
+        r = out[i]
+        r["mu"] = float(p.mu)
+        r["x0"] = np.asarray(p.x0, float)
+        r["target"][:2] = np.asarray(p.target, float)
+        r["tol"] = float(p.tol)
+        r["max_delta"] = math.inf if p.max_delta is None else float(p.max_delta)
+        r["alpha_reduction"] = float(p.alpha_reduction)
+        r["min_alpha"] = float(p.min_alpha)
+        r["armijo_c"] = float(p.armijo_c)
+        r["residual_indices"][:2] = list(p.residual_indices)
+        r["control_indices"][:2] = list(p.control_indices)
+        r["n_residuals"] = r["n_controls"] = 2
+        r["halo"] = int(p.extra_jacobian == "halo")
+        r["forward"] = int(p.forward)
+        r["max_attempts"] = int(p.max_attempts)
+    return out
+
+
+def results_from_records(rec: np.ndarray, problems: Sequence[CorrectionProblem]
+                         ) -> list[CorrectionResult]:
+    """CorrectionResult of each hh_correction_result_t record."""
+    out = []
+    for r, p in zip(rec, problems):
+        err = int(r["error"])
+        if err == HH_BANK_MAX_ITERATIONS:
+            msg = (f"Newton did not converge after {p.max_attempts} iterations "
+                   f"(|R|={float(r['residual_norm']):.2e}).")
+        elif err == HH_CORRECT_INVALID:
+            msg = "invalid correction problem"
+        else:
+            msg = _ERRORS.get(err, "")
+        out.append(CorrectionResult(bool(r["converged"]), np.array(r["x"], dtype=float),
+                                    float(r["residual_norm"]), int(r["iterations"]),
+                                    half_period=float(r["half_period"]), error=msg,
+                                    n_event_searches=int(r["n_event_searches"]),
+                                    n_stm=int(r["n_stm"])))
+    return out
+
+
+def launch_device(prob_ptr: int, res_ptr: int, n: int, stream=None,
+                  stream_handle: Optional[int] = None) -> None:
+    """hh_correct on device (or mapped) records: ``n`` problems at
+    ``prob_ptr``, results to ``res_ptr`` (their done flags cleared); on a
+    torch stream or a raw stream handle."""
+    from . import _lib
+    from .singlehit import _default_windows
+    (ts, span), (ts2, span2) = _default_windows()
+    lib = _lib.load()
+    sh = _lib.stream_handle(stream) if stream_handle is None else stream_handle
+    _lib.check(lib.hh_correct(int(n), prob_ptr, res_ptr, ts, span, ts2, span2, sh),
+               "hh_correct")
+
+
+def correct_device(problems: Sequence[CorrectionProblem], stream=None) -> list[CorrectionResult]:
+    """Every problem's whole Newton-Armijo loop in one launch (hh_correct):
+    one workgroup per correction, no host round trip between iterations.
+    Agrees with correct_batch to the corrector's tolerance (the Halo term's
+    powers are correctly rounded on the device)."""
+    import torch
+    n = len(problems)
+    if n == 0:
+        return []
+    dev = batch._device()
+    rec = pack_problems(problems)
+    prob = torch.from_numpy(rec.view(np.uint8)).to(dev)
+    res = torch.zeros(n * RESULT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    launch_device(prob.data_ptr(), res.data_ptr(), n, stream)
+    host = res.cpu().numpy().view(RESULT_DTYPE)
+    return results_from_records(host, problems)
+
+
+def correct(problem: CorrectionProblem, device: bool = False, **kw) -> CorrectionResult:
+    """Correct one orbit; raises ConvergenceError like _NewtonBackend.run.
+    ``device=True`` runs the whole loop in one launch (correct_device)."""
+    r = correct_device([problem])[0] if device else correct_batch([problem], **kw)[0]
     if not r.converged:
         if "Armijo" in r.error:
             raise ConvergenceError(f"Step strategy failed to produce an update: {r.error}")

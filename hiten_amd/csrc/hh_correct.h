@@ -1,0 +1,339 @@
+/* hh_correct.h -- the device-resident single-shooting corrector.
+ *
+ * One workgroup of two waves runs one correction's whole Newton-Armijo loop
+ * (the row functions of hh_newton.h: corrector/backends/newton.py:39-143,
+ * stepping/armijo.py:58-164) without returning to the host:
+ *
+ *   residual   _y_plane_crossing (poincare/singlehit/backend.py:237-280):
+ *              the first window, then the fallback window (alignment
+ *              propagation + event search), in the Group6 layout -- the
+ *              sixteen 8-lane groups of the workgroup evaluate the residual,
+ *              or all line-search trials of a step at once (in chunks of
+ *              sixteen, stopping as soon as a trial in order is accepted);
+ *   Jacobian   Phi(t_event) from [I | x] (corrector/operators.py:298-307,
+ *              437-452) in the Pair42 layout (state wave and STM wave), then
+ *              the Halo term (services/orbits.py:917-947);
+ *   step       the 2x2 dense solve with its regularisation
+ *              (corrector/backends/base.py:187-245), as NumPy's LAPACK
+ *              forms it (corr_solve2), capped and spread over the trials.
+ *
+ * The integrations are the library's kernels' loops (group6_run,
+ * group6_event, pair_final), so each evaluated state's bits are those of the
+ * batched path.  Two differences from the host-driven bank remain: the Halo
+ * term's powers are correctly rounded here (the C library's pow is within
+ * 0.52 ulp), and an exactly singular system takes a closed-form 2x2
+ * minimum-norm solution where NumPy calls LAPACK's dgelsd.  Either can move
+ * the iterates by an ulp, so results agree with the bank to the
+ * corrector's tolerance, not bit for bit.
+ *
+ * Included by hh_kernels.hip after the integrator loops it uses.
+ */
+#ifndef HH_CORRECT_H
+#define HH_CORRECT_H
+
+#include <float.h>
+
+#include "hh_newton.h"
+
+constexpr int CORR_GROUPS = 16;   /* 8-lane groups of the two waves */
+
+struct CorrArgs {
+    int n;
+    const hh_correction_t* prob;
+    hh_correction_result_t* res;
+    double ts[2], span[2];        /* the two search windows (start, length) */
+    int aligned[2], searched[2];  /* not numpy.isclose(0, start / length)    */
+    StepParams sp_align, sp_event, sp_stm;
+    EventParams ev;
+    unsigned long long flip_align, flip_stm;
+};
+
+struct CorrLds {
+    hh::NewtonRow row;
+    double t[hh::NEWTON_JMAX];
+    double xe[hh::NEWTON_JMAX][6];
+    int ok[hh::NEWTON_JMAX];
+    double phi[42];
+    int n_eval;   /* states to evaluate this round: 1 (residual) or nal (trials) */
+    int steps;    /* accepted integration steps so far                           */
+    int more;     /* 1 while the line search is not yet decided                  */
+    int fail;     /* the problem is not one this kernel takes                    */
+};
+
+/* The Halo term's powers, correctly rounded: squares of coordinates of
+ * either sign, and s^1.5 of a sum of squares (hh_pow takes positive bases). */
+struct DevPow {
+    __device__ double operator()(double a, double b) const {
+        if (b == 2.0) return a * a;
+        if (b == 1.5) return hh_pow15(a);
+        return hh_pow(a, b);
+    }
+};
+
+/* numpy.linalg.cond(J) (2-norm) of a 2x2 matrix: sigma_max^2 / |det|, with
+ * sigma_max = (|(a + d, c - b)| + |(a - d, c + b)|) / 2 and the determinant
+ * formed with one rounding error (fma), so the ratio is accurate to a few
+ * ulps at any condition (NumPy's, from dgesdd, to about eps * cond: near the
+ * 1e8 threshold the two differ by ~2e-8 relative); NaN for NaN entries, inf
+ * for a singular matrix.                                                  */
+__device__ __forceinline__ double corr_cond2(const double (&J)[4]) {
+    const double a = J[0], b = J[1], c = J[2], d = J[3];
+    if (a != a || b != b || c != c || d != d) return NAN;
+    const double p = a + d, q = c - b, u = a - d, v = c + b;
+    const double smax = 0.5 * (__builtin_sqrt(p * p + q * q) + __builtin_sqrt(u * u + v * v));
+    const double w = b * c;
+    const double det = __builtin_fma(a, d, -w) + __builtin_fma(-b, c, w);
+    if (smax == 0.0 || det == 0.0) return INFINITY;
+    return smax / (__builtin_fabs(det) / smax);
+}
+
+/* numpy.linalg.solve(A, b) for 2x2: LAPACK dgesv as OpenBLAS forms it --
+ * getf2 (partial pivoting on the first largest magnitude, the multiplier
+ * scaled by the pivot's reciprocal, the Schur complement a11 - l a01), then
+ * the unit-lower and upper triangular solves (fused multiply-adds in the
+ * axpy updates, divisions by the diagonal).  Pinned against NumPy on random
+ * systems.  Returns false for an exactly singular U (LinAlgError).        */
+__device__ __forceinline__ bool corr_lu2(const double (&A)[4], const double (&b)[2],
+                                         double (&x)[2]) {
+    double a00 = A[0], a01 = A[1], a10 = A[2], a11 = A[3], b0 = b[0], b1 = b[1];
+    if (__builtin_fabs(a10) > __builtin_fabs(a00)) {
+        double s = a00; a00 = a10; a10 = s;
+        s = a01; a01 = a11; a11 = s;
+        s = b0; b0 = b1; b1 = s;
+    }
+    if (a00 == 0.0) return false;
+    const double l = (__builtin_fabs(a00) >= DBL_MIN) ? a10 * (1.0 / a00) : a10 / a00;
+    const double u11 = a11 - l * a01;
+    if (u11 == 0.0) return false;
+    const double y1 = __builtin_fma(-l, b0, b1);
+    const double x1 = y1 / u11;
+    x[1] = x1;
+    x[0] = __builtin_fma(-a01, x1, b0) / a00;
+    return true;
+}
+
+/* numpy.linalg.lstsq(A, b, rcond=None) for 2x2: the minimum-norm solution
+ * over the singular values above eps * 2 * sigma_max, from a closed-form SVD
+ * (one Jacobi rotation of A^T A).  LAPACK's dgelsd forms it differently:
+ * agreement to rounding, not bits.                                         */
+__device__ __forceinline__ void corr_lstsq2(const double (&A)[4], const double (&b)[2],
+                                            double (&x)[2]) {
+    const double a = A[0], bb = A[1], c = A[2], d = A[3];
+    const double p = a * a + c * c, q = a * bb + c * d, r = bb * bb + d * d;
+    const double th = 0.5 * atan2(2.0 * q, p - r);
+    const double cs = cos(th), sn = sin(th);
+    const double v[2][2] = {{cs, sn}, {-sn, cs}};
+    double s[2], av[2][2];
+    for (int i = 0; i < 2; ++i) {
+        av[i][0] = a * v[i][0] + bb * v[i][1];
+        av[i][1] = c * v[i][0] + d * v[i][1];
+        s[i] = __builtin_sqrt(av[i][0] * av[i][0] + av[i][1] * av[i][1]);
+    }
+    const double smax = s[0] > s[1] ? s[0] : s[1];
+    const double tol = (2.0 * DBL_EPSILON) * smax;
+    x[0] = 0.0;
+    x[1] = 0.0;
+    for (int i = 0; i < 2; ++i) {
+        if (!(s[i] > tol)) continue;
+        const double g = (av[i][0] * b[0] + av[i][1] * b[1]) / (s[i] * s[i]);
+        x[0] += v[i][0] * g;
+        x[1] += v[i][1] * g;
+    }
+}
+
+/* _solve_delta_dense (backends/base.py:187-245) for a square 2x2 system:
+ * Tikhonov 1e-12 on the diagonal when cond(J) is NaN or above 1e8, then
+ * solve, or least squares when the regularised system is singular.       */
+__device__ __forceinline__ void corr_solve2(const double (&J)[4], const double (&r)[2],
+                                            double (&d)[2]) {
+    const double cj = corr_cond2(J);
+    const bool reg = (cj != cj) || cj > 1e8;
+    /* J + numpy.eye(2) * 1e-12: the off-diagonal entries gain +0.0 */
+    const double A[4] = {reg ? J[0] + 1e-12 : J[0], reg ? J[1] + 0.0 : J[1],
+                         reg ? J[2] + 0.0 : J[2], reg ? J[3] + 1e-12 : J[3]};
+    const double b[2] = {-r[0], -r[1]};
+    if (!corr_lu2(A, b, d)) corr_lstsq2(A, b, d);
+}
+
+/* corr_solve2 and corr_cond2 per thread, for tests against NumPy */
+__global__ void k_eval_solve2(int n, const double* __restrict__ J, const double* __restrict__ r,
+                              double* __restrict__ delta, double* __restrict__ cond) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double Ji[4] = {J[4 * i], J[4 * i + 1], J[4 * i + 2], J[4 * i + 3]};
+    const double ri[2] = {r[2 * i], r[2 * i + 1]};
+    double d[2];
+    corr_solve2(Ji, ri, d);
+    delta[2 * i] = d[0];
+    delta[2 * i + 1] = d[1];
+    cond[i] = corr_cond2(Ji);
+}
+
+/* _y_plane_crossing of one state in this lane's 8-lane group (y: this
+ * lane's component): the first window, then the fallback one, each an
+ * optional alignment propagation to the window's start and an event search
+ * over its length (singlehit/backend.py:202-280).  ok, t and (this lane's
+ * component of) x_event as the batched path forms them.                   */
+__device__ __forceinline__ void corr_residual(const Group6& ga, const Group6& ge,
+                                              const CorrArgs& a, double y, int& ok, double& t,
+                                              double& xe, int& steps) {
+    ok = 0;
+    t = NAN;
+    xe = NAN;
+    steps = 0;
+    for (int w = 0; w < 2; ++w) {
+        double ya = y;
+        if (a.aligned[w]) {
+            int st, na, nr;
+            group6_run<false>(ga, y, 0.0, a.ts[w], a.sp_align, nullptr, 0, nullptr, ya, st, na,
+                              nr);
+            steps += na;
+        }
+        if (!a.searched[w]) continue;
+        EventOut6 o;
+        group6_event(ge, ya, 0.0, a.span[w], a.sp_event, a.ev, o);
+        steps += o.nacc;
+        const double t_rel = o.found ? o.t_hit : a.span[w];
+        if (o.found && t_rel < a.span[w] && t_rel >= 0.0) {
+            ok = 1;
+            t = a.ts[w] + t_rel;
+            xe = o.y_hit;
+            return;
+        }
+    }
+}
+
+/* One correction per 128-thread workgroup.  Thread 0 keeps the row in LDS
+ * and takes the decisions between barriers; every branch the two waves take
+ * on LDS values is read after a barrier, so all threads meet every barrier
+ * (the Pair42 integration's included).  Each loop ends: every integration
+ * is bounded by max_steps and the Newton loop by max_attempts.             */
+__global__ __launch_bounds__(128) void k_correct(CorrArgs a) {
+    __shared__ PairLds plds;
+    __shared__ CorrLds L;
+    const int i = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int grp = tid >> 3;
+    const int role = __builtin_amdgcn_readfirstlane((int)(tid >> 6));
+    if (i >= a.n) return;
+    if (tid == 0) {
+        const hh_correction_t& pb = a.prob[i];
+        L.fail = 1;
+        if (pb.n_residuals == 2 && pb.n_controls == 2)
+            L.fail = hh::newton_init(L.row, 2, 2, pb.mu, pb.x0, pb.residual_indices,
+                                     pb.control_indices, pb.target, pb.halo, pb.forward,
+                                     pb.max_attempts, pb.tol, pb.max_delta, pb.alpha_reduction,
+                                     pb.min_alpha, pb.armijo_c) != 0;
+        L.n_eval = 1;
+        L.steps = 0;
+    }
+    __syncthreads();
+    if (!L.fail) {
+        const double mu = L.row.mu;
+        Group6 ga, ge;
+        ga.setup(lane, mu, L.row.fwd, a.flip_align);
+        ge.setup(lane, mu, 1, 0ULL);
+        Pair42 P;
+        P.setup(lane, role, mu, 1, a.flip_stm, &plds);
+        for (;;) {
+            /* the residual, or the line-search trials of the step */
+            const int n_eval = L.n_eval;
+            const bool trials = L.row.stage == hh::NS_ARMIJO;
+            for (int j0 = 0; j0 < n_eval; j0 += CORR_GROUPS) {
+                const int j = j0 + grp;
+                if (j < n_eval) {
+                    const double* q = trials ? L.row.trial[j] : L.row.x;
+                    double y = 0.0;
+                    if (ge.act) {
+                        y = L.row.base[ge.d];
+                        for (int c = 0; c < 2; ++c)
+                            if (L.row.ctrl[c] == ge.d) y = q[c];
+                    }
+                    int ok, steps;
+                    double t, xe;
+                    corr_residual(ga, ge, a, y, ok, t, xe, steps);
+                    if (ge.act) L.xe[j][ge.d] = xe;
+                    if (ge.d == 0) {
+                        L.t[j] = t;
+                        L.ok[j] = ok;
+                        atomicAdd(&L.steps, steps);
+                    }
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    const int done_n = (j0 + CORR_GROUPS < n_eval) ? j0 + CORR_GROUPS : n_eval;
+                    L.more = (trials && !hh::newton_armijo_decided(L.row, 2, done_n, &L.xe[0][0],
+                                                                   L.ok)) ? 1 : 0;
+                }
+                __syncthreads();
+                if (!L.more) break;
+            }
+            if (tid == 0) {
+                if (L.row.stage == hh::NS_RESID)
+                    hh::newton_residual(L.row, 2, L.t[0], L.xe[0], L.ok[0]);
+                else
+                    hh::newton_armijo(L.row, 2, 2, L.t, &L.xe[0][0], L.ok);
+            }
+            __syncthreads();
+            if (L.row.stage == hh::NS_DONE) break;
+            /* Phi(t_event) from [I | x] */
+            {
+                double y0 = 0.0;
+                if (P.holds()) {
+                    const int c = P.comp();
+                    if (c < 36) {
+                        y0 = (c / 6 == c % 6) ? 1.0 : 0.0;
+                    } else {
+                        y0 = L.row.base[c - 36];
+                        for (int q = 0; q < 2; ++q)
+                            if (L.row.ctrl[q] == c - 36) y0 = L.row.x[q];
+                    }
+                }
+                double out;
+                int st, na, nr;
+                pair_final(P, a.sp_stm, y0, 0.0, L.row.t_ev, out, st, na, nr);
+                if (P.holds()) L.phi[P.comp()] = out;
+                if (P.leader()) L.steps += na;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double J[4], r[2], d[2];
+                hh::newton_jacobian(L.row, 2, 2, L.phi, J, r, DevPow());
+                corr_solve2(J, r, d);
+                hh::newton_set_step(L.row, 2, d);
+                L.n_eval = L.row.nal;
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) {
+        hh_correction_result_t& o = a.res[i];
+        if (L.fail) {
+            for (int q = 0; q < 6; ++q) o.x[q] = a.prob[i].x0[q];
+            o.residual_norm = NAN;
+            o.half_period = NAN;
+            o.iterations = 0;
+            o.n_event_searches = 0;
+            o.n_stm = 0;
+            o.converged = 0;
+            o.error = HH_CORRECT_INVALID;
+            o.n_steps = 0;
+        } else {
+            hh::newton_full_state(L.row, L.row.x, 2, o.x);
+            o.residual_norm = L.row.rn;
+            o.half_period = L.row.half_period;
+            o.iterations = L.row.k;
+            o.n_event_searches = L.row.n_ev;
+            o.n_stm = L.row.n_stm;
+            o.converged = L.row.ok;
+            o.error = L.row.err;
+            o.n_steps = L.steps;
+        }
+        /* the result before its flag, for a host that polls mapped memory */
+        __threadfence_system();
+        __hip_atomic_store(&o.done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+#endif

@@ -21,8 +21,10 @@
 #include "hh_pair.h"
 #include "hh_group6.h"
 #include "hh_synodic.h"
+#include "hh_newton.h"
 #include "../../include/hiten_hip.h"
 
+#include <float.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -390,26 +392,20 @@ __global__ __launch_bounds__(256) void k_dop853_run6(
 /* k_dop853_run6 with one trajectory per aligned 8-lane group, one component
  * per lane (Group6).  traj_init / traj_iterate and the FinalOut / GridOut
  * policies, statement for statement, with each lane holding its component. */
-template <int BLOCK, bool GRID>
-__global__ __launch_bounds__(BLOCK) void k_dop853_run6_group(
-    int n, const double* __restrict__ mu, const double* __restrict__ t0a,
-    const double* __restrict__ tfa, StepParams sp, int fwd, unsigned long long flip,
-    const double* __restrict__ y0, double* __restrict__ yout, int m,
-    const double* __restrict__ teval, int* __restrict__ status, int* __restrict__ nacc,
-    int* __restrict__ nrej) {
-    const long long gtid = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    const int i = (int)(gtid >> 3);
-    if (i >= n) return;
-    Group6 p;
-    p.setup(threadIdx.x & 63, mu[i], fwd, flip);
+/* One trajectory of k_dop853_run6 in the Group6 layout, statement for
+ * statement: the grid form writes this lane's component of every output
+ * point to gout ([m][6]); the final form returns it in yfin.  Status and
+ * counters are the same in all lanes of the group.                         */
+template <bool GRID>
+__device__ __forceinline__ void group6_run(const Group6& p, double y, double t0, double tf,
+                                           const StepParams& sp, double* gout, int m,
+                                           const double* teval, double& yfin, int& status,
+                                           int& nacc, int& nrej) {
     const int d = p.d;
-    double y = p.act ? y0[(size_t)d * n + i] : 0.0;
     double k[DOP_K];
 #pragma unroll
     for (int j = 0; j < DOP_K; ++j) k[j] = 0.0;
-    const double t0 = t0a[i], tf = tfa[i];
     /* GridOut: output times numpy.linspace(t0, tf, m) or teval */
-    double* gout = GRID ? yout + (size_t)i * m * 6 : nullptr;
     const double gdelta = tf - t0;
     const double gstep = GRID ? gdelta / (double)(m - 1) : 0.0;
     const bool gzstep = (gstep == 0.0);
@@ -423,7 +419,7 @@ __global__ __launch_bounds__(BLOCK) void k_dop853_run6_group(
         if (p.act) gout[(size_t)q * 6 + d] = v;
     };
     int q = 0;
-    double yfin = y;   /* FinalOut */
+    yfin = y;   /* FinalOut */
     double t = t0, h, err_prev = -1.0;
     int na = 0, nr = 0, st = ST_OK;
     bool done = isclose(t0, tf) || !((t0 - tf) * 1.0 < 0.0);
@@ -495,6 +491,29 @@ __global__ __launch_bounds__(BLOCK) void k_dop853_run6_group(
             nr += 1;
         }
     }
+    status = st;
+    nacc = na;
+    nrej = nr;
+}
+
+template <int BLOCK, bool GRID>
+__global__ __launch_bounds__(BLOCK) void k_dop853_run6_group(
+    int n, const double* __restrict__ mu, const double* __restrict__ t0a,
+    const double* __restrict__ tfa, StepParams sp, int fwd, unsigned long long flip,
+    const double* __restrict__ y0, double* __restrict__ yout, int m,
+    const double* __restrict__ teval, int* __restrict__ status, int* __restrict__ nacc,
+    int* __restrict__ nrej) {
+    const long long gtid = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    const int i = (int)(gtid >> 3);
+    if (i >= n) return;
+    Group6 p;
+    p.setup(threadIdx.x & 63, mu[i], fwd, flip);
+    const int d = p.d;
+    const double y = p.act ? y0[(size_t)d * n + i] : 0.0;
+    double yfin;
+    int st, na, nr;
+    group6_run<GRID>(p, y, t0a[i], tfa[i], sp, GRID ? yout + (size_t)i * m * 6 : nullptr, m,
+                     teval, yfin, st, na, nr);
     if constexpr (!GRID) {
         if (p.act) yout[(size_t)d * n + i] = yfin;
     }
@@ -607,29 +626,21 @@ __global__ __launch_bounds__(256) void k_dop853_event6(
     if (nrej) nrej[i] = nr;
 }
 
-/* k_dop853_event6 with one trajectory per aligned 8-lane group, one
- * component per lane (Group6): the latency form for small batches.  The
- * loop is the one above, statement for statement; the event component is
- * read from its lane of the group.                                          */
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_dop853_event6_group(
-    int n, const double* __restrict__ mu, const double* __restrict__ t0a,
-    const double* __restrict__ tmaxa, StepParams sp, EventParams ev, int fwd,
-    unsigned long long flip, const double* __restrict__ y0, int* __restrict__ hit,
-    double* __restrict__ t_hit, double* __restrict__ y_hit, double* __restrict__ y_last,
-    int* __restrict__ status, int* __restrict__ nacc, int* __restrict__ nrej) {
-    const long long gtid = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    const int i = (int)(gtid >> 3);
-    if (i >= n) return;
-    Group6 p;
-    p.setup(threadIdx.x & 63, mu[i], fwd, flip);
-    const int d = p.d;
-    double y = p.act ? y0[(size_t)d * n + i] : 0.0;
+/* One event search of k_dop853_event6 in the Group6 layout, statement for
+ * statement; the event component is read from its lane of the group.  Per
+ * lane: this lane's components of y_hit and y_last; the flags, t_hit and
+ * counters are the same in all lanes of the group.                         */
+struct EventOut6 {
+    int found, status, nacc, nrej;
+    double t_hit, y_hit, y_last;
+};
+
+__device__ __forceinline__ void group6_event(const Group6& p, double y, double t, double tmax,
+                                             const StepParams& sp, const EventParams& ev,
+                                             EventOut6& o) {
     double k[DOP_K];
 #pragma unroll
     for (int j = 0; j < DOP_K; ++j) k[j] = 0.0;
-    double t = t0a[i];
-    const double tmax = tmaxa[i];
     k[0] = p.rhs(y);
     const int ev_addr = p.base + (ev.index << 2);
     double gp = bperm(y, ev_addr) - ev.offset;
@@ -685,16 +696,43 @@ __global__ __launch_bounds__(BLOCK) void k_dop853_event6_group(
         yh_out = y;
         ylast = y;
     }
+    o.found = found;
+    o.status = st;
+    o.nacc = na;
+    o.nrej = nr;
+    o.t_hit = th;
+    o.y_hit = yh_out;
+    o.y_last = ylast;
+}
+
+/* k_dop853_event6 with one trajectory per aligned 8-lane group, one
+ * component per lane (Group6): the latency form for small batches.          */
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_dop853_event6_group(
+    int n, const double* __restrict__ mu, const double* __restrict__ t0a,
+    const double* __restrict__ tmaxa, StepParams sp, EventParams ev, int fwd,
+    unsigned long long flip, const double* __restrict__ y0, int* __restrict__ hit,
+    double* __restrict__ t_hit, double* __restrict__ y_hit, double* __restrict__ y_last,
+    int* __restrict__ status, int* __restrict__ nacc, int* __restrict__ nrej) {
+    const long long gtid = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    const int i = (int)(gtid >> 3);
+    if (i >= n) return;
+    Group6 p;
+    p.setup(threadIdx.x & 63, mu[i], fwd, flip);
+    const int d = p.d;
+    const double y = p.act ? y0[(size_t)d * n + i] : 0.0;
+    EventOut6 o;
+    group6_event(p, y, t0a[i], tmaxa[i], sp, ev, o);
     if (p.act) {
-        y_hit[(size_t)d * n + i] = yh_out;
-        if (y_last) y_last[(size_t)d * n + i] = ylast;
+        y_hit[(size_t)d * n + i] = o.y_hit;
+        if (y_last) y_last[(size_t)d * n + i] = o.y_last;
     }
     if (d == 0) {
-        hit[i] = found;
-        t_hit[i] = th;
-        if (status) status[i] = st;
-        if (nacc) nacc[i] = na;
-        if (nrej) nrej[i] = nr;
+        hit[i] = o.found;
+        t_hit[i] = o.t_hit;
+        if (status) status[i] = o.status;
+        if (nacc) nacc[i] = o.nacc;
+        if (nrej) nrej[i] = o.nrej;
     }
 }
 
@@ -802,14 +840,16 @@ struct NoHook {
     __device__ __forceinline__ void end() {}
 };
 
+/* traj_iterate + FinalOut for one trajectory in a wave-wide layout (Wave42
+ * or Pair42), from the current (y, k[0], t, h, err_prev, counters) to its
+ * end: this lane's component of the final state in `out`.                  */
 template <class WL, class Hook = NoHook>
-__device__ __forceinline__ void wide_finish(const Final42Args& a, const WL& W, int traj,
-                                            double y, double (&k)[DOP_K], double t, double tf,
-                                            double h, double err_prev, int nacc, int nrej,
-                                            Hook hook = Hook()) {
-    const StepParams& sp = a.sp;
-    double out = y;
-    int status = ST_OK;
+__device__ __forceinline__ void wide_loop(const StepParams& sp, const WL& W, double y,
+                                          double (&k)[DOP_K], double t, double tf, double h,
+                                          double err_prev, int& nacc, int& nrej, double& out,
+                                          int& status, Hook hook = Hook()) {
+    out = y;
+    status = ST_OK;
     for (;;) {
         hook.attempt(nacc + nrej, t);
         if (nacc + nrej >= sp.max_steps) {
@@ -848,6 +888,17 @@ __device__ __forceinline__ void wide_finish(const Final42Args& a, const WL& W, i
         }
     }
     hook.end();
+}
+
+/* wide_loop, then the stores of store42 */
+template <class WL, class Hook = NoHook>
+__device__ __forceinline__ void wide_finish(const Final42Args& a, const WL& W, int traj,
+                                            double y, double (&k)[DOP_K], double t, double tf,
+                                            double h, double err_prev, int nacc, int nrej,
+                                            Hook hook = Hook()) {
+    double out;
+    int status;
+    wide_loop(a.sp, W, y, k, t, tf, h, err_prev, nacc, nrej, out, status, hook);
     if (W.holds()) a.yout[(size_t)W.comp() * a.n + traj] = out;
     if (W.leader()) {
         if (a.status) a.status[traj] = status;
@@ -928,7 +979,36 @@ __global__ __launch_bounds__(BLOCK) void k_dop853_final42_wide(Final42Args a) {
 }
 
 /* One trajectory per pair of waves (128-thread workgroup) from the start:
- * the state wave and the STM wave of Pair42 side by side on two SIMDs.     */
+ * the state wave and the STM wave of Pair42 side by side on two SIMDs.
+ * traj_init (rk.py:2441-2448) with the block-tree sums of Lane42, then
+ * wide_loop; a zero or reversed span leaves y0 unchanged
+ * (dynamics/base.py:421-424).  Both waves of the workgroup must call it.   */
+__device__ __forceinline__ void pair_final(const Pair42& P, const StepParams& sp, double y0,
+                                           double t0, double tf, double& out, int& status,
+                                           int& nacc, int& nrej) {
+    nacc = 0;
+    nrej = 0;
+    if (isclose(t0, tf) || !((t0 - tf) * 1.0 < 0.0)) {
+        out = y0;
+        status = ST_OK;
+        return;
+    }
+    double k[DOP_K];
+#pragma unroll
+    for (int j = 0; j < DOP_K; ++j) k[j] = 0.0;
+    k[0] = P.rhs(y0);
+    const double sc = sp.atol + sp.rtol * __builtin_fabs(y0);
+    const double ua = y0 / sc;
+    const double ub = k[0] / sc;
+    double s0, s1;
+    P.pair_tree(P.holds() ? ua * ua : 0.0, P.holds() ? ub * ub : 0.0, s0, s1);
+    const double sn = __builtin_sqrt(42.0);
+    const double d0 = __builtin_sqrt(s0) / sn;
+    const double d1 = __builtin_sqrt(s1) / sn;
+    const double h = select_initial_step(d0, d1, sp.min_step, sp.max_step);
+    wide_loop(sp, P, y0, k, t0, tf, h, -1.0, nacc, nrej, out, status);
+}
+
 __global__ __launch_bounds__(128) void k_dop853_final42_pair(Final42Args a) {
     __shared__ PairLds lds;
     const int traj = blockIdx.x;
@@ -940,33 +1020,15 @@ __global__ __launch_bounds__(128) void k_dop853_final42_pair(Final42Args a) {
     Pair42 P;
     P.setup(lane, role, a.mu[traj], a.fwd, a.flip, &lds);
     const double y0 = P.holds() ? a.y0[(size_t)P.comp() * a.n + traj] : 0.0;
-    const double t0 = a.t0[traj], tf = a.tf[traj];
-    if (isclose(t0, tf) || !((t0 - tf) * 1.0 < 0.0)) {
-        /* zero or reversed span: y0 unchanged (dynamics/base.py:421-424) */
-        if (P.holds()) a.yout[(size_t)P.comp() * a.n + traj] = y0;
-        if (P.leader()) {
-            if (a.status) a.status[traj] = ST_OK;
-            if (a.nacc) a.nacc[traj] = 0;
-            if (a.nrej) a.nrej[traj] = 0;
-        }
-        return;
+    double out;
+    int status, nacc, nrej;
+    pair_final(P, a.sp, y0, a.t0[traj], a.tf[traj], out, status, nacc, nrej);
+    if (P.holds()) a.yout[(size_t)P.comp() * a.n + traj] = out;
+    if (P.leader()) {
+        if (a.status) a.status[traj] = status;
+        if (a.nacc) a.nacc[traj] = nacc;
+        if (a.nrej) a.nrej[traj] = nrej;
     }
-    double k[DOP_K];
-#pragma unroll
-    for (int j = 0; j < DOP_K; ++j) k[j] = 0.0;
-    k[0] = P.rhs(y0);
-    /* initial_step with the block-tree sums of Lane42 */
-    const StepParams& sp = a.sp;
-    const double sc = sp.atol + sp.rtol * __builtin_fabs(y0);
-    const double ua = y0 / sc;
-    const double ub = k[0] / sc;
-    double s0, s1;
-    P.pair_tree(P.holds() ? ua * ua : 0.0, P.holds() ? ub * ub : 0.0, s0, s1);
-    const double sn = __builtin_sqrt(42.0);
-    const double d0 = __builtin_sqrt(s0) / sn;
-    const double d1 = __builtin_sqrt(s1) / sn;
-    const double h = select_initial_step(d0, d1, sp.min_step, sp.max_step);
-    wide_finish(a, P, traj, y0, k, t0, tf, h, -1.0, 0, 0);
 }
 
 /* Work-stealing form.  Each wave keeps its 8 groups busy: after every step
@@ -1560,6 +1622,9 @@ StepParams make_sp(double rtol, double atol, double max_step, double min_step,
     return sp;
 }
 
+/* the device-resident corrector, on the loops above */
+#include "hh_correct.h"
+
 }  // namespace
 
 /* ======================================================================== */
@@ -1886,6 +1951,64 @@ int hh_dop853_event(int n, const double* mu, const double* t0, const double* tma
     return check_launch("hh_dop853_event launch");
 }
 
+int hh_correct(int n, const hh_correction_t* problems, hh_correction_result_t* results,
+               double window_start, double window_length, double fallback_start,
+               double fallback_length, void* stream) {
+    if (n < 0) return fail(HH_ERR_ARGS, "hh_correct: n < 0");
+    if (n == 0) return HH_OK;
+    if (!problems || !results) return fail(HH_ERR_ARGS, "hh_correct: null pointer");
+    if (!(window_length >= 0.0) || !(fallback_length >= 0.0))
+        return fail(HH_ERR_ARGS, "hh_correct: window lengths must be >= 0");
+    CorrArgs a;
+    a.n = n;
+    a.prob = problems;
+    a.res = results;
+    a.ts[0] = window_start;
+    a.ts[1] = fallback_start;
+    a.span[0] = window_length;
+    a.span[1] = fallback_length;
+    for (int w = 0; w < 2; ++w) {
+        /* numpy.isclose(0, v): alignment and search are skipped for ~0 */
+        a.aligned[w] = !(fabs(0.0 - a.ts[w]) <= 1e-8 + 1e-5 * fabs(a.ts[w]));
+        a.searched[w] = !(fabs(0.0 - a.span[w]) <= 1e-8 + 1e-5 * fabs(a.span[w]));
+    }
+    /* the reference's settings: DOP853 rtol = atol = 1e-12 everywhere;
+     * max_step 1e4 for propagations (dynamics/base.py:447-450), unbounded in
+     * the event search (singlehit/backend.py:202); min_step 10 eps
+     * (integrators/rk.py:833-834); x/g tolerances 1e-12 (utils.py:13-123)  */
+    const double min_step = 10.0 * DBL_EPSILON;
+    const long long max_steps = 1000000;
+    a.sp_align = make_sp(1e-12, 1e-12, 1e4, min_step, max_steps);
+    a.sp_event = make_sp(1e-12, 1e-12, INFINITY, min_step, max_steps);
+    a.sp_stm = make_sp(1e-12, 1e-12, 1e4, min_step, max_steps);
+    a.ev.index = 1;
+    a.ev.offset = 0.0;
+    a.ev.direction = 0;
+    a.ev.xtol = 1e-12;
+    a.ev.gtol = 1e-12;
+    a.flip_align = ~0ULL;                     /* backward alignment flips all six */
+    a.flip_stm = ((1ULL << 42) - 1) & ~((1ULL << 36) - 1);   /* the state block */
+    hipLaunchKernelGGL(k_correct, dim3(n), dim3(128), 0, (hipStream_t)stream, a);
+    return check_launch("hh_correct launch");
+}
+
+int hh_correct_sizes(int* problem_bytes, int* result_bytes) {
+    if (!problem_bytes || !result_bytes) return fail(HH_ERR_ARGS, "hh_correct_sizes: null pointer");
+    *problem_bytes = (int)sizeof(hh_correction_t);
+    *result_bytes = (int)sizeof(hh_correction_result_t);
+    return HH_OK;
+}
+
+int hh_eval_solve2(int n, const double* J, const double* r, double* delta, double* cond,
+                   void* stream) {
+    if (n < 0 || (n > 0 && (!J || !r || !delta || !cond)))
+        return fail(HH_ERR_ARGS, "hh_eval_solve2: invalid arguments");
+    if (n == 0) return HH_OK;
+    hipLaunchKernelGGL(k_eval_solve2, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                       n, J, r, delta, cond);
+    return check_launch("hh_eval_solve2 launch");
+}
+
 int hh_eval_rhs(int n, int dim, const double* mu, const double* y, double* out, void* stream) {
     if (n < 0 || (dim != 6 && dim != 42)) return fail(HH_ERR_ARGS, "hh_eval_rhs: bad n/dim");
     if (n == 0) return HH_OK;
@@ -1975,6 +2098,47 @@ int hh_memcpy_async(void* dst, const void* src, unsigned long long bytes, int to
                                         to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost,
                                         (hipStream_t)stream);
     return e == hipSuccess ? HH_OK : hip_fail(e, "hipMemcpyAsync");
+}
+
+/* A stream with a hardware queue of its own: a stream created with a CU
+ * mask (here every CU) is not multiplexed onto the runtime's shared queues,
+ * so a long-running launch on it holds up no other stream's work.          */
+void* hh_stream_create_dedicated(void) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        return nullptr;
+    const int words = (cus + 31) / 32;
+    uint32_t mask[64];
+    if (words > 64) return nullptr;
+    for (int w = 0; w < words; ++w) mask[w] = 0xFFFFFFFFu;
+    hipStream_t s = nullptr;
+    if (hipExtStreamCreateWithCUMask(&s, (uint32_t)words, mask) != hipSuccess) return nullptr;
+    return (void*)s;
+}
+
+int hh_stream_destroy(void* stream) {
+    if (!stream) return fail(HH_ERR_ARGS, "hh_stream_destroy: null stream");
+    const hipError_t e = hipStreamDestroy((hipStream_t)stream);
+    return e == hipSuccess ? HH_OK : hip_fail(e, "hipStreamDestroy");
+}
+
+/* Pinned, device-mapped, coherent host memory: device writes reach it
+ * without a cache flush, so a host loop can poll what a running kernel
+ * writes there.                                                            */
+int hh_host_alloc(unsigned long long bytes, void** ptr) {
+    if (!ptr || bytes == 0) return fail(HH_ERR_ARGS, "hh_host_alloc: invalid arguments");
+    *ptr = nullptr;
+    const hipError_t e = hipHostMalloc(ptr, (size_t)bytes,
+                                       hipHostMallocMapped | hipHostMallocCoherent);
+    return e == hipSuccess ? HH_OK : hip_fail(e, "hipHostMalloc");
+}
+
+int hh_host_free(void* ptr) {
+    if (!ptr) return HH_OK;
+    const hipError_t e = hipHostFree(ptr);
+    return e == hipSuccess ? HH_OK : hip_fail(e, "hipHostFree");
 }
 
 void* hh_event_create(void) {

@@ -1,29 +1,40 @@
 """Continuous-batching request loop for corrections and manifold fans.
 
 A request is either
-  * a single-shooting correction (corrector.CorrectionProblem), run by the
-    corrector's resumable Newton state machine, or
+  * a single-shooting correction (corrector.CorrectionProblem), or
   * a manifold fan: N event searches (DOP853 until y[index] = offset in a
     direction, dense-output bisection), as _integrate_dop853_until_event
     (integrators/rk.py:2678-2803) runs them per trajectory.
 
 The loop is continuous batching, not a queue: requests are admitted as they
 arrive while an HBM budget allows (each admitted job reserves the device
-bytes of its buffers until it completes); every admitted fan is launched at
-once on its own stream; all corrections that are waiting for the same kind of
-work at a given moment (event searches -- residuals and every trial of their
-line searches -- or STM propagations) go out together in one launch, on a
-stream of their own, as soon as the previous batch that held them has
-completed.  Kernels of different batches overlap on the device; the host only
-polls completion events, moves results to the state machines and launches the
-next batches.  Every result is a pure function of its own request: the
-kernels integrate each trajectory independently of the batch it is in, so a
-job gives the same bits alone and in the mix.
+bytes of its buffers until it completes), and every admitted fan is launched
+at once on a stream of its own.
+
+Corrections run device-resident by default (``corrector="device"``): all
+corrections admitted since the last launch go out in one hh_correct launch,
+one workgroup per correction running its whole Newton-Armijo loop, on a
+dedicated stream -- a hardware queue of its own, so a launch that runs for
+milliseconds holds up no fan and no other launch.  The kernel reads the
+problems from and writes each result to mapped host memory, setting the
+result's done flag last; the loop completes a job the moment its flag is
+set.  With ``corrector="bank"`` the corrections advance instead through the
+host-side state machine: all corrections waiting for the same kind of work
+(event searches -- residuals and every trial of their line searches -- or
+STM propagations) go out together in one launch as soon as the previous
+batch that held them has completed.
+
+Kernels of different launches overlap on the device; the host only polls
+completion events and flags, moves results and launches.  Every result is
+a pure function of its own request: the kernels integrate each trajectory
+(and run each correction) independently of the launch it is in, so a job
+gives the same bits alone and in the mix.
 """
 
 from __future__ import annotations
 
 import collections
+import ctypes
 import gc
 import math
 import time
@@ -34,6 +45,7 @@ import numpy as np
 import torch
 
 from . import _lib, batch
+from . import corrector as _corr
 from .corrector import CorrectionProblem, CorrectionResult, _Bank, halo_problem
 from .singlehit import _default_windows
 
@@ -369,22 +381,180 @@ class _StmBatch(_Pending):
         return True
 
 
+class _CorrSlot:
+    """A dedicated stream (a hardware queue of its own, so a launch that runs
+    for milliseconds holds up no other stream) with mapped, coherent host
+    records for device-resident correction launches: the kernel reads the
+    problems from and writes the results to host memory directly.  Buffers
+    grow to the largest batch; a slot takes a new batch only after its
+    previous launch completed."""
+
+    def __init__(self):
+        self.lib = _lib.load()
+        self.sh = self.lib.hh_stream_create_dedicated()
+        if not self.sh:
+            raise _lib.HipLibraryError("hh_stream_create_dedicated failed")
+        self.ev = self.lib.hh_event_create()
+        if not self.ev:
+            raise _lib.HipLibraryError("hh_event_create failed")
+        self.busy = False
+        self.cap = 0
+        self._ptrs = []
+
+    def _alloc(self, nbytes: int) -> int:
+        p = ctypes.c_void_p()
+        _lib.check(self.lib.hh_host_alloc(nbytes, ctypes.byref(p)), "hh_host_alloc")
+        self._ptrs.append(p.value)
+        return p.value
+
+    def reserve(self, n: int) -> None:
+        if n <= self.cap:
+            return
+        self._free()
+        cap = max(n, 2 * self.cap, 16)
+        pb, rb = _corr.PROBLEM_DTYPE.itemsize * cap, _corr.RESULT_DTYPE.itemsize * cap
+        self.pptr, self.rptr = self._alloc(pb), self._alloc(rb)
+        self.prob = np.frombuffer((ctypes.c_char * pb).from_address(self.pptr),
+                                  dtype=_corr.PROBLEM_DTYPE)
+        self.res = np.frombuffer((ctypes.c_char * rb).from_address(self.rptr),
+                                 dtype=_corr.RESULT_DTYPE)
+        self.cap = cap
+
+    def _free(self) -> None:
+        self.prob = self.res = None
+        for p in self._ptrs:
+            self.lib.hh_host_free(p)
+        self._ptrs = []
+
+    def record(self) -> None:
+        _lib.check(self.lib.hh_event_record(self.ev, self.sh), "hh_event_record")
+
+    def ready(self) -> bool:
+        r = self.lib.hh_event_query(self.ev)
+        if r < 0:
+            _lib.check(-r, "hh_event_query")
+        return r == 1
+
+    def close(self) -> None:
+        """Release the stream, event and buffers (after the last launch)."""
+        if getattr(self, "sh", None):
+            self.lib.hh_stream_destroy(self.sh)
+            self.sh = None
+        if getattr(self, "ev", None):
+            self.lib.hh_event_destroy(self.ev)
+            self.ev = None
+        self._free()
+
+
+class _CorrLaunch:
+    """Corrections run whole on the device (hh_correct: one workgroup per
+    correction runs its Newton-Armijo loop to the end) on a dedicated
+    stream; each job completes as soon as its own result's done flag is
+    set, without waiting for the rest of its launch."""
+
+    def __init__(self, jobs: list, slot: _CorrSlot):
+        n = len(jobs)
+        slot.reserve(n)
+        slot.busy = True
+        self.slot, self.jobs = slot, jobs
+        slot.prob[:n] = _corr.pack_problems([j.request for j in jobs])
+        slot.res["done"][:n] = 0
+        _corr.launch_device(slot.pptr, slot.rptr, n, stream_handle=slot.sh)
+        slot.record()
+        self.pending = np.arange(n)
+
+    def poll(self, server: "Server") -> bool:
+        """Complete the jobs whose results are in; True once the launch is
+        over (every result in and the kernel finished)."""
+        if self.pending.size:
+            res = self.slot.res
+            flags = res["done"][self.pending]
+            if flags.any():
+                got = self.pending[flags != 0]
+                recs = res[got].copy()
+                for i, r in zip(got, _corr.results_from_records(
+                        recs, [self.jobs[i].request for i in got])):
+                    job = self.jobs[i]
+                    job.result = r
+                    server._complete(job)
+                server.steps += int(recs["n_steps"].sum())
+                self.pending = self.pending[flags == 0]
+        if self.pending.size == 0 and self.slot.ready():
+            self.slot.busy = False
+            return True
+        return False
+
+
 class Server:
     """Continuous-batching loop; see the module docstring.
 
     ``hbm_budget`` bounds the device bytes reserved by admitted jobs.  Each
     batch in flight holds a staging slot (stream, event, reusable pinned and
     device buffers); slots are spread over ``n_streams`` default-priority
-    streams.
+    streams (torch's pool, whose streams share the runtime's few hardware
+    queues, or with ``dedicated_streams`` streams with queues of their own).
+    Device-resident correction launches go to up to ``n_correction_streams``
+    dedicated streams; at 2k requests/s four keep the correction latency low
+    without delaying fans (more hardware queues than that make the
+    scheduler time-slice them and fans wait).
     """
 
     CORRECTION_BYTES = 64 * 1024          # event batch rows (<= 15 per iteration) and STM
 
-    def __init__(self, hbm_budget: int = 32 << 30, n_streams: int = 8):
+    def __init__(self, hbm_budget: int = 32 << 30, n_streams: int = 8,
+                 corrector: str = "device", n_correction_streams: int = 4,
+                 dedicated_streams: bool = False):
+        if corrector not in ("device", "bank"):
+            raise ValueError("corrector must be 'device' or 'bank'")
         self.hbm_budget = int(hbm_budget)
-        self.streams = [torch.cuda.Stream() for _ in range(max(2, n_streams))]
+        self.corrector = corrector
+        self._own_streams = []
+        if dedicated_streams:
+            lib = _lib.load()
+            for _ in range(max(2, n_streams)):
+                h = lib.hh_stream_create_dedicated()
+                if not h:
+                    raise _lib.HipLibraryError("hh_stream_create_dedicated failed")
+                self._own_streams.append(h)
+            self.streams = [torch.cuda.ExternalStream(h) for h in self._own_streams]
+        else:
+            self.streams = [torch.cuda.Stream() for _ in range(max(2, n_streams))]
         self.slots: List[_Slot] = []
+        self.n_corr_slots = max(1, int(n_correction_streams))
+        self.corr_slots: List[_CorrSlot] = []
         self.reset()
+
+    def close(self) -> None:
+        """Release the dedicated correction streams and their host records
+        (waits for launches still running)."""
+        torch.cuda.synchronize()
+        for sl in self.corr_slots:
+            sl.close()
+        self.corr_slots = []
+        if self._own_streams:
+            # the staging slots run on these streams: the server is done
+            lib = _lib.load()
+            for h in self._own_streams:
+                lib.hh_stream_destroy(h)
+            self._own_streams = []
+            self.slots, self.streams = [], []
+
+    def __del__(self):
+        try:
+            if self.corr_slots or self._own_streams:
+                self.close()
+        except Exception:
+            pass
+
+    def _corr_slot(self) -> Optional[_CorrSlot]:
+        for sl in self.corr_slots:
+            if not sl.busy:
+                return sl
+        if len(self.corr_slots) < self.n_corr_slots:
+            sl = _CorrSlot()
+            self.corr_slots.append(sl)
+            return sl
+        return None
 
     def _slot(self) -> _Slot:
         """A slot whose previous batch has completed; a new one (on the next
@@ -406,6 +576,8 @@ class Server:
         self.ready = {}                 # bank key -> list of rows ready to launch
         self.owner = {}                 # (bank key, row) -> Job
         self.inflight: List[_Pending] = []
+        self.corr_waiting: list = []    # admitted corrections waiting for a device launch
+        self.corr_inflight: List[_CorrLaunch] = []
         self.t0 = time.perf_counter()
         self.max_queue = 0
 
@@ -452,6 +624,8 @@ class Server:
             job.admitted = self.now()
             if job.kind == "fan":
                 self.inflight.append(_FanLaunch(job, self._slot()))
+            elif self.corrector == "device" and _corr.device_supported(job.request):
+                self.corr_waiting.append(job)
             else:
                 p = job.request
                 key = (len(p.residual_indices), len(p.control_indices))
@@ -464,6 +638,11 @@ class Server:
         self.max_queue = max(self.max_queue, len(self.waiting))
 
     def _launch_ready(self) -> None:
+        if self.corr_waiting:
+            sl = self._corr_slot()
+            if sl is not None:
+                self.corr_inflight.append(_CorrLaunch(self.corr_waiting, sl))
+                self.corr_waiting = []
         for key, rows in self.ready.items():
             if not rows:
                 continue
@@ -483,6 +662,10 @@ class Server:
         """Collect finished batches.  Work on one stream completes in launch
         order, so only the oldest batch of each stream is queried."""
         progressed = False
+        n_jobs_open = sum(len(c.pending) for c in self.corr_inflight)
+        self.corr_inflight = [c for c in self.corr_inflight if not c.poll(self)]
+        if sum(len(c.pending) for c in self.corr_inflight) != n_jobs_open:
+            progressed = True
         seen = set()
         keep, relaunched = [], []
         for b in self.inflight:
@@ -504,16 +687,20 @@ class Server:
         return progressed
 
     def busy(self) -> bool:
-        return bool(self.waiting or any(self.ready.values()) or self.inflight)
+        return bool(self.waiting or any(self.ready.values()) or self.inflight
+                    or self.corr_waiting or self.corr_inflight)
 
     def step(self) -> bool:
         """One pass of the loop: admit, launch what is ready, collect what has
         finished.  Returns True if anything moved."""
-        n_before = len(self.inflight) + len(self.waiting)
+        def count():
+            return (len(self.inflight) + len(self.waiting) + len(self.corr_inflight)
+                    + len(self.corr_waiting))
+        n_before = count()
         self._admit()
         self._launch_ready()
         moved = self._poll()
-        return moved or n_before != len(self.inflight) + len(self.waiting)
+        return moved or n_before != count()
 
     def run(self, stream_of_requests, time_limit: float = 120.0) -> List[Job]:
         """Serve requests (dicts with arrival time ``t``, as

@@ -237,6 +237,14 @@ int hh_traj_filters(int n, int m, const double* mu, const double* states, double
  * error.  No counterpart in the reference.                                 */
 int hh_memcpy_async(void* dst, const void* src, unsigned long long bytes, int to_device,
                     void* stream);
+/* A stream with a hardware queue of its own (created with an all-CU mask):
+ * long-running launches on it hold up no other stream.  NULL on error.     */
+void* hh_stream_create_dedicated(void);
+int hh_stream_destroy(void* stream);
+/* Pinned, device-mapped, coherent host memory (the device pointer equals the
+ * host pointer), for results a host loop polls while a kernel runs.        */
+int hh_host_alloc(unsigned long long bytes, void** ptr);
+int hh_host_free(void* ptr);
 void* hh_event_create(void);
 void hh_event_destroy(void* event);
 int hh_event_record(void* event, void* stream);
@@ -287,6 +295,58 @@ int hh_bank_result(void* bank, int row, double* x_full, double* residual_norm, i
                    double* half_period);
 /* a finished row whose result has been read; hh_bank_add reuses it         */
 int hh_bank_release(void* bank, int row);
+
+/* ---- device-resident corrector -----------------------------------------
+ * One correction per 128-thread workgroup, its whole Newton-Armijo loop on
+ * the device: replaces the host-driven rounds of the bank above (the
+ * reference's _NewtonBackend.run, corrector/backends/newton.py:39-143, with
+ * _ArmijoLineSearch, stepping/armijo.py:58-164; residual and Jacobian
+ * operators corrector/operators.py:319-452; _solve_delta_dense
+ * corrector/backends/base.py:187-245; Halo term services/orbits.py:917-947)
+ * for square 2x2 problems (the Halo and Lyapunov configurations).  The
+ * event search of every residual is _y_plane_crossing's
+ * (poincare/singlehit/backend.py:237-280) with the two windows given as
+ * (start, length); the STM is _compute_stm's (corrector/operators.py:
+ * 298-307).  Results agree with the bank's to the corrector's tolerance (the
+ * Halo term's powers are correctly rounded on the device).
+ *
+ * problems[n] and results[n] are device (or host-mapped) memory.  Each
+ * result is written when its correction ends, its `done` set to 1 last with
+ * a system-scope release, so a host polling mapped memory can take it
+ * before the launch completes (clear `done` before the launch).  error:
+ * HH_BANK_* or HH_CORRECT_INVALID (not 2x2, an index outside 0..5, or more
+ * than 64 line-search steps).                                             */
+#define HH_CORRECT_INVALID 4
+typedef struct {
+    double mu;
+    double x0[6];
+    double target[6];        /* entries 0..n_residuals-1 */
+    double tol, max_delta, alpha_reduction, min_alpha, armijo_c;
+    int residual_indices[6]; /* entries 0..n_residuals-1 */
+    int control_indices[6];  /* entries 0..n_controls-1  */
+    int n_residuals, n_controls;
+    int halo, forward, max_attempts, reserved;
+} hh_correction_t;
+
+typedef struct {
+    double x[6];             /* the full corrected state */
+    double residual_norm;
+    double half_period;      /* the event time of the converged state, or NaN */
+    int iterations, n_event_searches, n_stm, converged, error;
+    int n_steps;             /* accepted integration steps, all searches and STMs */
+    int done, reserved;
+} hh_correction_result_t;
+
+int hh_correct(int n, const hh_correction_t* problems, hh_correction_result_t* results,
+               double window_start, double window_length, double fallback_start,
+               double fallback_length, void* stream);
+/* sizeof of the two records, for bindings that lay them out themselves     */
+int hh_correct_sizes(int* problem_bytes, int* result_bytes);
+/* Test entry: the device corrector's 2x2 step, delta[n][2] solving
+ * J[n][2][2] delta = -r[n][2] as _solve_delta_dense does
+ * (corrector/backends/base.py:187-245), and cond[n] = numpy.linalg.cond(J). */
+int hh_eval_solve2(int n, const double* J, const double* r, double* delta, double* cond,
+                   void* stream);
 
 #ifdef __cplusplus
 }

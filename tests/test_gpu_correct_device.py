@@ -1,0 +1,140 @@
+"""GPU: the device-resident corrector (hh_correct) -- its 2x2 step against
+NumPy on this host bit for bit, and whole corrections against the
+host-driven bank (correct_batch) and the reference's sequential loop on the
+oracle's operators, to the corrector's tolerance (the Halo term's powers are
+correctly rounded on the device, the C library's pow is within 0.52 ulp:
+iterates may move by an ulp)."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from hiten_amd import _lib, corrector as Cr, workloads as W
+
+pytestmark = pytest.mark.gpu
+MU = W.MU_EM
+TOL_X = 1e-10          # corrected states and half periods
+
+
+def say(msg):
+    print(f"\n[case] {msg}", flush=True)
+
+
+def _problems():
+    xl1 = W.l1_position(MU)
+    c2 = (1 - MU) / abs(xl1 + MU)**3 + MU / abs(xl1 - 1 + MU)**3
+    wp = np.sqrt((2 - c2 + np.sqrt(9 * c2**2 - 8 * c2)) / 2)
+    kap = (wp**2 + 1 + 2 * c2) / (2 * wp)
+    lyap = lambda ax: [xl1 - ax, 0, 0, 0, kap * ax * wp, 0]
+    ps = [
+        Cr.halo_problem(MU, [0.8234, 0, 0.0224, 0, 0.1343, 0], max_attempts=20),
+        Cr.halo_problem(MU, [0.84, 0, 0.1, 0, 0.22, 0], max_attempts=20),
+        Cr.halo_problem(MU, [0.82, 0, 0.05, 0, 0.15, 0], max_attempts=4),     # no convergence
+        Cr.lyapunov_problem(MU, lyap(0.005), max_attempts=20),
+        Cr.lyapunov_problem(MU, lyap(0.01), max_attempts=20),                # Armijo failure
+        Cr.halo_problem(MU, [0.8234, 0, 0.0224, 0, 0.1343, 0], max_attempts=20,
+                        alpha_reduction=0.7, armijo_c=0.3),                 # 26 trial steps
+        Cr.halo_problem(MU, [3.0, 0, 0.0, 0, 0.0, 0.0], max_attempts=5),       # far away
+    ]
+    rng = np.random.default_rng(20261202)
+    for _ in range(24):      # seeds drawn as the serving stream draws them
+        ps.append(Cr.halo_problem(MU, [rng.uniform(0.80, 0.88), 0, rng.uniform(-0.2, 0.2), 0,
+                                       rng.uniform(-0.3, 0.3), 0], max_attempts=10))
+    return ps
+
+
+def _same_outcome(g, w):
+    assert g.converged == w.converged
+    assert g.error.split(" ")[0] == w.error.split(" ")[0]
+    if g.converged:
+        assert np.max(np.abs(g.x_corrected - w.x_corrected)) <= TOL_X
+        assert abs(g.half_period - w.half_period) <= TOL_X
+        assert g.residual_norm < 1e-12 and w.residual_norm < 1e-12
+
+
+def test_step_solve_is_numpys_bit_for_bit(gpu):
+    say("hh_eval_solve2: 20000 random 2x2 systems and 400 near the cond threshold vs NumPy")
+    rng = np.random.default_rng(7)
+    n = 20000
+    J = rng.standard_normal((n, 2, 2)) * np.exp(rng.uniform(-8, 8, (n, 1, 1)))
+    r = rng.standard_normal((n, 2)) * np.exp(rng.uniform(-8, 8, (n, 1)))
+    extra = []
+    for _ in range(400):        # regularised and not, either side of cond = 1e8
+        s = 10.0 ** rng.uniform(7.5, 8.5)
+        U, _ = np.linalg.qr(rng.standard_normal((2, 2)))
+        V, _ = np.linalg.qr(rng.standard_normal((2, 2)))
+        extra.append(U @ np.diag([1.0, 1.0 / s]) @ V.T)
+    J = np.concatenate([J, np.stack(extra)])
+    r = np.concatenate([r, rng.standard_normal((400, 2))])
+    n = len(J)
+    dev = torch.device("cuda:0")
+    Jd = torch.from_numpy(J.reshape(n, 4).copy()).to(dev)
+    rd = torch.from_numpy(r.copy()).to(dev)
+    dd = torch.empty((n, 2), dtype=torch.float64, device=dev)
+    cd = torch.empty(n, dtype=torch.float64, device=dev)
+    lib = _lib.load()
+    _lib.check(lib.hh_eval_solve2(n, Jd.data_ptr(), rd.data_ptr(), dd.data_ptr(), cd.data_ptr(),
+                                  None), "hh_eval_solve2")
+    d, c = dd.cpu().numpy(), cd.cpu().numpy()
+    want = Cr.solve_delta_stack(J, r)
+    assert np.array_equal(d, want)
+    cw = np.linalg.cond(J)
+    # NumPy's cond (dgesdd) carries about eps * cond relative error in
+    # sigma_min, the device's accurate determinant does not: compare to that
+    assert np.allclose(c, cw, rtol=1e-15 * np.maximum(cw, 1.0) + 1e-13)
+    assert ((c > 1e8) == (cw > 1e8)).all()
+
+
+def test_device_corrector_matches_the_bank_and_the_reference(gpu, oracle):
+    ps = _problems()
+    say(f"hh_correct: {len(ps)} problems vs correct_batch and the sequential reference loop")
+    got = Cr.correct_device(ps)
+    bank = Cr.correct_batch(ps)
+    same_iters = 0
+    for i, (g, b) in enumerate(zip(got, bank)):
+        _same_outcome(g, b)
+        same_iters += (g.iterations == b.iterations)
+    assert same_iters >= len(ps) - 2
+    kinds = {(g.converged, g.error.split(" ")[0]) for g in got}
+    assert {(True, ""), (False, "Newton"), (False, "Armijo")} <= kinds
+    for p, g in list(zip(ps, got))[:7]:
+        w = oracle.correct_reference(p)
+        assert g.converged == w["converged"]
+        if g.converged:
+            assert np.max(np.abs(g.x_corrected - w["x"])) <= TOL_X
+            assert abs(g.half_period - w["half_period"]) <= TOL_X
+
+
+def test_alone_and_in_a_batch_are_identical(gpu):
+    ps = _problems()
+    say("hh_correct: each problem alone vs inside the batch, bit for bit")
+    together = Cr.correct_device(ps)
+    for i in (0, 3, 5, 9, len(ps) - 1):
+        alone = Cr.correct_device([ps[i]])[0]
+        t = together[i]
+        assert np.array_equal(alone.x_corrected, t.x_corrected)
+        assert alone.iterations == t.iterations and alone.error == t.error
+        assert alone.n_event_searches == t.n_event_searches
+        assert (alone.half_period == t.half_period) or (math.isnan(alone.half_period)
+                                                        and math.isnan(t.half_period))
+
+
+def test_invalid_record_and_done_flags(gpu):
+    say("hh_correct: a record the kernel must refuse, and every done flag set")
+    ps = _problems()[:3]
+    rec = Cr.pack_problems(ps)
+    rec[1]["n_residuals"] = 3
+    dev = torch.device("cuda:0")
+    prob = torch.from_numpy(rec.view(np.uint8).copy()).to(dev)
+    res = torch.zeros(3 * Cr.RESULT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    Cr.launch_device(prob.data_ptr(), res.data_ptr(), 3)
+    out = res.cpu().numpy().view(Cr.RESULT_DTYPE)
+    assert out["done"].tolist() == [1, 1, 1]
+    assert out["error"][1] == Cr.HH_CORRECT_INVALID and out["converged"][1] == 0
+    assert np.array_equal(out["x"][1], rec["x0"][1])
+    good = Cr.results_from_records(out[[0, 2]], [ps[0], ps[2]])
+    want = Cr.correct_device([ps[0], ps[2]])
+    for g, w in zip(good, want):
+        assert np.array_equal(g.x_corrected, w.x_corrected)

@@ -2,9 +2,12 @@
 
 * Every job of a mixed stream (corrections + manifold fans arriving as a
   Poisson process) gives the same bits as the same request run alone, both
-  through the loop and through the direct batched calls.
+  through the loop and through the direct calls (correct_device for the
+  device-resident corrector, correct_batch for the host-driven bank).
 * A sample of results against the oracle (the reference's sequential
-  Newton-Armijo loop; its event integration per fan trajectory).
+  Newton-Armijo loop, bit for bit for the bank and to the corrector's
+  tolerance for the device corrector, whose Halo-term powers are correctly
+  rounded; its event integration per fan trajectory).
 * HBM-budget admission: with a budget that holds only a few jobs the loop
   queues the rest, never exceeds the budget, and still completes every job.
 """
@@ -45,25 +48,32 @@ def _same_fan(a, b):
     assert np.array_equal(a.t_hit, b.t_hit) and np.array_equal(a.y_hit, b.y_hit)
 
 
-@pytest.fixture(scope="module")
-def mixed_run(gpu):
+@pytest.fixture(scope="module", params=["device", "bank"])
+def mixed_run(gpu, request):
     reqs = _stream()
-    say(f"serving loop: {len(reqs)} requests over 60 ms")
-    server = S.Server()
+    say(f"serving loop ({request.param} corrector): {len(reqs)} requests over 60 ms")
+    server = S.Server(corrector=request.param)
     jobs = server.run(reqs, time_limit=60.0)
+    server.close()
     assert all(not math.isnan(j.finished) for j in jobs)
-    return reqs, jobs, server
+    return reqs, jobs, request.param
 
 
 def test_mixed_results_equal_direct_calls(mixed_run, oracle):
-    reqs, jobs, _ = mixed_run
-    say("serving results vs direct batched calls and the oracle")
+    reqs, jobs, kind = mixed_run
+    say(f"serving results ({kind} corrector) vs direct calls and the oracle")
     corr = [j for j in jobs if j.kind == "correction"]
     fans = [j for j in jobs if j.kind == "fan"]
     assert corr and fans
-    direct = Cr.correct_batch([j.request for j in corr])
+    probs = [j.request for j in corr]
+    direct = Cr.correct_device(probs) if kind == "device" else Cr.correct_batch(probs)
     for j, d in zip(corr, direct):
         _same_correction(j.result, d)
+    if kind == "device":
+        for j, b in zip(corr, Cr.correct_batch(probs)):
+            assert j.result.converged == b.converged
+            if b.converged:
+                assert np.max(np.abs(j.result.x_corrected - b.x_corrected)) <= 1e-10
     for j in fans:
         r = j.request
         ev = batch.dop853_event(r.mu, r.y0, 0.0, r.tmax, ev_index=1, direction=-1,
@@ -72,8 +82,12 @@ def test_mixed_results_equal_direct_calls(mixed_run, oracle):
                                         ev.y_hit.cpu().numpy(), ev.n_accepted.cpu().numpy()))
     for j in corr[:4]:
         w = oracle.correct_reference(j.request)
-        assert np.array_equal(j.result.x_corrected, w["x"])
-        assert j.result.iterations == w["iterations"] and j.result.error == w["error"]
+        assert j.result.converged == w["converged"]
+        if kind == "bank":
+            assert np.array_equal(j.result.x_corrected, w["x"])
+            assert j.result.iterations == w["iterations"] and j.result.error == w["error"]
+        elif w["converged"]:
+            assert np.max(np.abs(j.result.x_corrected - w["x"])) <= 1e-10
     for j in fans[:2]:
         r = j.request
         hit, th, yh, _, _ = oracle.batch_dop853_event(r.mu, r.y0, r.tmax, ev_index=1,
@@ -83,16 +97,18 @@ def test_mixed_results_equal_direct_calls(mixed_run, oracle):
 
 
 def test_alone_equals_mixed(mixed_run):
-    reqs, jobs, _ = mixed_run
+    reqs, jobs, kind = mixed_run
     pick = [i for i, r in enumerate(reqs) if r["kind"] == "correction"][:3] + \
            [i for i, r in enumerate(reqs) if r["kind"] == "fan"][:2]
-    say(f"alone vs mixed: requests {pick}")
+    say(f"alone vs mixed ({kind} corrector): requests {pick}")
+    server = S.Server(corrector=kind)
     for i in pick:
-        alone = S.Server().run([dict(reqs[i], t=0.0)], time_limit=30.0)[0]
+        alone = server.run([dict(reqs[i], t=0.0)], time_limit=30.0)[0]
         if alone.kind == "correction":
             _same_correction(alone.result, jobs[i].result)
         else:
             _same_fan(alone.result, jobs[i].result)
+    server.close()
 
 
 def test_hbm_budget_admission(gpu):
@@ -102,6 +118,22 @@ def test_hbm_budget_admission(gpu):
     say(f"admission with a {budget} B budget")
     server = S.Server(hbm_budget=budget)
     jobs = server.run(reqs, time_limit=60.0)
+    server.close()
     assert server.max_queue > 0
     assert server.peak_used <= budget
     assert all(not math.isnan(j.finished) and j.admitted >= j.arrival for j in jobs)
+
+
+def test_dedicated_streams_give_the_same_results(mixed_run):
+    reqs, jobs, kind = mixed_run
+    if kind != "device":
+        pytest.skip("one corrector is enough")
+    say("serving loop on dedicated streams: same bits as on torch's pool")
+    server = S.Server(dedicated_streams=True, n_streams=4)
+    again = server.run(reqs, time_limit=60.0)
+    server.close()
+    for a, b in zip(again, jobs):
+        if a.kind == "correction":
+            _same_correction(a.result, b.result)
+        else:
+            _same_fan(a.result, b.result)
