@@ -222,6 +222,57 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
             out["cpu_baseline"] = {"value": 2 * m / dt, "unit": "samples/s", "cores": 1,
                                    "kind": "port",
                                    "sample": f"2 of {n} trajectories, pure-Python restatement"}
+    elif name == "continuation":
+        from hiten_amd import continuation as Ct, corrector as Cr
+        mu = W.MU_EM
+        rng = np.random.default_rng(W.SEED + 7)
+        nf, members = 64, 8
+        # Halo family seeds: corrected orbits from serving-distributed guesses
+        guesses = [Cr.halo_problem(mu, [rng.uniform(0.80, 0.88), 0, rng.uniform(-0.2, 0.2),
+                                        0, rng.uniform(-0.3, 0.3), 0], max_attempts=20)
+                   for _ in range(4 * nf)]
+        seeds = [r.x_corrected for r in Cr.correct_device(guesses) if r.converged][:nf]
+        nf = len(seeds)
+        make = lambda x: Cr.halo_problem(mu, x, max_attempts=20)
+        kw = dict(step=np.array([1e-3]), state_indices=[2], stepper="natural",
+                  max_members=members, max_retries_per_step=6, window=2, corrector="device")
+        Ct.continue_families(seeds[:2], make, **dict(kw, max_members=2))     # warm
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fams = Ct.continue_families(seeds, make, **kw)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
+        acc = sum(f.accepted_count - 1 for f in fams)
+        corr = sum(f.corrections for f in fams)
+        # the reference's loop: one family after another, one correction per call
+        t = time.perf_counter()
+        seq = Ct.continue_families(seeds[:1], make, **dict(kw, window=1))
+        seq_wall = time.perf_counter() - t
+        seq_acc = seq[0].accepted_count - 1
+        out.update(metric="Halo family members continued/s (natural parameter in z, "
+                          "device-resident corrector, 64 families in lock-step)",
+                   value=acc / wall, unit="members/s", ms_per_step=wall * 1e3, steps=1,
+                   warmup=1, corrections=corr, members=acc, families=nf,
+                   sequential_one_family={"value": seq_acc / seq_wall, "unit": "members/s",
+                                          "note": "window 1, one family: the reference's "
+                                                  "order, one correction per launch"},
+                   config={"workload": f"continuation: {nf} Halo families (mu_EM) x "
+                                       f"{members} members, z step 1e-3, retry window 2",
+                           "kernel": "k_correct"},
+                   roofline=None)
+        if not no_cpu:
+            from oracle import oracle as O
+            t = time.perf_counter()
+            n_c = 0
+            for x in seeds[:2]:
+                p = make(x + np.array([0, 0, 1e-3, 0, 0, 0]))
+                O.correct_reference(p)
+                n_c += 1
+            dt = time.perf_counter() - t
+            out["cpu_baseline"] = {"value": n_c / dt, "unit": "members/s", "cores": 1,
+                                   "kind": "port",
+                                   "sample": f"{n_c} corrections of predicted members, "
+                                             "sequential Newton loop on the C oracle"}
     elif name == "manifold_fan":
         w = W.manifold_fan()
         n = w["y0"].shape[0]
@@ -297,7 +348,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--workload", default="persistent_262k",
                     choices=["persistent_262k", "rk8_grid", "stm_uneven", "manifold_fan",
-                             "synodic"])
+                             "synodic", "continuation"])
     args = ap.parse_args()
 
     if args.gpus != 1:

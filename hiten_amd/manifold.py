@@ -193,6 +193,34 @@ class ManifoldResult:
         return (self.ysos, self.dysos, self.states_list, self.times_list, self.successes,
                 self.attempts)
 
+    @property
+    def trajectories(self) -> List[Tuple[np.ndarray, np.ndarray]]:
+        """(times, states) of each kept arc, as Manifold.compute wraps them in
+        Trajectory objects (system/manifold.py)."""
+        return list(zip(self.times_list, self.states_list))
+
+    def to_df(self):
+        """Manifold.to_df (system/manifold.py:373-387): one row per sample,
+        columns trajectory_id, time, x, y, z, vx, vy, vz."""
+        import pandas as pd
+        cols = ["trajectory_id", "time", "x", "y", "z", "vx", "vy", "vz"]
+        if not self.states_list:
+            return pd.DataFrame([], columns=cols)
+        ids = np.concatenate([np.full(len(t), i, dtype=np.int64)
+                              for i, t in enumerate(self.times_list)])
+        t = np.concatenate([np.asarray(t, float) for t in self.times_list])
+        X = np.concatenate([np.asarray(x, float)[:, :6] for x in self.states_list])
+        df = pd.DataFrame(X, columns=cols[2:])
+        df.insert(0, "time", t)
+        df.insert(0, "trajectory_id", ids)
+        return df
+
+    def to_csv(self, filepath: str) -> None:
+        """Manifold.to_csv: to_df written without the index."""
+        import os
+        os.makedirs(os.path.dirname(os.path.abspath(filepath)), exist_ok=True)
+        self.to_df().to_csv(filepath, index=False)
+
 
 def compute_manifold(mu: float, orbit_state, period: float, *, stable: bool = True,
                      direction: Literal["positive", "negative"] = "positive", step: float = 0.02,

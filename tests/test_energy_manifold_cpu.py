@@ -121,3 +121,27 @@ def test_manifold_seeds_identity_stm():
     want = np.stack([xx[0], xx[5], xx[10]])
     want[:, 1] += -1e-6                     # d = 1e-6 / |MAN[:3]|, MAN = -(0, 2, 0, ...)
     assert np.array_equal(seeds, want)
+
+
+def test_manifold_to_df_rows_and_columns(tmp_path):
+    """ManifoldResult.to_df against Manifold.to_df's row-by-row construction
+    (system/manifold.py:373-387) restated literally."""
+    import pandas as pd
+    from hiten_amd.manifold import ManifoldResult
+    rng = np.random.default_rng(4)
+    times = [np.linspace(0.0, 1.0, 5), np.linspace(0.0, -2.0, 3)]
+    states = [rng.standard_normal((5, 6)), rng.standard_normal((3, 6))]
+    res = ManifoldResult(states_list=states, times_list=times, successes=2, attempts=2)
+    rows = []
+    for i, (t, x) in enumerate(res.trajectories):
+        for j in range(x.shape[0]):
+            rows.append([i, t[j], *x[j, :6]])
+    want = pd.DataFrame(rows, columns=["trajectory_id", "time", "x", "y", "z", "vx", "vy", "vz"])
+    got = res.to_df()
+    assert list(got.columns) == list(want.columns)
+    assert np.array_equal(got.to_numpy(), want.to_numpy())
+    path = tmp_path / "out" / "m.csv"
+    res.to_csv(str(path))
+    back = pd.read_csv(path)
+    assert back.shape == (8, 8) and back["trajectory_id"].tolist() == [0] * 5 + [1] * 3
+    assert ManifoldResult().to_df().shape == (0, 8)
