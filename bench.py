@@ -32,16 +32,16 @@ FLOPS_PER_STEP_42 = 8622          # algorithmic FLOPs of one accepted 42-d DOP85
 FP64_PEAK_TFLOPS = 78.6           # MI355X FP64 vector/matrix peak
 
 
-def profile_path(kernel: str) -> str:
-    return os.path.join(ROOT, "profiles", f"{kernel}__persistent_262k.md")
+def profile_path(kernel: str, workload: str = "persistent_262k") -> str:
+    return os.path.join(ROOT, "profiles", f"{kernel}__{workload}.md")
 
 
-def profiled_traffic(kernel: str):
-    """HBM bytes per launch of ``kernel`` on the headline workload from its
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel
-    calibrated), or None without one."""
+def profiled_traffic(kernel: str, workload: str = "persistent_262k"):
+    """HBM bytes per launch of ``kernel`` on ``workload`` from its committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated), or
+    None without one."""
     try:
-        with open(profile_path(kernel)) as f:
+        with open(profile_path(kernel, workload)) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch"):
                     return int(line.split("=")[1])
@@ -209,7 +209,7 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
                                        "segment_refine 20", "kernel": "k_synodic_detect"},
                    roofline={"bound": "hbm", "achieved": nbytes / ls / 1e9, "peak": 8000.0,
                              "unit": "GB/s", "frac": nbytes / ls / 1e9 / 8000.0,
-                             "traffic": None})
+                             "traffic": profiled_traffic("k_synodic_detect", "synodic")})
         if not no_cpu:
             from oracle import synodic as OS
             host = states[:2].cpu().numpy()
