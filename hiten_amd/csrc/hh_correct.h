@@ -9,7 +9,9 @@
  *              propagation + event search), in the Group6 layout -- the
  *              sixteen 8-lane groups of the workgroup evaluate the residual,
  *              or all line-search trials of a step at once (in chunks of
- *              sixteen, stopping as soon as a trial in order is accepted);
+ *              sixteen); as soon as the trials finished so far fix the
+ *              line search's outcome (a trial accepted with all trials
+ *              before it known), the groups still on later trials stop;
  *   Jacobian   Phi(t_event) from [I | x] (corrector/operators.py:298-307,
  *              437-452) in the Pair42 layout (state wave and STM wave), then
  *              the Halo term (services/orbits.py:917-947);
@@ -55,6 +57,8 @@ struct CorrLds {
     int ok[hh::NEWTON_JMAX];
     double phi[42];
     int n_eval;   /* states to evaluate this round: 1 (residual) or nal (trials) */
+    int decided;  /* the line search's outcome is fixed: the other trials stop   */
+    int known[hh::NEWTON_JMAX];   /* trial j evaluated (its results are in)     */
     int steps;    /* accepted integration steps so far                           */
     int more;     /* 1 while the line search is not yet decided                  */
     int fail;     /* the problem is not one this kernel takes                    */
@@ -174,9 +178,9 @@ __global__ void k_eval_solve2(int n, const double* __restrict__ J, const double*
  * optional alignment propagation to the window's start and an event search
  * over its length (singlehit/backend.py:202-280).  ok, t and (this lane's
  * component of) x_event as the batched path forms them.                   */
-__device__ __forceinline__ void corr_residual(const Group6& ga, const Group6& ge,
+__device__ __forceinline__ bool corr_residual(const Group6& ga, const Group6& ge,
                                               const CorrArgs& a, double y, int& ok, double& t,
-                                              double& xe, int& steps) {
+                                              double& xe, int& steps, const int* cancel) {
     ok = 0;
     t = NAN;
     xe = NAN;
@@ -185,22 +189,48 @@ __device__ __forceinline__ void corr_residual(const Group6& ga, const Group6& ge
         double ya = y;
         if (a.aligned[w]) {
             int st, na, nr;
-            group6_run<false>(ga, y, 0.0, a.ts[w], a.sp_align, nullptr, 0, nullptr, ya, st, na,
-                              nr);
+            group6_run<false, true>(ga, y, 0.0, a.ts[w], a.sp_align, nullptr, 0, nullptr, ya, st,
+                                    na, nr, cancel);
             steps += na;
+            if (st == ST_CANCELLED) return false;
         }
         if (!a.searched[w]) continue;
         EventOut6 o;
-        group6_event(ge, ya, 0.0, a.span[w], a.sp_event, a.ev, o);
+        group6_event<true>(ge, ya, 0.0, a.span[w], a.sp_event, a.ev, o, cancel);
         steps += o.nacc;
+        if (o.status == ST_CANCELLED) return false;
         const double t_rel = o.found ? o.t_hit : a.span[w];
         if (o.found && t_rel < a.span[w] && t_rel >= 0.0) {
             ok = 1;
             t = a.ts[w] + t_rel;
             xe = o.y_hit;
-            return;
+            return true;
         }
     }
+    return true;
+}
+
+/* After trial j is in: whether the trials known so far, in order, fix the
+ * line search (armijo.py:111-164 is sequential: the first accepted trial
+ * ends it, and trials after it are never looked at); if so, set `decided`
+ * so that the groups still integrating later trials stop.  Each group
+ * publishes its trial before scanning, so of two groups that finish
+ * together at least one sees both; if neither did, the round would only
+ * end later, when every group has finished -- the decision itself is taken
+ * by thread 0 afterwards from the results, never from this flag.        */
+__device__ __forceinline__ void corr_try_decide(CorrLds& L, int n_eval) {
+    const hh::NewtonRow& w = L.row;
+    for (int j = 0; j < n_eval; ++j) {
+        if (!__hip_atomic_load(&L.known[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP))
+            return;
+        if (!L.ok[j]) continue;
+        double r[2];
+        for (int q = 0; q < 2; ++q) r[q] = L.xe[j][w.res[q]] - w.tgt[q];
+        if (hh::newton_inf_norm(r, 2) <= (1.0 - w.c * w.alpha[j]) * w.rn) break;
+    }
+    int expect = 0;
+    __hip_atomic_compare_exchange_strong(&L.decided, &expect, 1, __ATOMIC_RELAXED,
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 /* One correction per 128-thread workgroup.  Thread 0 keeps the row in LDS
@@ -227,6 +257,8 @@ __global__ __launch_bounds__(128) void k_correct(CorrArgs a) {
                                      pb.min_alpha, pb.armijo_c) != 0;
         L.n_eval = 1;
         L.steps = 0;
+        L.decided = 0;
+        L.known[0] = 0;
     }
     __syncthreads();
     if (!L.fail) {
@@ -252,12 +284,21 @@ __global__ __launch_bounds__(128) void k_correct(CorrArgs a) {
                     }
                     int ok, steps;
                     double t, xe;
-                    corr_residual(ga, ge, a, y, ok, t, xe, steps);
-                    if (ge.act) L.xe[j][ge.d] = xe;
+                    const bool in = corr_residual(ga, ge, a, y, ok, t, xe, steps, &L.decided);
+                    if (in && ge.act) L.xe[j][ge.d] = xe;
                     if (ge.d == 0) {
-                        L.t[j] = t;
-                        L.ok[j] = ok;
                         atomicAdd(&L.steps, steps);
+                        if (in) {
+                            L.t[j] = t;
+                            L.ok[j] = ok;
+                        }
+                    }
+                    /* this group's results, then its flag, then the scan */
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    if (in && ge.d == 0 && trials) {
+                        __hip_atomic_store(&L.known[j], 1, __ATOMIC_SEQ_CST,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                        corr_try_decide(L, n_eval);
                     }
                 }
                 __syncthreads();
@@ -303,6 +344,8 @@ __global__ __launch_bounds__(128) void k_correct(CorrArgs a) {
                 corr_solve2(J, r, d);
                 hh::newton_set_step(L.row, 2, d);
                 L.n_eval = L.row.nal;
+                L.decided = 0;
+                for (int j = 0; j < L.row.nal; ++j) L.known[j] = 0;
             }
             __syncthreads();
         }

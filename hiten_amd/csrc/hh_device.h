@@ -66,6 +66,7 @@ constexpr int RK8_S = 13;
 constexpr int ST_OK = 0;
 constexpr int ST_MAXSTEPS = 1;
 constexpr int ST_NONFINITE = 2;
+constexpr int ST_CANCELLED = 3;   /* stopped by its caller; the result is not used */
 
 __device__ __forceinline__ double np_max(double a, double b) {
     /* numpy.maximum: NaN-propagating */

@@ -392,15 +392,25 @@ __global__ __launch_bounds__(256) void k_dop853_run6(
 /* k_dop853_run6 with one trajectory per aligned 8-lane group, one component
  * per lane (Group6).  traj_init / traj_iterate and the FinalOut / GridOut
  * policies, statement for statement, with each lane holding its component. */
+/* A caller's request to stop a loop early: with CANCEL set, the loops below
+ * read the flag (LDS) before each attempt and stop with ST_CANCELLED once it
+ * is nonzero.  Without it they compile as before.                         */
+template <bool CANCEL>
+__device__ __forceinline__ bool cancelled(const int* flag) {
+    if constexpr (CANCEL)
+        return __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+    return false;
+}
+
 /* One trajectory of k_dop853_run6 in the Group6 layout, statement for
  * statement: the grid form writes this lane's component of every output
  * point to gout ([m][6]); the final form returns it in yfin.  Status and
  * counters are the same in all lanes of the group.                         */
-template <bool GRID>
+template <bool GRID, bool CANCEL = false>
 __device__ __forceinline__ void group6_run(const Group6& p, double y, double t0, double tf,
                                            const StepParams& sp, double* gout, int m,
                                            const double* teval, double& yfin, int& status,
-                                           int& nacc, int& nrej) {
+                                           int& nacc, int& nrej, const int* cancel = nullptr) {
     const int d = p.d;
     double k[DOP_K];
 #pragma unroll
@@ -436,6 +446,11 @@ __device__ __forceinline__ void group6_run(const Group6& p, double y, double t0,
             st = ST_MAXSTEPS;
             done = true;
             if constexpr (GRID) { for (; q < m; ++q) put(q, y); } else { yfin = y; }
+            break;
+        }
+        if (cancelled<CANCEL>(cancel)) {
+            st = ST_CANCELLED;
+            yfin = y;
             break;
         }
         double hc = clamp_step(h, sp.max_step, sp.min_step);
@@ -635,9 +650,10 @@ struct EventOut6 {
     double t_hit, y_hit, y_last;
 };
 
+template <bool CANCEL = false>
 __device__ __forceinline__ void group6_event(const Group6& p, double y, double t, double tmax,
                                              const StepParams& sp, const EventParams& ev,
-                                             EventOut6& o) {
+                                             EventOut6& o, const int* cancel = nullptr) {
     double k[DOP_K];
 #pragma unroll
     for (int j = 0; j < DOP_K; ++j) k[j] = 0.0;
@@ -651,6 +667,10 @@ __device__ __forceinline__ void group6_event(const Group6& p, double y, double t
     while ((t - tmax) * 1.0 < 0.0) {
         if (na + nr >= sp.max_steps) {
             st = ST_MAXSTEPS;
+            break;
+        }
+        if (cancelled<CANCEL>(cancel)) {
+            st = ST_CANCELLED;
             break;
         }
         h = clamp_step(h, sp.max_step, sp.min_step);
