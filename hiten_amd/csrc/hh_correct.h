@@ -56,6 +56,12 @@ struct CorrLds {
     double xe[hh::NEWTON_JMAX][6];
     int ok[hh::NEWTON_JMAX];
     double phi[42];
+    double spec_phi[42];  /* Phi of the round's state 0, integrated speculatively  */
+    double spec_tf;       /* its event time, once known                         */
+    int spec_known;       /* set when state 0's residual is in                  */
+    int spec_ok;          /* state 0 crossed the section                        */
+    int spec_valid;       /* spec_phi holds Phi(spec_tf) of state 0             */
+    int use_spec;         /* this round's STM is state 0's: take spec_phi       */
     int n_eval;   /* states to evaluate this round: 1 (residual) or nal (trials) */
     int decided;  /* the line search's outcome is fixed: the other trials stop   */
     int known[hh::NEWTON_JMAX];   /* trial j evaluated (its results are in)     */
@@ -233,19 +239,150 @@ __device__ __forceinline__ void corr_try_decide(CorrLds& L, int n_eval) {
                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-/* One correction per 128-thread workgroup.  Thread 0 keeps the row in LDS
- * and takes the decisions between barriers; every branch the two waves take
- * on LDS values is read after a barrier, so all threads meet every barrier
- * (the Pair42 integration's included).  Each loop ends: every integration
- * is bounded by max_steps and the Newton loop by max_attempts.             */
-__global__ __launch_bounds__(128) void k_correct(CorrArgs a) {
-    __shared__ PairLds plds;
+/* Phi(t_event) of the round's state 0 in the Wave42 layout, begun before
+ * t_event is known.  DOP853's steps do not depend on the end time until a
+ * step would pass it (integrators/utils.py:205-207 clips that step; the
+ * initial step, utils.py:149-157, ignores it), so the integration runs
+ * unclipped until state 0's event search publishes t_event, then returns
+ * to the last state from which no attempt reached t_event and finishes as
+ * the sequential integration does from there: same attempts, same bits.
+ * It keeps the starting state and the last two accepted states with the
+ * furthest time their attempts reached; if an older state's attempts
+ * reached t_event it starts over.  Returns 1 with this lane's component of
+ * the final state in `out`, 0 when state 0 has no crossing (no STM is
+ * needed), -1 when it gives up (a non-finite error norm before t_event is
+ * known, or the attempt cap): the caller then integrates as usual.       */
+__device__ __forceinline__ int corr_spec_stm(const Wave42& W, const StepParams& sp, double y0,
+                                             CorrLds& L, double& out, int& nacc_out) {
+    double k[DOP_K];
+#pragma unroll
+    for (int j = 0; j < DOP_K; ++j) k[j] = 0.0;
+    k[0] = W.rhs(y0);
+    const double k00 = k[0];
+    const double sc = sp.atol + sp.rtol * __builtin_fabs(y0);
+    const double ua = y0 / sc;
+    const double ub = k[0] / sc;
+    const double s0 = W.block_tree(W.act ? ua * ua : 0.0);
+    const double s1 = W.block_tree(W.act ? ub * ub : 0.0);
+    const double sn = __builtin_sqrt(42.0);
+    const double h0 = select_initial_step(__builtin_sqrt(s0) / sn, __builtin_sqrt(s1) / sn,
+                                          sp.min_step, sp.max_step);
+    /* current state */
+    double y = y0, t = 0.0, h = h0, ep = -1.0;
+    int na = 0, nr = 0;
+    /* B: the last accepted state (the start at first), A: the one before;
+     * rB, rA: furthest t + h their attempts reached; older: the same over
+     * every state before A                                                */
+    double yB = y0, kB = k00, tB = 0.0, hB = h0, epB = -1.0, rB = -INFINITY;
+    int naB = 0, nrB = 0;
+    double yA = y0, kA = k00, tA = 0.0, hA = h0, epA = -1.0, rA = -INFINITY;
+    int naA = 0, nrA = 0;
+    double older = -INFINITY;
+    const long long cap = sp.max_steps;
+    for (;;) {
+        if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                &L.spec_known, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)))
+            break;
+        if (na + nr >= cap) return -1;
+        const double hh = clamp_step(h, sp.max_step, sp.min_step);
+        rB = (t + hh > rB) ? t + hh : rB;
+        double yh;
+        const double err = W.attempt(y, k, hh, sp.rtol, sp.atol, yh);
+        if (err != err) return -1;
+        double fa, fr;
+        W.step_factors(err, ep, fa, fr);
+        if (err <= 1.0) {
+            older = (rA > older) ? rA : older;
+            yA = yB; kA = kB; tA = tB; hA = hB; epA = epB; naA = naB; nrA = nrB; rA = rB;
+            t = t + hh;
+            y = yh;
+            k[0] = k[DOP_S];
+            h = hh * fa;
+            ep = err;
+            na += 1;
+            yB = y; kB = k[0]; tB = t; hB = h; epB = ep; naB = na; nrB = nr; rB = -INFINITY;
+        } else {
+            h = clamp_step(hh * fr, sp.max_step, sp.min_step);
+            nr += 1;
+        }
+    }
+    if (!L.spec_ok) return 0;
+    const double tf = L.spec_tf;
+    if (isclose(0.0, tf) || !((0.0 - tf) * 1.0 < 0.0)) {
+        /* zero or reversed span: y0 unchanged (dynamics/base.py:421-424) */
+        out = y0;
+        nacc_out = 0;
+        return 1;
+    }
+    if (older >= tf) {
+        y = y0; k[0] = k00; t = 0.0; h = h0; ep = -1.0; na = 0; nr = 0;
+    } else if (rA >= tf) {
+        y = yA; k[0] = kA; t = tA; h = hA; ep = epA; na = naA; nr = nrA;
+    } else if (rB >= tf) {
+        y = yB; k[0] = kB; t = tB; h = hB; ep = epB; na = naB; nr = nrB;
+    }
+    int status;
+    wide_loop(sp, W, y, k, t, tf, h, ep, na, nr, out, status);
+    nacc_out = na;
+    return 1;
+}
+
+/* Phi(tf) from y0 in the Wave42 layout, the end time known from the start
+ * (k_dop853_final42_wide's loop).                                         */
+__device__ __forceinline__ void corr_stm(const Wave42& W, const StepParams& sp, double y0,
+                                         double tf, double& out, int& nacc) {
+    nacc = 0;
+    if (isclose(0.0, tf) || !((0.0 - tf) * 1.0 < 0.0)) {
+        out = y0;
+        return;
+    }
+    double k[DOP_K];
+#pragma unroll
+    for (int j = 0; j < DOP_K; ++j) k[j] = 0.0;
+    k[0] = W.rhs(y0);
+    const double sc = sp.atol + sp.rtol * __builtin_fabs(y0);
+    const double ua = y0 / sc;
+    const double ub = k[0] / sc;
+    const double s0 = W.block_tree(W.act ? ua * ua : 0.0);
+    const double s1 = W.block_tree(W.act ? ub * ub : 0.0);
+    const double sn = __builtin_sqrt(42.0);
+    const double h = select_initial_step(__builtin_sqrt(s0) / sn, __builtin_sqrt(s1) / sn,
+                                         sp.min_step, sp.max_step);
+    int nr = 0, status;
+    wide_loop(sp, W, y0, k, 0.0, tf, h, -1.0, nacc, nr, out, status);
+}
+
+/* This lane's component of [I | state]: the identity block of Phi, then
+ * the full 6-d state of the controls q.                                   */
+__device__ __forceinline__ double corr_stm_start(const Wave42& W, const hh::NewtonRow& w,
+                                                 const double* q) {
+    if (!W.holds()) return 0.0;
+    const int c = W.comp();
+    if (c < 36) return (c / 6 == c % 6) ? 1.0 : 0.0;
+    double v = w.base[c - 36];
+    for (int j = 0; j < 2; ++j)
+        if (w.ctrl[j] == c - 36) v = q[j];
+    return v;
+}
+
+/* One correction per 192-thread workgroup: waves 0 and 1 are the sixteen
+ * Group6 groups that evaluate residuals and line-search trials, wave 2
+ * integrates the STMs in the Wave42 layout -- in each round speculatively,
+ * for the round's state 0 (the residual's state, or the alpha = 1 trial,
+ * which is the next iterate in most steps) while the groups search for
+ * the crossings.  Thread 0 keeps the row in LDS and takes the decisions
+ * between barriers; every branch the waves take on LDS values is read after
+ * a barrier, so all threads meet every barrier.  Each loop ends: every
+ * integration is bounded by max_steps and the Newton loop by max_attempts,
+ * and the STM wave's speculation ends when state 0's search publishes its
+ * event time, which the first group always does.                         */
+__global__ __launch_bounds__(192) void k_correct(CorrArgs a) {
     __shared__ CorrLds L;
     const int i = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int grp = tid >> 3;
-    const int role = __builtin_amdgcn_readfirstlane((int)(tid >> 6));
+    const int wid = __builtin_amdgcn_readfirstlane((int)(tid >> 6));
     if (i >= a.n) return;
     if (tid == 0) {
         const hh_correction_t& pb = a.prob[i];
@@ -259,6 +396,8 @@ __global__ __launch_bounds__(128) void k_correct(CorrArgs a) {
         L.steps = 0;
         L.decided = 0;
         L.known[0] = 0;
+        L.spec_known = 0;
+        L.spec_valid = 0;
     }
     __syncthreads();
     if (!L.fail) {
@@ -266,15 +405,16 @@ __global__ __launch_bounds__(128) void k_correct(CorrArgs a) {
         Group6 ga, ge;
         ga.setup(lane, mu, L.row.fwd, a.flip_align);
         ge.setup(lane, mu, 1, 0ULL);
-        Pair42 P;
-        P.setup(lane, role, mu, 1, a.flip_stm, &plds);
+        Wave42 W;
+        W.setup(lane, mu, 1, a.flip_stm);
         for (;;) {
-            /* the residual, or the line-search trials of the step */
+            /* the residual, or the line-search trials of the step; the STM
+             * wave meanwhile integrates state 0's STM                      */
             const int n_eval = L.n_eval;
             const bool trials = L.row.stage == hh::NS_ARMIJO;
             for (int j0 = 0; j0 < n_eval; j0 += CORR_GROUPS) {
                 const int j = j0 + grp;
-                if (j < n_eval) {
+                if (wid < 2 && j < n_eval) {
                     const double* q = trials ? L.row.trial[j] : L.row.x;
                     double y = 0.0;
                     if (ge.act) {
@@ -292,13 +432,30 @@ __global__ __launch_bounds__(128) void k_correct(CorrArgs a) {
                             L.t[j] = t;
                             L.ok[j] = ok;
                         }
+                        if (j == 0) {
+                            L.spec_tf = t;
+                            L.spec_ok = in && ok;
+                        }
                     }
-                    /* this group's results, then its flag, then the scan */
+                    /* this group's results, then its flags, then the scan */
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    if (ge.d == 0 && j == 0)
+                        __hip_atomic_store(&L.spec_known, 1, __ATOMIC_RELEASE,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
                     if (in && ge.d == 0 && trials) {
                         __hip_atomic_store(&L.known[j], 1, __ATOMIC_SEQ_CST,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
                         corr_try_decide(L, n_eval);
+                    }
+                } else if (wid == 2 && j0 == 0) {
+                    const double y0 = corr_stm_start(W, L.row, trials ? L.row.trial[0] : L.row.x);
+                    double out = 0.0;
+                    int na = 0;
+                    const int r = corr_spec_stm(W, a.sp_stm, y0, L, out, na);
+                    if (r == 1 && W.holds()) L.spec_phi[W.comp()] = out;
+                    if (lane == 0) {
+                        L.spec_valid = (r == 1);
+                        L.steps += na;
                     }
                 }
                 __syncthreads();
@@ -311,31 +468,29 @@ __global__ __launch_bounds__(128) void k_correct(CorrArgs a) {
                 if (!L.more) break;
             }
             if (tid == 0) {
+                int taken = 0;
                 if (L.row.stage == hh::NS_RESID)
                     hh::newton_residual(L.row, 2, L.t[0], L.xe[0], L.ok[0]);
                 else
-                    hh::newton_armijo(L.row, 2, 2, L.t, &L.xe[0][0], L.ok);
+                    taken = hh::newton_armijo(L.row, 2, 2, L.t, &L.xe[0][0], L.ok);
+                /* the STM this step needs is state 0's when state 0 is the
+                 * next iterate: its speculative integration ended at the
+                 * same event time                                            */
+                L.use_spec = (L.row.stage == hh::NS_JAC) && taken == 0 && L.spec_valid;
             }
             __syncthreads();
             if (L.row.stage == hh::NS_DONE) break;
             /* Phi(t_event) from [I | x] */
-            {
-                double y0 = 0.0;
-                if (P.holds()) {
-                    const int c = P.comp();
-                    if (c < 36) {
-                        y0 = (c / 6 == c % 6) ? 1.0 : 0.0;
-                    } else {
-                        y0 = L.row.base[c - 36];
-                        for (int q = 0; q < 2; ++q)
-                            if (L.row.ctrl[q] == c - 36) y0 = L.row.x[q];
-                    }
+            if (wid == 2) {
+                if (L.use_spec) {
+                    if (W.holds()) L.phi[W.comp()] = L.spec_phi[W.comp()];
+                } else {
+                    double out;
+                    int na;
+                    corr_stm(W, a.sp_stm, corr_stm_start(W, L.row, L.row.x), L.row.t_ev, out, na);
+                    if (W.holds()) L.phi[W.comp()] = out;
+                    if (lane == 0) L.steps += na;
                 }
-                double out;
-                int st, na, nr;
-                pair_final(P, a.sp_stm, y0, 0.0, L.row.t_ev, out, st, na, nr);
-                if (P.holds()) L.phi[P.comp()] = out;
-                if (P.leader()) L.steps += na;
             }
             __syncthreads();
             if (tid == 0) {
@@ -346,6 +501,8 @@ __global__ __launch_bounds__(128) void k_correct(CorrArgs a) {
                 L.n_eval = L.row.nal;
                 L.decided = 0;
                 for (int j = 0; j < L.row.nal; ++j) L.known[j] = 0;
+                L.spec_known = 0;
+                L.spec_valid = 0;
             }
             __syncthreads();
         }

@@ -2008,7 +2008,7 @@ int hh_correct(int n, const hh_correction_t* problems, hh_correction_result_t* r
     a.ev.gtol = 1e-12;
     a.flip_align = ~0ULL;                     /* backward alignment flips all six */
     a.flip_stm = ((1ULL << 42) - 1) & ~((1ULL << 36) - 1);   /* the state block */
-    hipLaunchKernelGGL(k_correct, dim3(n), dim3(128), 0, (hipStream_t)stream, a);
+    hipLaunchKernelGGL(k_correct, dim3(n), dim3(192), 0, (hipStream_t)stream, a);
     return check_launch("hh_correct launch");
 }
 

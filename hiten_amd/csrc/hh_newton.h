@@ -162,9 +162,10 @@ HH_HD inline bool newton_armijo_decided(const NewtonRow& w, int nr, int n, const
 
 /* armijo.py:111-164 over the speculatively evaluated trials: the first
  * accepted one, else the best one, else failure; the accepted trial's
- * event search is the residual at the next iterate.                       */
-HH_HD inline void newton_armijo(NewtonRow& w, int nr, int nc, const double* t, const double* xe,
-                                const int* ok) {
+ * event search is the residual at the next iterate.  Returns the trial
+ * taken, or -1.                                                           */
+HH_HD inline int newton_armijo(NewtonRow& w, int nr, int nc, const double* t, const double* xe,
+                               const int* ok) {
     int chosen = -1, best = -1;
     double best_norm = w.rn;
     for (int j = 0; j < w.nal; ++j) {
@@ -185,12 +186,13 @@ HH_HD inline void newton_armijo(NewtonRow& w, int nr, int nc, const double* t, c
     if (chosen < 0) chosen = best;
     if (chosen < 0) {
         newton_finish(w, 0, HH_BANK_LINE_SEARCH_FAILED, NAN);
-        return;
+        return -1;
     }
     for (int q = 0; q < nc; ++q) w.x[q] = w.trial[chosen][q];
     w.k += 1;
     w.stage = NS_RESID;
     newton_residual(w, nr, t[chosen], xe + 6 * chosen, ok[chosen]);
+    return chosen;
 }
 
 /* services/orbits.py:917-947 */
