@@ -1,29 +1,31 @@
 /* hh_correct.h -- the device-resident single-shooting corrector.
  *
- * One workgroup of two waves runs one correction's whole Newton-Armijo loop
- * (the row functions of hh_newton.h: corrector/backends/newton.py:39-143,
- * stepping/armijo.py:58-164) without returning to the host:
+ * One workgroup of three waves runs one correction's whole Newton-Armijo
+ * loop (the row functions of hh_newton.h: corrector/backends/newton.py:
+ * 39-143, stepping/armijo.py:58-164) without returning to the host:
  *
  *   residual   _y_plane_crossing (poincare/singlehit/backend.py:237-280):
  *              the first window, then the fallback window (alignment
  *              propagation + event search), in the Group6 layout -- the
- *              sixteen 8-lane groups of the workgroup evaluate the residual,
+ *              sixteen 8-lane groups of waves 0 and 1 evaluate the residual,
  *              or all line-search trials of a step at once (in chunks of
  *              sixteen); as soon as the trials finished so far fix the
  *              line search's outcome (a trial accepted with all trials
  *              before it known), the groups still on later trials stop;
  *   Jacobian   Phi(t_event) from [I | x] (corrector/operators.py:298-307,
- *              437-452) in the Pair42 layout (state wave and STM wave), then
- *              the Halo term (services/orbits.py:917-947);
+ *              437-452) in the Wave42 layout on wave 2 -- for the round's
+ *              state 0 speculatively, while the groups search for its
+ *              crossing (corr_spec_stm) -- then the Halo term
+ *              (services/orbits.py:917-947);
  *   step       the 2x2 dense solve with its regularisation
  *              (corrector/backends/base.py:187-245), as NumPy's LAPACK
  *              forms it (corr_solve2), capped and spread over the trials.
  *
  * The integrations are the library's kernels' loops (group6_run,
- * group6_event, pair_final), so each evaluated state's bits are those of the
- * batched path.  Two differences from the host-driven bank remain: the Halo
- * term's powers are correctly rounded here (the C library's pow is within
- * 0.52 ulp), and an exactly singular system takes a closed-form 2x2
+ * group6_event, wide_loop), so each evaluated state's bits are those of
+ * the batched path.  Two differences from the host-driven bank remain: the
+ * Halo term's powers are correctly rounded here (the C library's pow is
+ * within 0.52 ulp), and an exactly singular system takes a closed-form 2x2
  * minimum-norm solution where NumPy calls LAPACK's dgelsd.  Either can move
  * the iterates by an ulp, so results agree with the bank to the
  * corrector's tolerance, not bit for bit.
@@ -37,7 +39,7 @@
 
 #include "hh_newton.h"
 
-constexpr int CORR_GROUPS = 16;   /* 8-lane groups of the two waves */
+constexpr int CORR_GROUPS = 16;   /* 8-lane groups of waves 0 and 1 */
 
 struct CorrArgs {
     int n;

@@ -297,7 +297,7 @@ int hh_bank_result(void* bank, int row, double* x_full, double* residual_norm, i
 int hh_bank_release(void* bank, int row);
 
 /* ---- device-resident corrector -----------------------------------------
- * One correction per 128-thread workgroup, its whole Newton-Armijo loop on
+ * One correction per 192-thread workgroup, its whole Newton-Armijo loop on
  * the device: replaces the host-driven rounds of the bank above (the
  * reference's _NewtonBackend.run, corrector/backends/newton.py:39-143, with
  * _ArmijoLineSearch, stepping/armijo.py:58-164; residual and Jacobian
