@@ -12,7 +12,8 @@ the timed region contains only kernel launches on the current stream.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 
-Prints one JSON line (see README of the metric fields in DESIGN.md).
+Prints one JSON line: metric, value, unit, timing, config, roofline of the
+dominant kernel and the CPU baseline.
 """
 
 from __future__ import annotations
@@ -352,9 +353,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "trajectories are independent, see DESIGN.md for how the batch would shard.",
-              file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the "
+              "trajectories are independent, so a batch shards by trajectory index with one "
+              "gather of the results.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -187,7 +187,7 @@ def _propagate_dynsys(dynsys, state0: Sequence[float], t0: float, tf: float, for
                                 rtol=kwargs.get("rtol", 1e-12), atol=kwargs.get("atol", 1e-12))
     elif method == "symplectic":
         raise NotImplementedError("symplectic integration applies to polynomial Hamiltonian "
-                                  "systems, which are not part of this build")
+                                  "systems, which hiten_amd does not integrate")
     else:
         raise ValueError(f"unknown method {method!r}")
     sol = integrator.integrate(dynsys_dir, state0_np, t_eval, event_fn=event_fn,

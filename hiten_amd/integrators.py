@@ -312,4 +312,4 @@ class RungeKutta:
             return AdaptiveRK(order=8, **opts)
         if order == 8:
             return FixedRK(order=8, **opts)
-        raise NotImplementedError("Runge-Kutta orders 4, 6 and 45 are not part of this build")
+        raise NotImplementedError("hiten_amd implements Runge-Kutta orders 8 and 853 only")

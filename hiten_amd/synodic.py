@@ -11,7 +11,7 @@ deduplicate, and return the hits with their 2-d section points.  One
 
 Parity: hit counts and times equal the reference's; hit states agree to an
 ulp (the reference's state interpolation squares through the C library's
-pow, this build multiplies).  The plane value is summed left to right
+pow, the kernel multiplies).  The plane value is summed left to right
 (exact for coordinate-plane normals; the reference's BLAS dot product order
 is implementation-defined for general normals).
 """

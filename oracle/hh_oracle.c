@@ -151,7 +151,7 @@ void or_rhs(or_sys *s, double t, const double *y, double *dy) {
 /* ---------------------------------------------------------------------- */
 
 /* Sum of squares.  The reference calls np.dot / np.linalg.norm (BLAS ddot /
- * dnrm2, whose summation order is implementation-defined).  This build fixes
+ * dnrm2, whose summation order is implementation-defined).  The restatement fixes
  * one order: sequential from 0.0 for dim != 42; for dim == 42 the components
  * are grouped in blocks -- b0 = state (36..41), b1..b6 = STM column j (rows
  * 0..5 of column j) -- each summed sequentially, then combined as
