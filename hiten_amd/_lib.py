@@ -93,6 +93,8 @@ _SIGNATURES = {
     "hh_bank_release": ([_P, _I], _I),
     "hh_correct": ([_I, _P, _P, _D, _D, _D, _D, _P], _I),
     "hh_correct_sizes": ([_P, _P], _I),
+    "hh_plane_crossing": ([_I, _P, _P, _I, _I, _D, _I, _D, _D, _D, _D, _D, _D, _P, _P, _P, _P,
+                           _P], _I),
     "hh_eval_solve2": ([_I, _P, _P, _P, _P, _P], _I),
 }
 

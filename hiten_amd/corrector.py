@@ -297,8 +297,11 @@ class _Bank:
 
     def feed_events(self, rows, t, xe, ok) -> None:
         rows = _i32(rows)
-        self._check(self._l.hh_bank_feed_events(self._h, len(rows), _p(rows), _p(_f64(t)),
-                                                _p(_f64(xe)), _p(_i32(ok))), "hh_bank_feed_events")
+        # converted arrays are held by names for the call: a temporary would
+        # be freed before the library reads it
+        t, xe, ok = _f64(t), _f64(xe), _i32(ok)
+        self._check(self._l.hh_bank_feed_events(self._h, len(rows), _p(rows), _p(t), _p(xe),
+                                                _p(ok)), "hh_bank_feed_events")
 
     def stm_inputs(self, rows):
         rows = _i32(rows)
@@ -313,7 +316,8 @@ class _Bank:
         rows = _i32(rows)
         n = len(rows)
         J, r = np.empty((n, self.nr, self.nc)), np.empty((n, self.nr))
-        self._check(self._l.hh_bank_jacobians(self._h, n, _p(rows), _p(_f64(Phi)), _p(J), _p(r)),
+        Phi = _f64(Phi)
+        self._check(self._l.hh_bank_jacobians(self._h, n, _p(rows), _p(Phi), _p(J), _p(r)),
                     "hh_bank_jacobians")
         d = _f64(solve_delta_stack(J, r))
         self._check(self._l.hh_bank_set_steps(self._h, n, _p(rows), _p(d)), "hh_bank_set_steps")

@@ -41,15 +41,24 @@
 
 constexpr int CORR_GROUPS = 16;   /* 8-lane groups of waves 0 and 1 */
 
+/* The search of _y_plane_crossing without a time guess: two windows
+ * (start, length), each an optional alignment propagation to its start and
+ * an event search over its length.                                        */
+struct CrossArgs {
+    double ts[2], span[2];        /* the two search windows (start, length) */
+    int aligned[2], searched[2];  /* not numpy.isclose(0, start / length)    */
+    StepParams sp_align, sp_event;
+    EventParams ev;
+    unsigned long long flip_align;
+};
+
 struct CorrArgs {
     int n;
     const hh_correction_t* prob;
     hh_correction_result_t* res;
-    double ts[2], span[2];        /* the two search windows (start, length) */
-    int aligned[2], searched[2];  /* not numpy.isclose(0, start / length)    */
-    StepParams sp_align, sp_event, sp_stm;
-    EventParams ev;
-    unsigned long long flip_align, flip_stm;
+    CrossArgs x;
+    StepParams sp_stm;
+    unsigned long long flip_stm;
 };
 
 struct CorrLds {
@@ -186,8 +195,9 @@ __global__ void k_eval_solve2(int n, const double* __restrict__ J, const double*
  * optional alignment propagation to the window's start and an event search
  * over its length (singlehit/backend.py:202-280).  ok, t and (this lane's
  * component of) x_event as the batched path forms them.                   */
+template <bool CANCEL>
 __device__ __forceinline__ bool corr_residual(const Group6& ga, const Group6& ge,
-                                              const CorrArgs& a, double y, int& ok, double& t,
+                                              const CrossArgs& a, double y, int& ok, double& t,
                                               double& xe, int& steps, const int* cancel) {
     ok = 0;
     t = NAN;
@@ -197,14 +207,14 @@ __device__ __forceinline__ bool corr_residual(const Group6& ga, const Group6& ge
         double ya = y;
         if (a.aligned[w]) {
             int st, na, nr;
-            group6_run<false, true>(ga, y, 0.0, a.ts[w], a.sp_align, nullptr, 0, nullptr, ya, st,
-                                    na, nr, cancel);
+            group6_run<false, CANCEL>(ga, y, 0.0, a.ts[w], a.sp_align, nullptr, 0, nullptr, ya,
+                                      st, na, nr, cancel);
             steps += na;
             if (st == ST_CANCELLED) return false;
         }
         if (!a.searched[w]) continue;
         EventOut6 o;
-        group6_event<true>(ge, ya, 0.0, a.span[w], a.sp_event, a.ev, o, cancel);
+        group6_event<CANCEL>(ge, ya, 0.0, a.span[w], a.sp_event, a.ev, o, cancel);
         steps += o.nacc;
         if (o.status == ST_CANCELLED) return false;
         const double t_rel = o.found ? o.t_hit : a.span[w];
@@ -405,7 +415,7 @@ __global__ __launch_bounds__(192) void k_correct(CorrArgs a) {
     if (!L.fail) {
         const double mu = L.row.mu;
         Group6 ga, ge;
-        ga.setup(lane, mu, L.row.fwd, a.flip_align);
+        ga.setup(lane, mu, L.row.fwd, a.x.flip_align);
         ge.setup(lane, mu, 1, 0ULL);
         Wave42 W;
         W.setup(lane, mu, 1, a.flip_stm);
@@ -426,7 +436,8 @@ __global__ __launch_bounds__(192) void k_correct(CorrArgs a) {
                     }
                     int ok, steps;
                     double t, xe;
-                    const bool in = corr_residual(ga, ge, a, y, ok, t, xe, steps, &L.decided);
+                    const bool in = corr_residual<true>(ga, ge, a.x, y, ok, t, xe, steps,
+                                                        &L.decided);
                     if (in && ge.act) L.xe[j][ge.d] = xe;
                     if (ge.d == 0) {
                         atomicAdd(&L.steps, steps);
@@ -535,6 +546,39 @@ __global__ __launch_bounds__(192) void k_correct(CorrArgs a) {
         /* the result before its flag, for a host that polls mapped memory */
         __threadfence_system();
         __hip_atomic_store(&o.done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+
+/* _y_plane_crossing for a batch of states in one launch (the windows of
+ * poincare/singlehit/backend.py:237-280 without a time guess): one state
+ * per 8-lane group, the fallback window taken in the same group.  Outputs
+ * as section_crossing_batch builds them: ok, t = start + t_rel, and the
+ * crossing state ([6][n], structure of arrays).                           */
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_plane_crossing(int n, const double* __restrict__ mu,
+                                                          int fwd, const double* __restrict__ y0,
+                                                          CrossArgs x, int* __restrict__ ok,
+                                                          double* __restrict__ t_hit,
+                                                          double* __restrict__ y_hit,
+                                                          int* __restrict__ n_steps) {
+    const long long gtid = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    const int i = (int)(gtid >> 3);
+    if (i >= n) return;
+    const int lane = threadIdx.x & 63;
+    Group6 ga, ge;
+    ga.setup(lane, mu[i], fwd, x.flip_align);
+    ge.setup(lane, mu[i], 1, 0ULL);
+    const int d = ge.d;
+    const double y = ge.act ? y0[(size_t)d * n + i] : 0.0;
+    int okv, steps;
+    double tv, xv;
+    corr_residual<false>(ga, ge, x, y, okv, tv, xv, steps, nullptr);
+    if (ge.act) y_hit[(size_t)d * n + i] = xv;
+    if (d == 0) {
+        ok[i] = okv;
+        t_hit[i] = tv;
+        if (n_steps) n_steps[i] = steps;
     }
 }
 

@@ -1971,45 +1971,83 @@ int hh_dop853_event(int n, const double* mu, const double* t0, const double* tma
     return check_launch("hh_dop853_event launch");
 }
 
+/* the windows and integration settings of _y_plane_crossing's search */
+static int make_cross(CrossArgs& x, double window_start, double window_length,
+                      double fallback_start, double fallback_length, int ev_index,
+                      double ev_offset, int direction, double xtol, double gtol,
+                      const char* who) {
+    if (!(window_length >= 0.0) || !(fallback_length >= 0.0))
+        return fail(HH_ERR_ARGS, who);
+    x.ts[0] = window_start;
+    x.ts[1] = fallback_start;
+    x.span[0] = window_length;
+    x.span[1] = fallback_length;
+    for (int w = 0; w < 2; ++w) {
+        /* numpy.isclose(0, v): alignment and search are skipped for ~0 */
+        x.aligned[w] = !(fabs(0.0 - x.ts[w]) <= 1e-8 + 1e-5 * fabs(x.ts[w]));
+        x.searched[w] = !(fabs(0.0 - x.span[w]) <= 1e-8 + 1e-5 * fabs(x.span[w]));
+    }
+    /* the reference's settings: DOP853 rtol = atol = 1e-12 everywhere;
+     * max_step 1e4 for the alignment propagation (dynamics/base.py:447-450),
+     * unbounded in the event search (singlehit/backend.py:202); min_step
+     * 10 eps (integrators/rk.py:833-834)                                  */
+    const double min_step = 10.0 * DBL_EPSILON;
+    const long long max_steps = 1000000;
+    x.sp_align = make_sp(1e-12, 1e-12, 1e4, min_step, max_steps);
+    x.sp_event = make_sp(1e-12, 1e-12, INFINITY, min_step, max_steps);
+    x.ev.index = ev_index;
+    x.ev.offset = ev_offset;
+    x.ev.direction = direction;
+    x.ev.xtol = xtol;
+    x.ev.gtol = gtol;
+    x.flip_align = ~0ULL;                     /* backward alignment flips all six */
+    return HH_OK;
+}
+
 int hh_correct(int n, const hh_correction_t* problems, hh_correction_result_t* results,
                double window_start, double window_length, double fallback_start,
                double fallback_length, void* stream) {
     if (n < 0) return fail(HH_ERR_ARGS, "hh_correct: n < 0");
     if (n == 0) return HH_OK;
     if (!problems || !results) return fail(HH_ERR_ARGS, "hh_correct: null pointer");
-    if (!(window_length >= 0.0) || !(fallback_length >= 0.0))
-        return fail(HH_ERR_ARGS, "hh_correct: window lengths must be >= 0");
     CorrArgs a;
     a.n = n;
     a.prob = problems;
     a.res = results;
-    a.ts[0] = window_start;
-    a.ts[1] = fallback_start;
-    a.span[0] = window_length;
-    a.span[1] = fallback_length;
-    for (int w = 0; w < 2; ++w) {
-        /* numpy.isclose(0, v): alignment and search are skipped for ~0 */
-        a.aligned[w] = !(fabs(0.0 - a.ts[w]) <= 1e-8 + 1e-5 * fabs(a.ts[w]));
-        a.searched[w] = !(fabs(0.0 - a.span[w]) <= 1e-8 + 1e-5 * fabs(a.span[w]));
-    }
-    /* the reference's settings: DOP853 rtol = atol = 1e-12 everywhere;
-     * max_step 1e4 for propagations (dynamics/base.py:447-450), unbounded in
-     * the event search (singlehit/backend.py:202); min_step 10 eps
-     * (integrators/rk.py:833-834); x/g tolerances 1e-12 (utils.py:13-123)  */
-    const double min_step = 10.0 * DBL_EPSILON;
-    const long long max_steps = 1000000;
-    a.sp_align = make_sp(1e-12, 1e-12, 1e4, min_step, max_steps);
-    a.sp_event = make_sp(1e-12, 1e-12, INFINITY, min_step, max_steps);
-    a.sp_stm = make_sp(1e-12, 1e-12, 1e4, min_step, max_steps);
-    a.ev.index = 1;
-    a.ev.offset = 0.0;
-    a.ev.direction = 0;
-    a.ev.xtol = 1e-12;
-    a.ev.gtol = 1e-12;
-    a.flip_align = ~0ULL;                     /* backward alignment flips all six */
+    /* the residual's search: y = 0, either direction, x/g tolerances 1e-12
+     * (utils.py:13-123)                                                    */
+    const int rc = make_cross(a.x, window_start, window_length, fallback_start, fallback_length,
+                              1, 0.0, 0, 1e-12, 1e-12, "hh_correct: window lengths must be >= 0");
+    if (rc != HH_OK) return rc;
+    a.sp_stm = make_sp(1e-12, 1e-12, 1e4, 10.0 * DBL_EPSILON, 1000000);
     a.flip_stm = ((1ULL << 42) - 1) & ~((1ULL << 36) - 1);   /* the state block */
     hipLaunchKernelGGL(k_correct, dim3(n), dim3(192), 0, (hipStream_t)stream, a);
     return check_launch("hh_correct launch");
+}
+
+int hh_plane_crossing(int n, const double* mu, const double* y0, int forward, int ev_index,
+                      double ev_offset, int direction, double xtol, double gtol,
+                      double window_start, double window_length, double fallback_start,
+                      double fallback_length, int* ok, double* t_hit, double* y_hit,
+                      int* n_steps, void* stream) {
+    if (n < 0) return fail(HH_ERR_ARGS, "hh_plane_crossing: n < 0");
+    if (n == 0) return HH_OK;
+    if (!mu || !y0 || !ok || !t_hit || !y_hit)
+        return fail(HH_ERR_ARGS, "hh_plane_crossing: null pointer");
+    if (ev_index < 0 || ev_index > 5)
+        return fail(HH_ERR_ARGS, "hh_plane_crossing: ev_index must be 0..5");
+    if (direction < -1 || direction > 1)
+        return fail(HH_ERR_ARGS, "hh_plane_crossing: direction must be -1, 0 or 1");
+    if (forward != 1 && forward != -1)
+        return fail(HH_ERR_ARGS, "hh_plane_crossing: forward must be 1 or -1");
+    CrossArgs x;
+    const int rc = make_cross(x, window_start, window_length, fallback_start, fallback_length,
+                              ev_index, ev_offset, direction, xtol, gtol,
+                              "hh_plane_crossing: window lengths must be >= 0");
+    if (rc != HH_OK) return rc;
+    hipLaunchKernelGGL(k_plane_crossing<128>, dim3(blocks_for((long long)n * 8, 128)), dim3(128),
+                       0, (hipStream_t)stream, n, mu, forward, y0, x, ok, t_hit, y_hit, n_steps);
+    return check_launch("hh_plane_crossing launch");
 }
 
 int hh_correct_sizes(int* problem_bytes, int* result_bytes) {

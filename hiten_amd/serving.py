@@ -95,11 +95,6 @@ class Job:
         return self.finished - self.arrival
 
 
-def _isclose0(v: float) -> bool:
-    """numpy.isclose(0.0, v) for a scalar (atol 1e-8, rtol 1e-5)."""
-    return abs(0.0 - v) <= 1e-8 + 1e-5 * abs(v)
-
-
 class _Slot:
     """Reusable staging for one batch in flight: a stream, a completion event,
     pinned host and device buffers that grow to the largest batch seen.  A
@@ -271,71 +266,37 @@ class _FanLaunch(_Pending):
 
 class _EventBatch(_Pending):
     """The correction residuals of section_crossing_batch (t_guess None) for
-    bank rows waiting for event searches: the first window for every state,
-    then alignment and the fallback window for the states without a hit
-    (poincare/singlehit/backend.py:237-280)."""
+    bank rows waiting for event searches: both windows of
+    poincare/singlehit/backend.py:237-280 in one hh_plane_crossing launch,
+    one state per 8-lane group."""
 
     def __init__(self, bank: _Bank, rows: np.ndarray, forward: int, slot: _Slot):
         super().__init__(slot)
         self.bank, self.rows, self.forward = bank, rows, forward
-        self.states, counts = bank.event_inputs(rows)
-        self.mu = np.repeat(bank.mu[rows], counts)
-        (self.ts, self.span), (self.ts2, self.span2) = _default_windows()
-        n = self.states.shape[0]
-        self.ok = np.zeros(n, bool)
-        self.t = np.full(n, np.nan)
-        self.xe = np.full((n, 6), np.nan)
-        self.stage = 1
-        self._launch(np.arange(n), self.ts, self.span)
-
-    def _launch(self, idx, ts, span):
-        self.idx = idx
-        self.cur_ts, self.cur_span = ts, span
-        self.aligned = not _isclose0(ts)
-        self.searched = not _isclose0(span)
-        k = len(idx)
-        # din rows: mu, t0 (0), span, alignment time, state (6)
-        fin = np.empty((10, k))
-        fin[0], fin[1], fin[2], fin[3], fin[4:10] = self.mu[idx], 0.0, span, ts, self.states[idx].T
-        sl = self.slot
-        # fout rows: t_hit, y_hit (6), y_last (6), aligned state (6);
-        # iout rows: hit, status, n_accepted (event), status, n_accepted (alignment)
-        pk = self.pk = _Packed(fin, 19, 5, sl)
-        y_addr = pk.din(4)
-        if self.aligned:
-            _launch_final(pk, 6, (0, 1, 3), y_addr, pk.fout(13), sl.sh, forward=self.forward,
-                          flip_mask=batch.flip_mask(6, None), variant=_lib.HH_VARIANT_WIDE,
-                          max_step=1e4, istatus=3, inacc=4)
-            y_addr = pk.fout(13)
-        if self.searched:
-            _launch_event(pk, 0, sl.sh, ev_index=Y_INDEX, ev_offset=0.0, direction=0, xtol=1e-12,
-                          gtol=1e-12, rtol=1e-12, atol=1e-12, y_addr=y_addr,
-                          variant=_lib.HH_VARIANT_WIDE)
-        pk.download(slice(0, 13))
-        sl.record()
+        states, counts = bank.event_inputs(rows)
+        mu = np.repeat(bank.mu[rows], counts)
+        (ts, span), (ts2, span2) = _default_windows()
+        k = states.shape[0]
+        # din rows: mu, state (6); fout rows: t_event, x_event (6);
+        # iout rows: ok, accepted steps
+        fin = np.empty((7, k))
+        fin[0], fin[1:7] = mu, states.T
+        pk = self.pk = _Packed(fin, 7, 2, slot)
+        lib = _lib.load()
+        rc = lib.hh_plane_crossing(k, pk.din(0), pk.din(1), int(forward), Y_INDEX, 0.0, 0,
+                                   1e-12, 1e-12, ts, span, ts2, span2, pk.iout(0), pk.fout(0),
+                                   pk.fout(1), pk.iout(1), slot.sh)
+        _lib.check(rc, "hh_plane_crossing")
+        pk.download(slice(0, 7))
+        slot.record()
 
     def finish(self, server: "Server") -> bool:
-        """Consume the finished stage; returns True when the batch is complete."""
         f, i = self.pk.hf, self.pk.hi
-        if self.aligned:
-            server.steps += int(i[4].sum())
-        if self.searched:
-            server.steps += int(i[2].sum())
-            hit = i[0].astype(bool)
-            th, yh, yl = f[0], f[1:7].T, f[7:13].T
-            t_rel = np.where(hit, th, self.cur_span)
-            ok = hit & (t_rel < self.cur_span) & (t_rel >= 0.0)
-            y_out = np.where(hit[:, None], yh, yl)
-            sel = self.idx[ok]
-            self.ok[sel] = True
-            self.t[sel] = self.cur_ts + t_rel[ok]
-            self.xe[sel] = y_out[ok]
-        miss = np.nonzero(~self.ok)[0]
-        if self.stage == 1 and miss.size:
-            self.stage = 2
-            self._launch(miss, self.ts2, self.span2)
-            return False
-        self.bank.feed_events(self.rows, self.t, self.xe, self.ok)
+        server.steps += int(i[1].sum())
+        ok = i[0].astype(bool)
+        t = np.where(ok, f[0], np.nan)
+        xe = np.where(ok[:, None], f[1:7].T, np.nan)
+        self.bank.feed_events(self.rows, t, xe, ok)
         server._advance(self.bank, self.rows)
         return True
 

@@ -23,7 +23,7 @@ from typing import Optional
 import numpy as np
 import torch
 
-from . import batch
+from . import _lib, batch
 from .dynamics import _DirectedSystem, _propagate_dynsys
 from .integrators import EventConfig, PlaneEventFn, RungeKutta
 
@@ -217,6 +217,34 @@ def _event_window(mu, states, t_start, span, surface, forward, align_steps_kw=No
     return ok, t_start + t_rel, y_out
 
 
+def plane_crossing_device(mu, states, surface: _PlaneEvent, forward: int, window, fallback,
+                          stream=None):
+    """_event_window over ``window`` and then ``fallback`` for the states
+    without a valid hit, in one hh_plane_crossing launch.  Returns ok (n,),
+    t = start + t_rel (n,) and the crossing states (n, 6) as NumPy arrays
+    (the same bits as the two-launch form)."""
+    import torch
+    lib = _lib.load()
+    states = np.atleast_2d(np.asarray(states, dtype=float))
+    n = states.shape[0]
+    if n == 0:
+        return np.zeros(0, bool), np.zeros(0), np.zeros((0, 6))
+    dev = batch._device()
+    mu_d = torch.as_tensor(np.broadcast_to(np.asarray(mu, float), (n,)).copy(), device=dev)
+    y0 = torch.as_tensor(np.ascontiguousarray(states.T), device=dev)
+    ok = torch.empty(n, dtype=torch.int32, device=dev)
+    t = torch.empty(n, dtype=torch.float64, device=dev)
+    y = torch.empty((6, n), dtype=torch.float64, device=dev)
+    ev_dir = 0 if surface.direction is None else int(surface.direction)
+    _lib.check(lib.hh_plane_crossing(n, mu_d.data_ptr(), y0.data_ptr(), int(forward),
+                                     surface.index, surface.offset, ev_dir, 1e-12, 1e-12,
+                                     float(window[0]), float(window[1]), float(fallback[0]),
+                                     float(fallback[1]), ok.data_ptr(), t.data_ptr(),
+                                     y.data_ptr(), None, _lib.stream_handle(stream)),
+               "hh_plane_crossing")
+    return ok.cpu().numpy().astype(bool), t.cpu().numpy(), y.cpu().numpy().T.copy()
+
+
 def _default_windows(t_guess=None, t0_offset: float = 0.15):
     """(start, span) of the first search window of _SingleHitBackend._cross
     (backend.py:244-256) and of its fallback window (269-279)."""
@@ -242,6 +270,14 @@ def section_crossing_batch(dynsys, states, *, surface: _PlaneEvent | None = None
     states = np.atleast_2d(np.asarray(states, dtype=float))
     n = states.shape[0]
     (ts, span), (ts2, span2) = _default_windows(t_guess, t0_offset)
+    if t_guess is None:
+        # both windows in one launch, one state per 8-lane group
+        ok, t_hit, y_hit = plane_crossing_device(mu, states, surface, forward,
+                                                 (ts, span), (ts2, span2))
+        hits = [None] * n
+        for i in np.nonzero(ok)[0]:
+            hits[i] = _SectionHit(float(t_hit[i]), y_hit[i].copy(), y_hit[i, :2].copy(), int(i))
+        return hits
     ok, t_hit, y_hit = _event_window(mu, states, ts, span, surface, forward)
     hits: list = [None] * n
     for i in np.nonzero(ok)[0]:

@@ -340,6 +340,19 @@ typedef struct {
 int hh_correct(int n, const hh_correction_t* problems, hh_correction_result_t* results,
                double window_start, double window_length, double fallback_start,
                double fallback_length, void* stream);
+/* _y_plane_crossing (poincare/singlehit/backend.py:237-280, no time guess)
+ * for n states in one launch: the window (start, length), then the
+ * fallback window for the states without a valid hit, each an alignment
+ * propagation to its start (skipped where numpy.isclose(0, start); the
+ * alignment honours `forward`, the event search does not) and an event
+ * search for y[ev_index] = ev_offset in `direction`.  mu[n], y0[6][n] and
+ * the outputs ok[n], t_hit[n] (start + t_rel), y_hit[6][n] and n_steps[n]
+ * (accepted steps, may be NULL) are device arrays.                        */
+int hh_plane_crossing(int n, const double* mu, const double* y0, int forward, int ev_index,
+                      double ev_offset, int direction, double xtol, double gtol,
+                      double window_start, double window_length, double fallback_start,
+                      double fallback_length, int* ok, double* t_hit, double* y_hit,
+                      int* n_steps, void* stream);
 /* sizeof of the two records, for bindings that lay them out themselves     */
 int hh_correct_sizes(int* problem_bytes, int* result_bytes);
 /* Test entry: the device corrector's 2x2 step, delta[n][2] solving
