@@ -82,11 +82,25 @@ def cpu_baseline(w, n_threads: int, target_s: float = 12.0):
                       f"{steps} accepted steps in {spent:.2f} s"}
 
 
-def serving_run(seconds: float, rate: float = 2000.0):
+def serving_run(seconds: float, rate: float = 2000.0, continuation_fraction: float = 0.0):
     """The serving stream: Poisson arrivals at ``rate``, 70% corrections and
-    30% manifold fans of 32-2048 trajectories, through hiten_amd.serving."""
-    from hiten_amd import serving as S, workloads as W
-    reqs = W.serving_stream(duration=seconds, rate=rate)
+    30% manifold fans of 32-2048 trajectories, through hiten_amd.serving.
+    With ``continuation_fraction`` that share of the corrections becomes
+    continuation jobs instead (Halo families of 4 members, z step 1e-3,
+    retry window 2, seeded by orbits corrected before the clock starts)."""
+    import numpy as np
+    from hiten_amd import corrector as Cr, serving as S, workloads as W
+    kw = {}
+    if continuation_fraction > 0:
+        rng = np.random.default_rng(W.SEED + 8)
+        guesses = [Cr.halo_problem(W.MU_EM, [rng.uniform(0.80, 0.88), 0,
+                                             rng.uniform(-0.2, 0.2), 0,
+                                             rng.uniform(-0.3, 0.3), 0], max_attempts=20)
+                   for _ in range(64)]
+        seeds = [r.x_corrected for r in Cr.correct_device(guesses) if r.converged][:16]
+        kw = dict(correction_fraction=0.7 - continuation_fraction,
+                  continuation_fraction=continuation_fraction, continuation_seeds=seeds)
+    reqs = W.serving_stream(duration=seconds, rate=rate, **kw)
     server = S.Server()
     # warm the kernels and the pinned-memory pool on a few requests first
     server.run(reqs[:20], time_limit=60.0)
@@ -473,6 +487,8 @@ def main():
     }
     if args.serve_seconds > 0:
         out["serving"] = serving_run(args.serve_seconds)
+        out["serving_with_continuations"] = serving_run(args.serve_seconds,
+                                                        continuation_fraction=0.1)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(w, threads)

@@ -145,6 +145,18 @@ class _Family:
             stp = self.on_reject(stp)
         return out
 
+    @staticmethod
+    def decided(results) -> int:
+        """How many leading results of a window fix its outcome, or 0 while
+        that is open: up to the first converged one, else all of them
+        (``None`` marks a result still outstanding)."""
+        for k, r in enumerate(results):
+            if r is None:
+                return 0
+            if r[2]:
+                return k + 1
+        return len(results)
+
     def consume(self, window, results) -> None:
         """Replay the sequential loop over a window's corrections."""
         for (stp, prediction), res in zip(window, results):
@@ -181,10 +193,17 @@ class _Family:
             corrections=self.corrections)
 
 
+def result_tuple(r: "_corr.CorrectionResult", prediction) -> tuple:
+    """(corrected, residual, converged, aux) of one correction, as the
+    interface's _correct returns them (interfaces.py:135-175)."""
+    x = np.asarray(r.x_corrected, dtype=float)
+    return (x, float(np.linalg.norm(x - prediction)), bool(r.converged),
+            {"period": float(2.0 * r.half_period)})
+
+
 def _correct_all(problems, predictions, corrector):
-    """(corrected, residual, converged, aux) per prediction, as the
-    interface's _correct returns them (interfaces.py:135-175).  A callable
-    ``corrector`` is called on the list of predictions instead."""
+    """result_tuple per prediction.  A callable ``corrector`` is called on
+    the list of predictions instead."""
     if callable(corrector):
         return list(corrector(predictions))
     if corrector == "device":
@@ -193,12 +212,7 @@ def _correct_all(problems, predictions, corrector):
         res = _corr.correct_batch(problems)
     else:
         raise ValueError("corrector must be 'device' or 'bank'")
-    out = []
-    for r, pred in zip(res, predictions):
-        x = np.asarray(r.x_corrected, dtype=float)
-        out.append((x, float(np.linalg.norm(x - pred)), bool(r.converged),
-                    {"period": float(2.0 * r.half_period)}))
-    return out
+    return [result_tuple(r, pred) for r, pred in zip(res, predictions)]
 
 
 def continue_families(seeds: Sequence, make_problem: Callable[[np.ndarray], "_corr.CorrectionProblem"],

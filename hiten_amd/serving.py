@@ -1,7 +1,15 @@
-"""Continuous-batching request loop for corrections and manifold fans.
+"""Continuous-batching request loop for corrections, continuations and
+manifold fans.
 
-A request is either
-  * a single-shooting correction (corrector.CorrectionProblem), or
+A request is one of
+  * a single-shooting correction (corrector.CorrectionProblem),
+  * a continuation: one periodic-orbit family grown from a seed by the
+    predictor-corrector loop of continuation/backends/pc.py:42-166 (as
+    hiten_amd.continuation batches it: each round corrects a window of
+    retry predictions); the loop runs each round's corrections as
+    correction work of its own, beside every other request's, and starts
+    the next round as soon as the results in hand fix the window's
+    outcome, or
   * a manifold fan: N event searches (DOP853 until y[index] = offset in a
     direction, dense-output bisection), as _integrate_dop853_until_event
     (integrators/rk.py:2678-2803) runs them per trajectory.
@@ -35,16 +43,18 @@ from __future__ import annotations
 
 import collections
 import ctypes
+import functools
 import gc
 import math
 import time
 from dataclasses import dataclass, field
-from typing import List, Optional
+from typing import Callable, List, Optional
 
 import numpy as np
 import torch
 
 from . import _lib, batch
+from . import continuation as _cont
 from . import corrector as _corr
 from .corrector import CorrectionProblem, CorrectionResult, _Bank, halo_problem
 from .singlehit import _default_windows
@@ -72,6 +82,30 @@ class FanRequest:
 
 
 @dataclass
+class ContinuationRequest:
+    """One family continued from ``seed`` (a corrected orbit):
+    hiten_amd.continuation.continue_family's arguments; ``make_problem``
+    turns a prediction into its correction problem."""
+    seed: np.ndarray
+    make_problem: Callable
+    step: np.ndarray
+    max_members: int = 8
+    max_retries_per_step: int = 50
+    stepper: str = "natural"
+    state_indices: object = None
+    shrink_policy: Optional[Callable] = None
+    step_min: float = 1e-10
+    step_max: float = 1.0
+    window: int = 2
+
+    def family_kwargs(self) -> dict:
+        return dict(step=self.step, max_members=self.max_members,
+                    max_retries_per_step=self.max_retries_per_step, stepper=self.stepper,
+                    state_indices=self.state_indices, shrink_policy=self.shrink_policy,
+                    step_min=self.step_min, step_max=self.step_max)
+
+
+@dataclass
 class FanResult:
     hit: np.ndarray
     t_hit: np.ndarray
@@ -82,13 +116,14 @@ class FanResult:
 @dataclass
 class Job:
     id: int
-    kind: str                      # "correction" | "fan"
+    kind: str                      # "correction" | "continuation" | "fan"
     request: object
     arrival: float                 # seconds since the loop started
     nbytes: int = 0
     admitted: float = math.nan
     finished: float = math.nan
     result: object = None
+    parent: object = None          # (run, round, index) of a continuation's correction
 
     @property
     def latency(self) -> float:
@@ -446,6 +481,55 @@ class _CorrLaunch:
         return False
 
 
+class _ContRun:
+    """A continuation job between rounds: the family's loop state, the
+    window of predictions being corrected and the results that came back.
+    Results of a superseded round (corrections the outcome no longer
+    needed) are dropped."""
+
+    def __init__(self, job: Job):
+        r: ContinuationRequest = job.request
+        if r.stepper not in ("natural", "secant"):
+            raise ValueError("stepper must be 'natural' or 'secant'")
+        if r.window < 1:
+            raise ValueError("window must be >= 1")
+        self.job = job
+        self.fam = _cont._Family(r.seed, r.make_problem, **r.family_kwargs())
+        self.round = 0
+
+    def start_round(self, server: "Server") -> None:
+        self.round += 1
+        self.win = self.fam.window(int(self.job.request.window))
+        self.res = [None] * len(self.win)
+        self.fam.corrections += len(self.win)
+        for k, (_, pred) in enumerate(self.win):
+            sub = Job(-1, "correction", self.fam.make_problem(pred), server.now())
+            sub.admitted = sub.arrival
+            sub.parent = (self, self.round, k)
+            server._enqueue_correction(sub)
+
+    def on_result(self, rnd: int, k: int, result: CorrectionResult) -> bool:
+        """Store a result; False if its round is over (or the family is)."""
+        if rnd != self.round:
+            return False
+        self.res[k] = _cont.result_tuple(result, self.win[k][1])
+        return True
+
+    def advance(self, server: "Server") -> bool:
+        """Consume the window once its outcome is fixed and start the next
+        round; True once the family is complete."""
+        n = self.fam.decided(self.res)
+        if n == 0:
+            return False
+        self.fam.consume(self.win[:n], self.res[:n])
+        if self.fam.growing:
+            self.start_round(server)
+            return False
+        self.round = -1              # corrections still in flight are dropped
+        self.job.result = self.fam.result()
+        return True
+
+
 class Server:
     """Continuous-batching loop; see the module docstring.
 
@@ -539,6 +623,8 @@ class Server:
         self.inflight: List[_Pending] = []
         self.corr_waiting: list = []    # admitted corrections waiting for a device launch
         self.corr_inflight: List[_CorrLaunch] = []
+        self.cont_runs: List[_ContRun] = []     # admitted continuations not yet complete
+        self.cont_dirty: List[_ContRun] = []    # continuations with results to consume
         self.t0 = time.perf_counter()
         self.max_queue = 0
 
@@ -552,8 +638,15 @@ class Server:
         return n * (6 * 8 * 4 + 4 * 8 + 4 * 4)
 
     def submit(self, kind: str, request, arrival: Optional[float] = None) -> Job:
+        if kind not in ("correction", "continuation", "fan"):
+            raise ValueError(f"unknown request kind {kind!r}")
         job = Job(len(self.jobs), kind, request, self.now() if arrival is None else arrival)
-        job.nbytes = (self.fan_bytes(request.size) if kind == "fan" else self.CORRECTION_BYTES)
+        if kind == "fan":
+            job.nbytes = self.fan_bytes(request.size)
+        elif kind == "continuation":
+            job.nbytes = max(1, int(request.window)) * self.CORRECTION_BYTES
+        else:
+            job.nbytes = self.CORRECTION_BYTES
         if job.nbytes > self.hbm_budget:
             raise ValueError(f"job {job.id} needs {job.nbytes} B, more than the HBM budget")
         self.jobs.append(job)
@@ -564,6 +657,34 @@ class Server:
     def _complete(self, job: Job) -> None:
         job.finished = self.now()
         self.used -= job.nbytes
+        if job.parent is not None:
+            run, rnd, k = job.parent
+            if run.on_result(rnd, k, job.result) and run not in self.cont_dirty:
+                self.cont_dirty.append(run)
+
+    def _enqueue_correction(self, job: Job) -> None:
+        """An admitted correction: to the next device-resident launch, or
+        into the bank of its problem's shape."""
+        if self.corrector == "device" and _corr.device_supported(job.request):
+            self.corr_waiting.append(job)
+            return
+        p = job.request
+        key = (len(p.residual_indices), len(p.control_indices))
+        bank = self.banks.get(key)
+        if bank is None:
+            bank = self.banks[key] = _Bank(*key)
+        i = bank.add(p)
+        self.owner[(key, i)] = job
+        self.ready.setdefault(key, []).append(i)
+
+    def _advance_continuations(self) -> bool:
+        """Continuations with new results: next round, or done."""
+        runs, self.cont_dirty = self.cont_dirty, []
+        for run in runs:
+            if run.advance(self):
+                self.cont_runs.remove(run)
+                self._complete(run.job)
+        return bool(runs)
 
     def _advance(self, bank: _Bank, rows: np.ndarray) -> None:
         """Rows whose batch completed: finished jobs out, the rest ready."""
@@ -585,17 +706,16 @@ class Server:
             job.admitted = self.now()
             if job.kind == "fan":
                 self.inflight.append(_FanLaunch(job, self._slot()))
-            elif self.corrector == "device" and _corr.device_supported(job.request):
-                self.corr_waiting.append(job)
+            elif job.kind == "continuation":
+                run = _ContRun(job)
+                if run.fam.growing:
+                    self.cont_runs.append(run)
+                    run.start_round(self)
+                else:                       # max_members <= 1: the seed alone
+                    job.result = run.fam.result()
+                    self._complete(job)
             else:
-                p = job.request
-                key = (len(p.residual_indices), len(p.control_indices))
-                bank = self.banks.get(key)
-                if bank is None:
-                    bank = self.banks[key] = _Bank(*key)
-                i = bank.add(p)
-                self.owner[(key, i)] = job
-                self.ready.setdefault(key, []).append(i)
+                self._enqueue_correction(job)
         self.max_queue = max(self.max_queue, len(self.waiting))
 
     def _launch_ready(self) -> None:
@@ -649,7 +769,7 @@ class Server:
 
     def busy(self) -> bool:
         return bool(self.waiting or any(self.ready.values()) or self.inflight
-                    or self.corr_waiting or self.corr_inflight)
+                    or self.corr_waiting or self.corr_inflight or self.cont_runs)
 
     def step(self) -> bool:
         """One pass of the loop: admit, launch what is ready, collect what has
@@ -661,6 +781,9 @@ class Server:
         self._admit()
         self._launch_ready()
         moved = self._poll()
+        if self._advance_continuations():
+            self._launch_ready()
+            moved = True
         return moved or n_before != count()
 
     def run(self, stream_of_requests, time_limit: float = 120.0) -> List[Job]:
@@ -692,17 +815,28 @@ class Server:
         return self.jobs
 
 
+def serving_halo_problem(mu: float, x0) -> CorrectionProblem:
+    """The stream's correction: Halo defaults, at most 10 iterations, tol 1e-12."""
+    return halo_problem(mu, x0, max_attempts=10, tol=1e-12)
+
+
 def to_request(r: dict):
     """(kind, request) of a workloads.serving_stream entry."""
     if r["kind"] == "correction":
-        return "correction", halo_problem(r["mu"], r["x0"], max_attempts=10, tol=1e-12)
+        return "correction", serving_halo_problem(r["mu"], r["x0"])
+    if r["kind"] == "continuation":
+        return "continuation", ContinuationRequest(
+            np.asarray(r["seed"], float), functools.partial(serving_halo_problem, r["mu"]),
+            np.asarray(r["step"], float), max_members=int(r["members"]),
+            max_retries_per_step=int(r["max_retries"]), state_indices=list(r["state_indices"]),
+            window=int(r["window"]))
     return "fan", FanRequest(r["mu"], np.asarray(r["y0"], float), np.asarray(r["tmax"], float))
 
 
 def summarize(jobs: List[Job], wall: float, steps: int) -> dict:
     """Latency percentiles (ms) per kind and overall, and throughput."""
     out = {}
-    for kind in ("correction", "fan", None):
+    for kind in ("correction", "continuation", "fan", None):
         lat = np.array([j.latency for j in jobs if kind is None or j.kind == kind]) * 1e3
         if lat.size == 0:
             continue

@@ -120,29 +120,47 @@ def persistent_262k(n: int = 262144, seed: int = SEED + 3):
 
 def serving_stream(duration: float = 1.0, rate: float = 2000.0, seed: int = SEED + 4,
                    correction_fraction: float = 0.7, fan_min: int = 32, fan_max: int = 2048,
-                   eps: float = 1e-6):
+                   eps: float = 1e-6, continuation_fraction: float = 0.0,
+                   continuation_seeds=None, members: int = 4, window: int = 2,
+                   z_step: float = 1e-3):
     """Poisson request stream for the serving loop.  Draw order per request:
-    inter-arrival time ~ Exp(rate), kind (correction with probability
-    ``correction_fraction``), then
-      correction: x ~ U(0.80, 0.88), z ~ U(-0.2, 0.2), vy ~ U(-0.3, 0.3)
-                  (a Halo-style single-shooting problem, <= 10 iterations);
-      fan:        size ~ log-uniform [fan_min, fan_max], one planar seed, then
-                  size/2 Gaussian 6-directions (+ and - each, eps offset) and
-                  t_max ~ U(2, 8) per trajectory (stop at y = 0 with vy < 0).
-    Returns a list of dicts with an arrival time ``t`` in seconds."""
+    inter-arrival time ~ Exp(rate), u ~ U(0, 1) for the kind (correction if
+    u < ``correction_fraction``, continuation if below that plus
+    ``continuation_fraction``, else fan), then
+      correction:   x ~ U(0.80, 0.88), z ~ U(-0.2, 0.2), vy ~ U(-0.3, 0.3)
+                    (a Halo-style single-shooting problem, <= 10 iterations);
+      continuation: a seed orbit drawn from ``continuation_seeds`` (corrected
+                    Halo orbits), the sign of a natural-parameter step of
+                    ``z_step`` in z; ``members`` members, retry window
+                    ``window``, at most 4 retries per step;
+      fan:          size ~ log-uniform [fan_min, fan_max], one planar seed,
+                    then size/2 Gaussian 6-directions (+ and - each, eps
+                    offset) and t_max ~ U(2, 8) per trajectory (stop at
+                    y = 0 with vy < 0).
+    With no continuations the draws are those of the corrections-and-fans
+    stream.  Returns a list of dicts with an arrival time ``t`` in seconds."""
     rng = np.random.default_rng(seed)
+    if continuation_fraction > 0 and not continuation_seeds:
+        raise ValueError("continuations need seed orbits")
     out = []
     t = 0.0
     while True:
         t += rng.exponential(1.0 / rate)
         if t >= duration:
             break
-        if rng.uniform() < correction_fraction:
+        u = rng.uniform()
+        if u < correction_fraction:
             x = rng.uniform(0.80, 0.88)
             z = rng.uniform(-0.2, 0.2)
             vy = rng.uniform(-0.3, 0.3)
             out.append(dict(t=t, kind="correction", mu=MU_EM,
                             x0=np.array([x, 0.0, z, 0.0, vy, 0.0])))
+        elif u < correction_fraction + continuation_fraction:
+            s = np.asarray(continuation_seeds[int(rng.integers(len(continuation_seeds)))], float)
+            sign = 1.0 if rng.uniform() < 0.5 else -1.0
+            out.append(dict(t=t, kind="continuation", mu=MU_EM, seed=s.copy(),
+                            step=np.array([sign * z_step]), state_indices=[2],
+                            members=members, window=window, max_retries=4))
         else:
             size = int(round(math.exp(rng.uniform(math.log(fan_min), math.log(fan_max)))))
             size += size % 2

@@ -4,6 +4,9 @@
   Poisson process) gives the same bits as the same request run alone, both
   through the loop and through the direct calls (correct_device for the
   device-resident corrector, correct_batch for the host-driven bank).
+* Continuation jobs in the mix (Halo families grown from corrected seeds,
+  their rounds' corrections run among everyone else's) give the same
+  families as continue_family run alone with the same corrector.
 * A sample of results against the oracle (the reference's sequential
   Newton-Armijo loop, bit for bit for the bank and to the corrector's
   tolerance for the device corrector, whose Halo-term powers are correctly
@@ -18,7 +21,7 @@ import sys
 import numpy as np
 import pytest
 
-from hiten_amd import batch, corrector as Cr, serving as S, workloads as W
+from hiten_amd import batch, continuation as Ct, corrector as Cr, serving as S, workloads as W
 
 pytestmark = pytest.mark.gpu
 
@@ -137,3 +140,38 @@ def test_dedicated_streams_give_the_same_results(mixed_run):
             _same_correction(a.result, b.result)
         else:
             _same_fan(a.result, b.result)
+
+
+@pytest.mark.parametrize("kind", ["device", "bank"])
+def test_continuations_in_the_mix_equal_continue_family(gpu, kind):
+    guesses = [Cr.halo_problem(W.MU_EM, [0.8234, 0, 0.0224, 0, 0.1343, 0], max_attempts=20),
+               Cr.halo_problem(W.MU_EM, [0.84, 0, 0.1, 0, 0.22, 0], max_attempts=20)]
+    seeds = [r.x_corrected for r in Cr.correct_device(guesses)]
+    reqs = W.serving_stream(duration=0.04, rate=2000.0, fan_max=256, correction_fraction=0.55,
+                            continuation_fraction=0.15, continuation_seeds=seeds)
+    n_cont = sum(r["kind"] == "continuation" for r in reqs)
+    say(f"serving loop with {n_cont} continuations among {len(reqs)} requests ({kind})")
+    assert n_cont > 0
+    server = S.Server(corrector=kind)
+    jobs = server.run(reqs, time_limit=120.0)
+    server.close()
+    assert all(not math.isnan(j.finished) for j in jobs)
+    conts = [j for j in jobs if j.kind == "continuation"]
+    for j in conts:
+        r = j.request
+        alone = Ct.continue_family(r.seed, r.make_problem, window=r.window, corrector=kind,
+                                   **r.family_kwargs())
+        g = j.result
+        assert (g.accepted_count, g.rejected_count, g.iterations) == (
+            alone.accepted_count, alone.rejected_count, alone.iterations)
+        for a, b in zip(g.family, alone.family):
+            assert np.array_equal(a, b)
+        assert g.aux == alone.aux and np.array_equal(g.final_step, alone.final_step)
+    assert any(j.result.accepted_count == j.request.max_members for j in conts)
+    corr = [j for j in jobs if j.kind == "correction"]
+    probs = [j.request for j in corr]
+    direct = Cr.correct_device(probs) if kind == "device" else Cr.correct_batch(probs)
+    for j, d in zip(corr, direct):
+        _same_correction(j.result, d)
+    summ = S.summarize(jobs, 1.0, server.steps)
+    assert summ["continuation"]["n"] == len(conts)
