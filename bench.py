@@ -154,7 +154,7 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
         per = n * (w["steps"] - 1)
         fl = per * 1167
         res = {}
-        for variant, kern in (("plain", "k_rk8_fixed6"), ("wide", "k_rk8_fixed6_group")):
+        for variant, kern in (("plain", "k_rk_fixed6"), ("wide", "k_rk_fixed6_group")):
             fn = lambda: batch.rk8_fixed(mu_d, y0, t0_d, tf_d, w["steps"], variant=variant)
             wall, ev = _timed(fn, steps, warmup)
             res[kern] = dict(value=per * steps / wall, wall=wall, launch_s=sum(ev) / len(ev))

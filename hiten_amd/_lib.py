@@ -58,6 +58,7 @@ _SIGNATURES = {
     "hh_set_tuning": ([C.POINTER(Tuning)], _I),
     "hh_get_tuning": ([C.POINTER(Tuning)], _I),
     "hh_rk8_fixed": ([_I, _I, _P, _P, _P, _I, _P, _I, _ULL, _P, _P, _P, _I, _I, _P], _I),
+    "hh_rk_fixed": ([_I, _I, _I, _P, _P, _P, _I, _P, _I, _ULL, _P, _P, _P, _I, _I, _P], _I),
     "hh_dop853_final": ([_I, _I, _P, _P, _P, _D, _D, _D, _D, _LL, _I, _ULL, _P, _P, _P, _P, _P,
                          _I, _I, _I, _P, _LL, _P, _P], _I),
     "hh_dop853_final_workspace": ([_I, _I, _I], _LL),

@@ -112,10 +112,19 @@ class EventResult:
     n_rejected: torch.Tensor
 
 
-def rk8_fixed(mu, y0, t0, tf, steps: int, *, dim: int = 6, t_vals=None, forward: int = 1,
-              flip_indices=None, return_trajectory: bool = False, variant: str = "auto",
-              block: int = 0, stream=None):
-    """Fixed-step RK8 over numpy.linspace(t0, tf, steps), or over the explicit
+FIXED_ORDERS = (4, 6, 8)
+
+
+def rk8_fixed(mu, y0, t0, tf, steps: int, **kw):
+    """rk_fixed with the 13-stage RK8 tableau."""
+    return rk_fixed(mu, y0, t0, tf, steps, order=8, **kw)
+
+
+def rk_fixed(mu, y0, t0, tf, steps: int, *, order: int = 8, dim: int = 6, t_vals=None,
+             forward: int = 1, flip_indices=None, return_trajectory: bool = False,
+             variant: str = "auto", block: int = 0, stream=None):
+    """Fixed-step RK of the given order (4, 6 or 8: the tableaux of
+    rk.py:755-783) over numpy.linspace(t0, tf, steps), or over the explicit
     grid ``t_vals`` shared by all trajectories (rk.py:531-588).
 
     Returns the final states (n, dim); with ``return_trajectory`` also the
@@ -124,6 +133,8 @@ def rk8_fixed(mu, y0, t0, tf, steps: int, *, dim: int = 6, t_vals=None, forward:
     latency form), "plain" one per lane; "auto" takes "wide" up to 65,536
     trajectories.  Both give the same bits.
     """
+    if order not in FIXED_ORDERS:
+        raise ValueError("RK order must be 4, 6, or 8")
     lib = _lib.load()
     y = _soa(y0, dim)
     n = y.shape[1]
@@ -139,12 +150,12 @@ def rk8_fixed(mu, y0, t0, tf, steps: int, *, dim: int = 6, t_vals=None, forward:
     traj = torch.empty((steps, dim, n), dtype=torch.float64, device=y.device) if return_trajectory else None
     if variant == "auto":
         variant = "wide" if n <= 65536 else "plain"
-    rc = lib.hh_rk8_fixed(n, dim, _ptr(mu_t), _ptr(t0_t), _ptr(tf_t), int(steps), _ptr(tv),
-                          int(forward), flip_mask(dim, flip_indices), _ptr(y), _ptr(out),
-                          _ptr(traj), {"plain": _lib.HH_VARIANT_PLAIN,
-                                       "wide": _lib.HH_VARIANT_WIDE}[variant],
-                          int(block), _lib.stream_handle(stream))
-    _lib.check(rc, "hh_rk8_fixed")
+    rc = lib.hh_rk_fixed(int(order), n, dim, _ptr(mu_t), _ptr(t0_t), _ptr(tf_t), int(steps),
+                         _ptr(tv), int(forward), flip_mask(dim, flip_indices), _ptr(y), _ptr(out),
+                         _ptr(traj), {"plain": _lib.HH_VARIANT_PLAIN,
+                                      "wide": _lib.HH_VARIANT_WIDE}[variant],
+                         int(block), _lib.stream_handle(stream))
+    _lib.check(rc, "hh_rk_fixed")
     if return_trajectory:
         return out.t(), traj.permute(2, 0, 1)
     return out.t()

@@ -17,6 +17,10 @@ double that Python computed here.
   formula.  The reference (``integrators/coefficients/rk8.py:4-21``) writes each
   coefficient as ``p.0/q.0``; ``float(Fraction(p, q))`` is the same correctly
   rounded quotient.
+* The classical 4-stage RK4 tableau and the 7-stage tableau the reference
+  names RK6 (``integrators/coefficients/rk4.py``, ``rk6.py``: its rows and
+  weights are the Dormand-Prince 5(4) pair's 5th-order formula) are written
+  there as ``p.0/q.0`` quotients too, generated here from the same rationals.
 * pow tables: ``log(1 + i/128)`` and ``2**(j/128)`` as double-double pairs and
   the rounded reciprocals ``1/(1 + i/128)``, evaluated with mpmath at 60
   significant digits.
@@ -66,6 +70,34 @@ RK8_B = [F(14005451, 335480064), 0, 0, 0, 0, F(-59238493, 1068277825),
          F(1, 4)]
 RK8_C = [0, F(1, 18), F(1, 12), F(1, 8), F(5, 16), F(3, 8), F(59, 400), F(93, 200),
          F(5490023248, 9719169821), F(13, 20), F(1201146811, 1299019798), 1, 1]
+
+
+# classical RK4
+RK4_A_ROWS = [[], [F(1, 2)], [0, F(1, 2)], [0, 0, 1]]
+RK4_B = [F(1, 6), F(1, 3), F(1, 3), F(1, 6)]
+RK4_C = [0, F(1, 2), F(1, 2), 1]
+
+# Dormand-Prince 5(4): the seven stages and the 5th-order weights
+RK6_A_ROWS = [
+    [],
+    [F(1, 5)],
+    [F(3, 40), F(9, 40)],
+    [F(44, 45), F(-56, 15), F(32, 9)],
+    [F(19372, 6561), F(-25360, 2187), F(64448, 6561), F(-212, 729)],
+    [F(9017, 3168), F(-355, 33), F(46732, 5247), F(49, 176), F(-5103, 18656)],
+    [F(35, 384), 0, F(500, 1113), F(125, 192), F(-2187, 6784), F(11, 84)],
+]
+RK6_B = [F(35, 384), 0, F(500, 1113), F(125, 192), F(-2187, 6784), F(11, 84), 0]
+RK6_C = [0, F(1, 5), F(3, 10), F(4, 5), F(8, 9), 1, 1]
+
+
+def tableau_arrays(rows, b, c):
+    s = len(b)
+    A = np.zeros((s, s))
+    for i, row in enumerate(rows):
+        for j, v in enumerate(row):
+            A[i, j] = float(v)
+    return A, np.array([float(v) for v in b]), np.array([float(v) for v in c])
 
 
 def rk8_arrays():
@@ -140,7 +172,7 @@ def main():
     parts = []
     parts.append("""/* Generated by hiten_amd/csrc/gen_tables.py -- do not edit by hand.
  *
- * Runge-Kutta tableaux (RK8 13-stage fixed step, DOP853 adaptive with its
+ * Runge-Kutta tableaux (RK4, RK6 and RK8 fixed step, DOP853 adaptive with its
  * dense-output extension) and the lookup tables of the double-double pow used
  * by the step-size controller.  Included by both the HIP kernels and the CPU
  * oracle, so both integrate with bit-identical coefficients.
@@ -154,6 +186,13 @@ def main():
 #error "define HH_TABLE before including hh_tables.h"
 #endif
 """)
+    for name, (rows, b, c) in (("RK4", (RK4_A_ROWS, RK4_B, RK4_C)),
+                               ("RK6", (RK6_A_ROWS, RK6_B, RK6_C))):
+        A, B, C = tableau_arrays(rows, b, c)
+        parts.append(f"#define HH_{name}_STAGES {len(B)}\n")
+        parts.append(emit_array(f"HH_{name}_A", A))
+        parts.append(emit_array(f"HH_{name}_B", B))
+        parts.append(emit_array(f"HH_{name}_C", C))
     parts.append("#define HH_RK8_STAGES 13\n")
     parts.append(emit_array("HH_RK8_A", A8))
     parts.append(emit_array("HH_RK8_B", B8))

@@ -61,7 +61,6 @@ constexpr int DOP_S = 12;     /* stages of one DOP853 step                 */
 constexpr int DOP_K = 13;     /* stored stage derivatives (k[12] = f(y_high)) */
 constexpr int DOP_SX = 16;    /* stages of the dense-output extension       */
 constexpr int DOP_P = 7;      /* interpolant rows                           */
-constexpr int RK8_S = 13;
 
 constexpr int ST_OK = 0;
 constexpr int ST_MAXSTEPS = 1;
@@ -498,20 +497,40 @@ __device__ __forceinline__ double refine_bisect(double y_ev0, const double (&fc)
 }
 
 /* ---------------------------------------------------------------------- */
-/* fixed-step RK8 (rk.py:154-215)                                           */
+/* fixed-step explicit RK (rk.py:154-215) over the tableaux of rk.py:755-783 */
 /* ---------------------------------------------------------------------- */
-template <class P>
-__device__ __forceinline__ void rk8_step(const P& p, double (&y)[6], double h) {
-    double k[RK8_S][6];
+
+/* Tableau tags: stages and coefficients (integrators/coefficients/rk4.py,
+ * rk6.py, rk8.py).  After unrolling every coefficient is a constant, so the
+ * zero tests below vanish at compile time.                                 */
+struct RK4 {
+    static constexpr int S = HH_RK4_STAGES, ORDER = 4;
+    static __device__ __forceinline__ double a(int i, int j) { return HH_RK4_A[i][j]; }
+    static __device__ __forceinline__ double b(int j) { return HH_RK4_B[j]; }
+};
+struct RK6 {
+    static constexpr int S = HH_RK6_STAGES, ORDER = 6;
+    static __device__ __forceinline__ double a(int i, int j) { return HH_RK6_A[i][j]; }
+    static __device__ __forceinline__ double b(int j) { return HH_RK6_B[j]; }
+};
+struct RK8 {
+    static constexpr int S = HH_RK8_STAGES, ORDER = 8;
+    static __device__ __forceinline__ double a(int i, int j) { return HH_RK8_A[i][j]; }
+    static __device__ __forceinline__ double b(int j) { return HH_RK8_B[j]; }
+};
+
+template <class M, class P>
+__device__ __forceinline__ void rk_step(const P& p, double (&y)[6], double h) {
+    double k[M::S][6];
     p.rhs(y, k[0]);
 #pragma unroll
-    for (int i = 1; i < RK8_S; ++i) {
+    for (int i = 1; i < M::S; ++i) {
         double ys[6];
 #pragma unroll
         for (int d = 0; d < 6; ++d) ys[d] = y[d];
 #pragma unroll
         for (int j = 0; j < i; ++j) {
-            const double a = HH_RK8_A[i][j];
+            const double a = M::a(i, j);
             if (a != 0.0) {
                 const double ha = h * a;
 #pragma unroll
@@ -521,8 +540,8 @@ __device__ __forceinline__ void rk8_step(const P& p, double (&y)[6], double h) {
         p.rhs(ys, k[i]);
     }
 #pragma unroll
-    for (int j = 0; j < RK8_S; ++j) {
-        const double b = HH_RK8_B[j];
+    for (int j = 0; j < M::S; ++j) {
+        const double b = M::b(j);
         if (b != 0.0) {
             const double hb = h * b;
 #pragma unroll

@@ -213,16 +213,17 @@ struct Group6 {
     }
 };
 
-/* rk8_step (rk.py:154-215) for this lane's component */
-__device__ __forceinline__ void rk8_step_group(const Group6& p, double& y, double h) {
-    double k[RK8_S];
+/* rk_step (rk.py:154-215) for this lane's component */
+template <class M>
+__device__ __forceinline__ void rk_step_group(const Group6& p, double& y, double h) {
+    double k[M::S];
     k[0] = p.rhs(y);
 #pragma unroll
-    for (int i = 1; i < RK8_S; ++i) {
+    for (int i = 1; i < M::S; ++i) {
         double ys = y;
 #pragma unroll
         for (int j = 0; j < i; ++j) {
-            const double a = HH_RK8_A[i][j];
+            const double a = M::a(i, j);
             if (a != 0.0) {
                 const double ha = h * a;
                 ys = ys + ha * k[j];
@@ -231,8 +232,8 @@ __device__ __forceinline__ void rk8_step_group(const Group6& p, double& y, doubl
         k[i] = p.rhs(ys);
     }
 #pragma unroll
-    for (int j = 0; j < RK8_S; ++j) {
-        const double b = HH_RK8_B[j];
+    for (int j = 0; j < M::S; ++j) {
+        const double b = M::b(j);
         if (b != 0.0) {
             const double hb = h * b;
             y = y + hb * k[j];

@@ -1,8 +1,10 @@
 /* hh_kernels.hip -- HIP kernels and the C-ABI of the CR3BP propagation path.
  *
  * Kernels (gfx950, wave64):
- *   k_rk8_fixed6        fixed-step RK8, 6-d, one trajectory per lane
- *                       (integrators/rk.py:531-588 over numpy.linspace grid)
+ *   k_rk_fixed6<M>      fixed-step RK of tableau M (RK4, RK6, RK8), 6-d, one
+ *                       trajectory per lane (integrators/rk.py:531-588 over
+ *                       numpy.linspace grid); k_rk_fixed6_group per 8-lane
+ *                       group, k_rk_fixed42 on the 42-d variational system
  *   k_dop853_final6     DOP853 to t_f, 6-d, one trajectory per lane; the
  *                       returned state is the dense-output value at t_f
  *                       (rk.py:2375-2549 with t_eval = [t0, tf])
@@ -239,8 +241,8 @@ __device__ __forceinline__ double grid_time(const double* t_vals, int s, int ste
 
 /* rk.py:531-588 with h = t[i+1] - t[i].  A span that numpy.isclose calls zero
  * keeps y0 at every point (dynamics/base.py:421-424).                       */
-template <class P>
-__device__ __forceinline__ void rk8_run(const P& p, double (&y)[6], const double* t_vals,
+template <class M, class P>
+__device__ __forceinline__ void rk_run(const P& p, double (&y)[6], const double* t_vals,
                                         int steps, double t0, double tf, double* traj_base,
                                         size_t traj_stride, int blk) {
     const double delta = tf - t0;
@@ -262,20 +264,22 @@ __device__ __forceinline__ void rk8_run(const P& p, double (&y)[6], const double
     double t_prev = grid_time(t_vals, 0, steps, t0, tf, step, delta, zstep);
     for (int s = 1; s < steps; ++s) {
         const double t_s = grid_time(t_vals, s, steps, t0, tf, step, delta, zstep);
-        if (!constant) rk8_step(p, y, t_s - t_prev);
+        if (!constant) rk_step<M>(p, y, t_s - t_prev);
         t_prev = t_s;
         store(s);
     }
 }
 
-__global__ __launch_bounds__(256) void k_rk8_fixed6(int n, const double* __restrict__ mu,
-                                                    const double* __restrict__ t0a,
-                                                    const double* __restrict__ tfa, int steps,
-                                                    const double* __restrict__ t_vals, int fwd,
-                                                    unsigned long long flip,
-                                                    const double* __restrict__ y0,
-                                                    double* __restrict__ yout,
-                                                    double* __restrict__ traj) {
+/* fixed-step RK of tableau M, 6-d, one trajectory per lane */
+template <class M>
+__global__ __launch_bounds__(256) void k_rk_fixed6(int n, const double* __restrict__ mu,
+                                                   const double* __restrict__ t0a,
+                                                   const double* __restrict__ tfa, int steps,
+                                                   const double* __restrict__ t_vals, int fwd,
+                                                   unsigned long long flip,
+                                                   const double* __restrict__ y0,
+                                                   double* __restrict__ yout,
+                                                   double* __restrict__ traj) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Lane6 p;
@@ -285,14 +289,15 @@ __global__ __launch_bounds__(256) void k_rk8_fixed6(int n, const double* __restr
     double y[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) y[d] = y0[(size_t)d * n + i];
-    rk8_run(p, y, t_vals, steps, t0a[i], tfa[i], traj ? traj + i : nullptr, (size_t)n, 0);
+    rk_run<M>(p, y, t_vals, steps, t0a[i], tfa[i], traj ? traj + i : nullptr, (size_t)n, 0);
 #pragma unroll
     for (int d = 0; d < 6; ++d) yout[(size_t)d * n + i] = y[d];
 }
 
-/* 42-d fixed-step RK8, one trajectory per 8-lane group; traj: [steps][42][n] */
+/* 42-d fixed-step RK, one trajectory per 8-lane group; traj: [steps][42][n] */
+template <class M>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
-void k_rk8_fixed42(int n, const double* __restrict__ mu, const double* __restrict__ t0a,
+void k_rk_fixed42(int n, const double* __restrict__ mu, const double* __restrict__ t0a,
                    const double* __restrict__ tfa, int steps, const double* __restrict__ t_vals,
                    int fwd, unsigned long long flip, const double* __restrict__ y0,
                    double* __restrict__ yout, double* __restrict__ traj) {
@@ -309,17 +314,17 @@ void k_rk8_fixed42(int n, const double* __restrict__ mu, const double* __restric
     double y[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) y[d] = (b < 7) ? y0[(size_t)comp42(b, d) * n + i] : 0.0;
-    rk8_run(p, y, t_vals, steps, t0a[i], tfa[i], traj ? traj + i : nullptr, (size_t)n, b);
+    rk_run<M>(p, y, t_vals, steps, t0a[i], tfa[i], traj ? traj + i : nullptr, (size_t)n, b);
     if (b < 7) {
 #pragma unroll
         for (int d = 0; d < 6; ++d) yout[(size_t)comp42(b, d) * n + i] = y[d];
     }
 }
 
-/* k_rk8_fixed6 with one trajectory per aligned 8-lane group, one component
- * per lane (Group6): rk8_run's grid, steps and stores, per component.       */
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_rk8_fixed6_group(
+/* k_rk_fixed6 with one trajectory per aligned 8-lane group, one component
+ * per lane (Group6): rk_run's grid, steps and stores, per component.        */
+template <class M, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_rk_fixed6_group(
     int n, const double* __restrict__ mu, const double* __restrict__ t0a,
     const double* __restrict__ tfa, int steps, const double* __restrict__ t_vals, int fwd,
     unsigned long long flip, const double* __restrict__ y0, double* __restrict__ yout,
@@ -340,7 +345,7 @@ __global__ __launch_bounds__(BLOCK) void k_rk8_fixed6_group(
     double t_prev = grid_time(t_vals, 0, steps, t0, tf, step, delta, zstep);
     for (int s_idx = 1; s_idx < steps; ++s_idx) {
         const double t_s = grid_time(t_vals, s_idx, steps, t0, tf, step, delta, zstep);
-        if (!constant) rk8_step_group(p, y, t_s - t_prev);
+        if (!constant) rk_step_group<M>(p, y, t_s - t_prev);
         t_prev = t_s;
         if (traj && p.act) traj[((size_t)s_idx * 6 + d) * n + i] = y;
     }
@@ -1645,6 +1650,29 @@ StepParams make_sp(double rtol, double atol, double max_step, double min_step,
 /* the device-resident corrector, on the loops above */
 #include "hh_correct.h"
 
+template <class M>
+void launch_rk_fixed(int n, int dim, const double* mu, const double* t0p,
+                            const double* tfp, int steps, const double* t_vals, int fwd,
+                            unsigned long long flip_mask, const double* y0, double* y_out,
+                            double* traj_out, int variant, int block, hipStream_t st) {
+    if (dim == 6 && variant == HH_VARIANT_WIDE) {
+        const int g = blocks_for((long long)n * 8, block);
+        switch (block) {
+            case 64: hipLaunchKernelGGL((k_rk_fixed6_group<M, 64>), dim3(g), dim3(64), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out); break;
+            case 128: hipLaunchKernelGGL((k_rk_fixed6_group<M, 128>), dim3(g), dim3(128), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out); break;
+            case 192: hipLaunchKernelGGL((k_rk_fixed6_group<M, 192>), dim3(g), dim3(192), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out); break;
+            default: hipLaunchKernelGGL((k_rk_fixed6_group<M, 256>), dim3(g), dim3(256), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out); break;
+        }
+    } else if (dim == 6) {
+        hipLaunchKernelGGL(k_rk_fixed6<M>, dim3(blocks_for(n, block)), dim3(block), 0, st, n, mu,
+                           t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out);
+    } else {
+        hipLaunchKernelGGL(k_rk_fixed42<M>, dim3(blocks_for((long long)n * 8, block)),
+                           dim3(block), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask,
+                           y0, y_out, traj_out);
+    }
+}
+
 }  // namespace
 
 /* ======================================================================== */
@@ -1689,40 +1717,43 @@ int hh_get_tuning(hh_tuning_t* t) {
     return HH_OK;
 }
 
+int hh_rk_fixed(int order, int n, int dim, const double* mu, const double* t0, const double* tf,
+                int steps, const double* t_vals, int fwd, unsigned long long flip_mask,
+                const double* y0, double* y_out, double* traj_out, int variant, int block,
+                void* stream) {
+    if (order != 4 && order != 6 && order != 8)
+        return fail(HH_ERR_ARGS, "hh_rk_fixed: order must be 4, 6 or 8");
+    if (n < 0) return fail(HH_ERR_ARGS, "hh_rk_fixed: n < 0");
+    if (n == 0) return HH_OK;
+    if (dim != 6 && dim != 42) return fail(HH_ERR_ARGS, "hh_rk_fixed: dim must be 6 or 42");
+    if (steps < 2) return fail(HH_ERR_ARGS, "hh_rk_fixed: steps must be >= 2");
+    if (!mu || !y0 || !y_out || (!t_vals && (!t0 || !tf)))
+        return fail(HH_ERR_ARGS, "hh_rk_fixed: null pointer");
+    if (variant != HH_VARIANT_PLAIN && variant != HH_VARIANT_WIDE)
+        return fail(HH_ERR_ARGS, "hh_rk_fixed: variant must be PLAIN or WIDE");
+    if (block <= 0) block = dim == 6 ? 128 : 256;
+    if (block > 256 || block % 64)
+        return fail(HH_ERR_ARGS, "hh_rk_fixed: block must be 64..256, multiple of 64");
+    const double* t0p = t0 ? t0 : mu;   /* unused when t_vals is given */
+    const double* tfp = tf ? tf : mu;
+    hipStream_t st = (hipStream_t)stream;
+    if (order == 4)
+        launch_rk_fixed<RK4>(n, dim, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out,
+                             traj_out, variant, block, st);
+    else if (order == 6)
+        launch_rk_fixed<RK6>(n, dim, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out,
+                             traj_out, variant, block, st);
+    else
+        launch_rk_fixed<RK8>(n, dim, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out,
+                             traj_out, variant, block, st);
+    return check_launch("hh_rk_fixed launch");
+}
+
 int hh_rk8_fixed(int n, int dim, const double* mu, const double* t0, const double* tf, int steps,
                  const double* t_vals, int fwd, unsigned long long flip_mask, const double* y0,
                  double* y_out, double* traj_out, int variant, int block, void* stream) {
-    if (n < 0) return fail(HH_ERR_ARGS, "hh_rk8_fixed: n < 0");
-    if (n == 0) return HH_OK;
-    if (dim != 6 && dim != 42) return fail(HH_ERR_ARGS, "hh_rk8_fixed: dim must be 6 or 42");
-    if (steps < 2) return fail(HH_ERR_ARGS, "hh_rk8_fixed: steps must be >= 2");
-    if (!mu || !y0 || !y_out || (!t_vals && (!t0 || !tf)))
-        return fail(HH_ERR_ARGS, "hh_rk8_fixed: null pointer");
-    if (block <= 0) block = dim == 6 ? 128 : 256;
-    if (block > 256 || block % 64) return fail(HH_ERR_ARGS, "hh_rk8_fixed: block must be 64..256, multiple of 64");
-    const double* t0p = t0 ? t0 : mu;   /* unused when t_vals is given */
-    const double* tfp = tf ? tf : mu;
-    if (dim == 6 && variant == HH_VARIANT_WIDE) {
-        const int g = blocks_for((long long)n * 8, block);
-        hipStream_t st = (hipStream_t)stream;
-        switch (block) {
-            case 64: hipLaunchKernelGGL(k_rk8_fixed6_group<64>, dim3(g), dim3(64), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out); break;
-            case 128: hipLaunchKernelGGL(k_rk8_fixed6_group<128>, dim3(g), dim3(128), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out); break;
-            case 192: hipLaunchKernelGGL(k_rk8_fixed6_group<192>, dim3(g), dim3(192), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out); break;
-            default: hipLaunchKernelGGL(k_rk8_fixed6_group<256>, dim3(g), dim3(256), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out); break;
-        }
-    } else if (variant != HH_VARIANT_PLAIN && variant != HH_VARIANT_WIDE) {
-        return fail(HH_ERR_ARGS, "hh_rk8_fixed: variant must be PLAIN or WIDE");
-    } else if (dim == 6) {
-        hipLaunchKernelGGL(k_rk8_fixed6, dim3(blocks_for(n, block)), dim3(block), 0,
-                           (hipStream_t)stream, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0,
-                           y_out, traj_out);
-    } else {
-        hipLaunchKernelGGL(k_rk8_fixed42, dim3(blocks_for((long long)n * 8, block)), dim3(block),
-                           0, (hipStream_t)stream, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask,
-                           y0, y_out, traj_out);
-    }
-    return check_launch("hh_rk8_fixed launch");
+    return hh_rk_fixed(8, n, dim, mu, t0, tf, steps, t_vals, fwd, flip_mask, y0, y_out,
+                       traj_out, variant, block, stream);
 }
 
 /* Resident workgroups of a kernel on the current device (CU count times the

@@ -224,8 +224,9 @@ def _compute_monodromy(dynsys, x0, period):
 def propagate_batch(mu, states0, t0, tf, *, forward: int = 1, steps: int = 2,
                     method: str = "adaptive", flip_indices=None, rtol: float = 1e-12,
                     atol: float = 1e-12, max_step: float = 1e4, dim: int = 6, variant: str = "auto",
-                    max_steps: int = batch.DEFAULT_MAX_STEPS):
-    """_propagate_dynsys for many trajectories in one launch.
+                    max_steps: int = batch.DEFAULT_MAX_STEPS, order: int = 8):
+    """_propagate_dynsys for many trajectories in one launch (``order`` 4, 6
+    or 8 with method "fixed"; DOP853 with method "adaptive").
 
     ``steps == 2`` returns the final states (n, dim) (the dense value at tf);
     otherwise the (n, steps, dim) grid.  ``t0``/``tf``/``mu`` may be scalars or
@@ -236,12 +237,15 @@ def propagate_batch(mu, states0, t0, tf, *, forward: int = 1, steps: int = 2,
     fixed_variant = variant if variant in ("plain", "wide") else "auto"
     if method == "fixed":
         if steps == 2:
-            return batch.rk8_fixed(mu, states0, t0, tf, steps, dim=dim, forward=forward,
-                                   flip_indices=flip_indices, variant=fixed_variant)
-        _, traj = batch.rk8_fixed(mu, states0, t0, tf, steps, dim=dim, forward=forward,
-                                  flip_indices=flip_indices, return_trajectory=True,
+            return batch.rk_fixed(mu, states0, t0, tf, steps, order=order, dim=dim,
+                                  forward=forward, flip_indices=flip_indices,
                                   variant=fixed_variant)
+        _, traj = batch.rk_fixed(mu, states0, t0, tf, steps, order=order, dim=dim,
+                                 forward=forward, flip_indices=flip_indices,
+                                 return_trajectory=True, variant=fixed_variant)
         return traj
+    if order not in (8, 853):
+        raise NotImplementedError("adaptive propagation runs DOP853 (order 8) on the GPU")
     if steps == 2:
         return batch.dop853_final(mu, states0, t0, tf, dim=dim, rtol=rtol, atol=atol,
                                   max_step=max_step, forward=forward, flip_indices=flip_indices,

@@ -208,15 +208,30 @@ class _FixedStepRK(_Integrator):
         if event_fn is not None:
             raise NotImplementedError("fixed-step event search runs through DOP853 on the GPU; "
                                       "use AdaptiveRK(order=8)")
-        fin, traj = batch.rk8_fixed(mu, y0[None, :], None, None, 0, dim=dim, t_vals=t_vals,
-                                    forward=fwd, flip_indices=flip, return_trajectory=True)
+        fin, traj = batch.rk_fixed(mu, y0[None, :], None, None, 0, order=self._p, dim=dim,
+                                   t_vals=t_vals, forward=fwd, flip_indices=flip,
+                                   return_trajectory=True)
         states = traj[0].cpu().numpy()
         derivs = _derivatives(system, mu, dim, fwd, flip, states)
         return _Solution(times=t_vals.copy(), states=states, derivatives=derivs)
 
 
+class _RK4(_FixedStepRK):
+    """rk.py:755-763"""
+    name = "_RK4"
+    _p = 4
+
+
+class _RK6(_FixedStepRK):
+    """rk.py:766-773"""
+    name = "_RK6"
+    _p = 6
+
+
 class _RK8(_FixedStepRK):
+    """rk.py:776-783"""
     name = "_RK8"
+    _p = 8
 
 
 class _AdaptiveStepRK(_Integrator):
@@ -285,31 +300,35 @@ class _DOP853(_AdaptiveStepRK):
 
 
 class FixedRK:
-    """rk.py:2871-2906 (order 8 only: the configurations use RK8)."""
-    _map = {8: _RK8}
+    """rk.py:2871-2906: orders 4, 6 and 8 (default 4)."""
+    _map = {4: _RK4, 6: _RK6, 8: _RK8}
 
-    def __new__(cls, order=8, **opts):
+    def __new__(cls, order=4, **opts):
         if order not in cls._map:
-            raise NotImplementedError("only the 8th-order fixed-step method runs on the GPU")
+            raise ValueError("RK order must be 4, 6, or 8")
         return cls._map[order](**opts)
 
 
 class AdaptiveRK:
-    """rk.py:2909-2943 (order 8: DOP853)."""
+    """rk.py:2909-2943: order 8 is DOP853.  Order 5 (the reference's default,
+    Dormand-Prince 5(4)) has no GPU kernel and raises NotImplementedError."""
     _map = {8: _DOP853}
 
-    def __new__(cls, order=8, **opts):
+    def __new__(cls, order=5, **opts):
+        if order == 5:
+            raise NotImplementedError("adaptive RK45 has no GPU kernel; use AdaptiveRK(order=8) "
+                                      "(DOP853)")
         if order not in cls._map:
-            raise NotImplementedError("only DOP853 (order 8) runs on the GPU")
+            raise ValueError("Adaptive RK order not supported")
         return cls._map[order](**opts)
 
 
 class RungeKutta:
-    """rk.py:2945-2979: 8 -> fixed RK8, 853 -> DOP853."""
+    """rk.py:2945-2979: 4, 6, 8 -> FixedRK; 45 -> AdaptiveRK(5); 853 -> DOP853."""
+    _map = {4: FixedRK, 45: AdaptiveRK, 6: FixedRK, 8: FixedRK, 853: AdaptiveRK}
 
-    def __new__(cls, order=8, **opts):
-        if order == 853:
-            return AdaptiveRK(order=8, **opts)
-        if order == 8:
-            return FixedRK(order=8, **opts)
-        raise NotImplementedError("hiten_amd implements Runge-Kutta orders 8 and 853 only")
+    def __new__(cls, order=4, **opts):
+        if order not in cls._map:
+            raise ValueError("Runge-Kutta order not supported")
+        sub = {45: 5, 853: 8}.get(order, order)
+        return cls._map[order](order=sub, **opts)

@@ -13,7 +13,7 @@ manifold propagated in ONE launch:
    (_compute_manifold_section, manifold.py:470-526; _totime 528-560).
 4. All seeds propagated together on numpy.linspace(0, tf, steps) with
    flip_indices = slice(0, 6) (manifold.py:392-409): one 6-d DOP853 grid
-   launch (or one RK8 launch for method="fixed").
+   launch (or one fixed-step RK launch of the given order for method="fixed").
 5. The proximity and Jacobi-drift filters (manifold.py:411-432) for all arcs
    in one ``hh_traj_filters`` launch.
 
@@ -239,10 +239,10 @@ def compute_manifold(mu: float, orbit_state, period: float, *, stable: bool = Tr
     as manifold.py:340-345 forms them, ``distance * 1e3`` included.
     """
     if method not in ("fixed", "adaptive"):
-        raise NotImplementedError(f"method {method!r}: only 'fixed' (RK8) and 'adaptive' "
+        raise NotImplementedError(f"method {method!r}: only 'fixed' (RK4/6/8) and 'adaptive' "
                                   "(DOP853) integrate the CR3BP here")
-    if method == "fixed" and order != 8:
-        raise NotImplementedError("fixed-step integration is RK8 only")
+    if method == "fixed" and order not in batch.FIXED_ORDERS:
+        raise ValueError("RK order must be 4, 6, or 8")
     if method == "adaptive" and order not in (8, 853):
         raise NotImplementedError("adaptive integration is DOP853 only")
     stable_i = 1 if stable else -1
@@ -278,8 +278,9 @@ def compute_manifold(mu: float, orbit_state, period: float, *, stable: bool = Tr
         states = batch.dop853_grid(mu, seeds, 0.0, tf, steps, dim=6, forward=forward,
                                    flip_indices=slice(0, 6)).states
     else:
-        _, states = batch.rk8_fixed(mu, seeds, 0.0, tf, steps, dim=6, forward=forward,
-                                    flip_indices=slice(0, 6), return_trajectory=True)
+        _, states = batch.rk_fixed(mu, seeds, 0.0, tf, steps, order=order, dim=6,
+                                   forward=forward, flip_indices=slice(0, 6),
+                                   return_trajectory=True)
     r1, r2, err = traj_filters(states, mu)
     r1, r2, err = r1.cpu().numpy(), r2.cpu().numpy(), err.cpu().numpy()
     times = forward * np.linspace(0.0, tf, steps)

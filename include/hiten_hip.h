@@ -101,6 +101,18 @@ int hh_rk8_fixed(int n, int dim, const double* mu, const double* t0, const doubl
                  const double* t_vals, int fwd, unsigned long long flip_mask, const double* y0,
                  double* y_out, double* traj_out, int variant, int block, void* stream);
 
+/* hh_rk8_fixed for the tableau of the given order: 4 (classical RK4), 6
+ * (the 7-stage tableau the reference names RK6) or 8.
+ * Replaces: _RK4 / _RK6 / _RK8 (integrators/rk.py:755-783, tableaux
+ * integrators/coefficients/rk4.py, rk6.py, rk8.py) through
+ * _FixedStepRK._integrate_fixed_rk (rk.py:531-588), i.e.
+ * _propagate_dynsys(..., method="fixed", order=4|6|8).  Other arguments as
+ * hh_rk8_fixed.                                                            */
+int hh_rk_fixed(int order, int n, int dim, const double* mu, const double* t0, const double* tf,
+                int steps, const double* t_vals, int fwd, unsigned long long flip_mask,
+                const double* y0, double* y_out, double* traj_out, int variant, int block,
+                void* stream);
+
 /* Adaptive DOP853 from t0[i] to tf[i]; y_out is the dense-output value at
  * tf[i], i.e. the last row of _propagate_dynsys(..., steps=2,
  * method="adaptive") and the final Phi of _compute_stm.

@@ -15,6 +15,10 @@
  *   or_pow15/25/pow     bitwise vs C library pow on random inputs; mpmath.
  *   or_rk_fixed         RK8 order conditions; 8th-order convergence; mu=0
  *                       circular-orbit closed form; linear-ODE exactness.
+ *   or_rk_fixed_order   RK4 / RK6 tableaux: their order conditions (RK6 is
+ *                       the Dormand-Prince 5th-order formula, so order 5);
+ *                       measured convergence orders 4 and 5; RK4 on a
+ *                       linear ODE against the closed-form step polynomial.
  *   or_dop853           one step vs SciPy's DOP853 step (same tableau);
  *                       dense output vs the exact solution of linear ODEs;
  *                       harmonic oscillator vs closed form; the reference's
@@ -260,25 +264,42 @@ void or_linspace(double t0, double tf, int n, double *out) {
 /* fixed-step RK: rk.py:154-215 (step), 531-588 (driver)                   */
 /* ---------------------------------------------------------------------- */
 
-static void rk8_step(or_sys *s, double t, const double *y, double h, double *y_high,
-                     double (*k)[DIM_MAX]) {
+/* the fixed-step tableaux of rk.py:755-783 (coefficients/rk4.py, rk6.py,
+ * rk8.py): stages, A (row-major s x s), B, C                               */
+typedef struct {
+    int s;
+    const double *a, *b, *c;
+} rk_tab;
+
+static const rk_tab RK_TAB4 = {HH_RK4_STAGES, &HH_RK4_A[0][0], HH_RK4_B, HH_RK4_C};
+static const rk_tab RK_TAB6 = {HH_RK6_STAGES, &HH_RK6_A[0][0], HH_RK6_B, HH_RK6_C};
+static const rk_tab RK_TAB8 = {HH_RK8_STAGES, &HH_RK8_A[0][0], HH_RK8_B, HH_RK8_C};
+
+static const rk_tab *rk_tableau(int order) {
+    return order == 4 ? &RK_TAB4 : order == 6 ? &RK_TAB6 : order == 8 ? &RK_TAB8 : NULL;
+}
+
+/* rk.py:154-215: y_stage += (h * a_ij) * k[j] for nonzero a_ij, then the
+ * high-order combination the same way */
+static void rk_step(const rk_tab *tb, or_sys *s, double t, const double *y, double h,
+                    double *y_high, double (*k)[DIM_MAX]) {
     int n = s->dim;
     double ys[DIM_MAX];
     or_rhs(s, t, y, k[0]);
-    for (int i = 1; i < RK8_S; ++i) {
+    for (int i = 1; i < tb->s; ++i) {
         memcpy(ys, y, sizeof(double) * n);
         for (int j = 0; j < i; ++j) {
-            double a = HH_RK8_A[i][j];
+            double a = tb->a[i * tb->s + j];
             if (a != 0.0) {
                 double ha = h * a;
                 for (int d = 0; d < n; ++d) ys[d] += ha * k[j][d];
             }
         }
-        or_rhs(s, t + HH_RK8_C[i] * h, ys, k[i]);
+        or_rhs(s, t + tb->c[i] * h, ys, k[i]);
     }
     memcpy(y_high, y, sizeof(double) * n);
-    for (int j = 0; j < RK8_S; ++j) {
-        double b = HH_RK8_B[j];
+    for (int j = 0; j < tb->s; ++j) {
+        double b = tb->b[j];
         if (b != 0.0) {
             double hb = h * b;
             for (int d = 0; d < n; ++d) y_high[d] += hb * k[j][d];
@@ -286,8 +307,15 @@ static void rk8_step(or_sys *s, double t, const double *y, double h, double *y_h
     }
 }
 
-int or_rk_fixed(or_sys *s, const double *y0, const double *t_vals, int n_t, double *states,
-                double *derivs) {
+static void rk8_step(or_sys *s, double t, const double *y, double h, double *y_high,
+                     double (*k)[DIM_MAX]) {
+    rk_step(&RK_TAB8, s, t, y, h, y_high, k);
+}
+
+int or_rk_fixed_order(int order, or_sys *s, const double *y0, const double *t_vals, int n_t,
+                      double *states, double *derivs) {
+    const rk_tab *tb = rk_tableau(order);
+    if (!tb) return OR_ERR_ARGS;
     int n = s->dim;
     double k[RK8_S][DIM_MAX];
     memcpy(states, y0, sizeof(double) * n);
@@ -295,11 +323,16 @@ int or_rk_fixed(or_sys *s, const double *y0, const double *t_vals, int n_t, doub
     for (int idx = 0; idx < n_t - 1; ++idx) {
         double tn = t_vals[idx];
         double h = t_vals[idx + 1] - tn;
-        rk8_step(s, tn, states + (size_t)idx * n, h, states + (size_t)(idx + 1) * n, k);
+        rk_step(tb, s, tn, states + (size_t)idx * n, h, states + (size_t)(idx + 1) * n, k);
         if (derivs) or_rhs(s, t_vals[idx + 1], states + (size_t)(idx + 1) * n,
                            derivs + (size_t)(idx + 1) * n);
     }
     return OR_OK;
+}
+
+int or_rk_fixed(or_sys *s, const double *y0, const double *t_vals, int n_t, double *states,
+                double *derivs) {
+    return or_rk_fixed_order(8, s, y0, t_vals, n_t, states, derivs);
 }
 
 /* rk.py:286-327 cubic Hermite */
@@ -718,6 +751,12 @@ static void set_threads(int n_threads) {
 
 int or_batch_rk8_final(int n, const double *mu, const double *y0, double t0, double tf, int steps,
                        double *y_out, int n_threads) {
+    return or_batch_rk_final(8, n, mu, y0, t0, tf, steps, y_out, n_threads);
+}
+
+int or_batch_rk_final(int order, int n, const double *mu, const double *y0, double t0, double tf,
+                      int steps, double *y_out, int n_threads) {
+    if (!rk_tableau(order)) return OR_ERR_ARGS;
     set_threads(n_threads);
     double *tv = malloc(sizeof(double) * steps);
     if (!tv) return OR_ERR_NOMEM;
@@ -733,7 +772,7 @@ int or_batch_rk8_final(int n, const double *mu, const double *y0, double t0, dou
 #pragma omp for schedule(dynamic, 4)
             for (int i = 0; i < n; ++i) {
                 or_sys s = {OR_KIND_CR3BP6, 6, mu[i], 1, 0, 0, 0, 0, 0};
-                or_rk_fixed(&s, y0 + (size_t)i * 6, tv, steps, states, NULL);
+                or_rk_fixed_order(order, &s, y0 + (size_t)i * 6, tv, steps, states, NULL);
                 memcpy(y_out + (size_t)i * 6, states + (size_t)(steps - 1) * 6, sizeof(double) * 6);
             }
             free(states);

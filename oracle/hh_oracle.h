@@ -68,9 +68,12 @@ double or_pow(double x, double y);
 /* numpy.linspace(t0, tf, n) */
 void or_linspace(double t0, double tf, int n, double *out);
 
-/* fixed-step explicit RK over t_vals (rk.py:531-588). order 8 only. */
+/* fixed-step explicit RK8 over t_vals (rk.py:531-588) */
 int or_rk_fixed(or_sys *s, const double *y0, const double *t_vals, int n_t,
                 double *states, double *derivs);
+/* the same for the tableau of order 4, 6 or 8 (rk.py:755-783) */
+int or_rk_fixed_order(int order, or_sys *s, const double *y0, const double *t_vals, int n_t,
+                      double *states, double *derivs);
 
 /* fixed-step RK until event (rk.py:655-718) */
 int or_rk_fixed_event(or_sys *s, const double *y0, const double *t_vals, int n_t,
@@ -98,6 +101,8 @@ double or_dop853_single_step(or_sys *s, double t, const double *y, double h, dou
  * Arrays are trajectory-major: y0[i*dim + c].                              */
 int or_batch_rk8_final(int n, const double *mu, const double *y0, double t0, double tf,
                        int steps, double *y_out, int n_threads);
+int or_batch_rk_final(int order, int n, const double *mu, const double *y0, double t0, double tf,
+                      int steps, double *y_out, int n_threads);
 int or_batch_dop853_final(int n, int dim, const double *mu, const double *y0,
                           const double *t0, const double *tf, double rtol, double atol,
                           double max_step, long long max_steps, double *y_out,

@@ -98,6 +98,9 @@ def _declare(L):
                                         C.c_double, _D, _D, _D, _D]
     L.or_dop853_single_step.restype = C.c_double
     L.or_batch_rk8_final.argtypes = [C.c_int, _D, _D, C.c_double, C.c_double, C.c_int, _D, C.c_int]
+    L.or_batch_rk_final.argtypes = [C.c_int, C.c_int, _D, _D, C.c_double, C.c_double, C.c_int,
+                                    _D, C.c_int]
+    L.or_rk_fixed_order.argtypes = [C.c_int, C.POINTER(OrSys), _D, _D, C.c_int, _D, _D]
     L.or_batch_dop853_final.argtypes = [C.c_int, C.c_int, _D, _D, _D, _D, C.c_double, C.c_double,
                                         C.c_double, C.c_longlong, _D, _L, _L, _I, C.c_int]
     L.or_jacobi.argtypes = [_D, C.c_double]
@@ -196,13 +199,15 @@ def linspace(t0, tf, n):
     return out
 
 
-def rk_fixed(system: System, y0, t_vals, derivs=False):
+def rk_fixed(system: System, y0, t_vals, derivs=False, order=8):
     t_vals = _f64(t_vals)
     y0 = _f64(y0)
     st = np.empty((t_vals.size, system.dim))
     dv = np.empty_like(st) if derivs else None
-    lib().or_rk_fixed(C.byref(system.s), _p(y0), _p(t_vals), t_vals.size, _p(st),
-                      _p(dv) if derivs else None)
+    rc = lib().or_rk_fixed_order(int(order), C.byref(system.s), _p(y0), _p(t_vals), t_vals.size,
+                                 _p(st), _p(dv) if derivs else None)
+    if rc != 0:
+        raise ValueError(f"or_rk_fixed_order: order {order} (error {rc})")
     return (st, dv) if derivs else st
 
 
@@ -282,9 +287,7 @@ def propagate(mu, state0, t0, tf, forward=1, steps=1000, method="adaptive", orde
         return forward * t_eval, np.repeat(y0[None, :], len(t_eval), axis=0)
     sysm = System(kind, mu=mu, fwd=forward, flip=flip_indices, libm_pow=libm_pow)
     if method == "fixed":
-        if order != 8:
-            raise ValueError("only RK8 is restated")
-        states = rk_fixed(sysm, y0, t_eval)
+        states = rk_fixed(sysm, y0, t_eval, order=order)
     elif method == "adaptive":
         states, _ = dop853(sysm, y0, t_eval, rtol=rtol, atol=atol, max_step=max_step)
     else:
@@ -368,13 +371,15 @@ def plane_crossing(mu, x0, forward=1, t_guess=None, t_window=None, index=1, offs
 # batched (CPU baseline)
 # ---------------------------------------------------------------------------
 
-def batch_rk8_final(mu, y0, t0, tf, steps, n_threads=0):
+def batch_rk8_final(mu, y0, t0, tf, steps, n_threads=0, order=8):
     y0 = _f64(y0)
     n = y0.shape[0]
     mu = _f64(np.broadcast_to(mu, (n,)))
     out = np.empty((n, 6))
-    lib().or_batch_rk8_final(n, _p(mu), _p(y0), float(t0), float(tf), int(steps), _p(out),
-                             int(n_threads))
+    rc = lib().or_batch_rk_final(int(order), n, _p(mu), _p(y0), float(t0), float(tf), int(steps),
+                                 _p(out), int(n_threads))
+    if rc != 0:
+        raise ValueError(f"or_batch_rk_final: order {order} (error {rc})")
     return out
 
 

@@ -66,3 +66,29 @@ def test_no_gpu_raises():
         pytest.skip("a GPU is visible")
     with pytest.raises(_lib.HipUnavailable):
         batch.eval_math([1.0], [2.0])
+
+
+def test_rk_fixed_order_checked_without_gpu():
+    from hiten_amd import _lib
+    lib = _lib.load(require_gpu=False)
+    rc = lib.hh_rk_fixed(5, 4, 6, None, None, None, 10, None, 1, 0, None, None, None, 0, 0, None)
+    assert rc == 1 and b"order must be 4, 6 or 8" in lib.hh_last_error()
+
+
+def test_integrator_factories_follow_the_reference():
+    """rk.py:2871-2979: FixedRK orders 4/6/8 (default 4), RungeKutta 4/6/8 fixed
+    and 853 adaptive (default 4); the adaptive order 5 has no kernel."""
+    from hiten_amd import integrators as I
+    assert type(I.FixedRK()).__name__ == "_RK4"
+    assert [I.FixedRK(order=k).order for k in (4, 6, 8)] == [4, 6, 8]
+    assert type(I.RungeKutta()).__name__ == "_RK4"
+    assert type(I.RungeKutta(order=6)).__name__ == "_RK6"
+    assert type(I.RungeKutta(order=853)).__name__ == "_DOP853"
+    with pytest.raises(ValueError):
+        I.FixedRK(order=5)
+    with pytest.raises(ValueError):
+        I.RungeKutta(order=7)
+    with pytest.raises(NotImplementedError):
+        I.AdaptiveRK()
+    with pytest.raises(NotImplementedError):
+        I.RungeKutta(order=45)

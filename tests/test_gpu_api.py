@@ -106,6 +106,13 @@ def test_integrator_objects(gpu, oracle, seeds):
     same(sol.derivatives, d_want, "derivatives")
     sol8 = I.RungeKutta(order=8).integrate(sysm, x0, tv)
     same(sol8.states, oracle.rk_fixed(oracle.System("cr3bp6", mu=MU), x0, tv), "rk8 integrate")
+    for order in (4, 6):
+        sol = I.RungeKutta(order=order).integrate(sysm, x0, tv)
+        same(sol.states, oracle.rk_fixed(oracle.System("cr3bp6", mu=MU), x0, tv, order=order),
+             f"rk{order} integrate")
+        sol = D._propagate_dynsys(sysm, x0, 0.0, 1.3, steps=50, method="fixed", order=order)
+        same(sol.states, oracle.rk_fixed(oracle.System("cr3bp6", mu=MU), x0, tv, order=order),
+             f"rk{order} propagate")
     # event through the integrator interface (rk.py:2256-2325)
     ev = I.RungeKutta(order=853, rtol=1e-12, atol=1e-12).integrate(
         sysm, x0, np.array([0.0, 4.0]), event_fn=I._g_y0, event_cfg=I.EventConfig(direction=-1))

@@ -317,6 +317,42 @@ def test_dop853_6d_max_steps_stop(gpu, oracle, variant):
         assert np.array_equal(st[i, -1], fin[i])
 
 
+@pytest.mark.parametrize("order", [4, 6])
+@pytest.mark.parametrize("variant", ["plain", "wide"])
+def test_rk4_rk6_fixed_match_oracle(gpu, oracle, order, variant):
+    say(f"hh_rk_fixed order {order} {variant}: 96 trajectories x 2000 grid points to 2*pi, "
+        "then backward with flips")
+    w = W.rk8_grid(n=96)
+    got_final, got_traj = batch.rk_fixed(w["mu"], w["y0"], w["t0"], w["tf"], w["steps"],
+                                         order=order, return_trajectory=True, variant=variant)
+    want = oracle.batch_rk8_final(w["mu"], w["y0"], w["t0"], w["tf"], w["steps"], order=order)
+    assert_bitwise(got_final.cpu().numpy(), want, f"rk{order} final", rtol=1e-12)
+    tv = np.linspace(w["t0"], w["tf"], w["steps"])
+    traj = got_traj.cpu().numpy()
+    for i in (0, 53):
+        st = oracle.rk_fixed(oracle.System("cr3bp6", mu=w["mu"]), w["y0"][i], tv, order=order)
+        assert_bitwise(traj[i], st, f"rk{order} grid traj {i}")
+    got = batch.rk_fixed(w["mu"], w["y0"][:32], 0.0, 1.5, 300, order=order, forward=-1,
+                         variant=variant).cpu().numpy()
+    tv = np.linspace(0.0, 1.5, 300)
+    want = np.stack([oracle.rk_fixed(oracle.System("cr3bp6", mu=w["mu"], fwd=-1), w["y0"][i], tv,
+                                     order=order)[-1] for i in range(32)])
+    assert_bitwise(got, want, f"rk{order} backward")
+
+
+@pytest.mark.parametrize("order", [4, 6])
+def test_rk4_rk6_fixed_42d(gpu, oracle, order):
+    say(f"hh_rk_fixed order {order} dim=42 over an explicit grid")
+    w = W.stm_uneven(n=16)
+    tv = np.linspace(0.0, 1.2, 150)
+    fin, traj = batch.rk_fixed(w["mu"], w["y0"], None, None, 0, order=order, dim=42, t_vals=tv,
+                               return_trajectory=True)
+    traj = traj.cpu().numpy()
+    for i in range(0, 16, 5):
+        want = oracle.rk_fixed(oracle.System("cr3bp42", mu=w["mu"]), w["y0"][i], tv, order=order)
+        assert_bitwise(traj[i], want, f"rk{order} 42-d traj {i}")
+
+
 def test_rk8_fixed_42d_and_explicit_grid(gpu, oracle):
     say("hh_rk8_fixed dim=42 over an explicit non-uniform grid")
     w = W.stm_uneven(n=24)

@@ -61,6 +61,105 @@ def test_rk8_table_order_conditions():
     assert abs(b @ (c**5 * (A @ c)) - 1.0 / 16.0) < 2e-15
 
 
+def _stability_coeffs(A, b):
+    """coefficients of R(z) = 1 + sum_k (b^T A^(k-1) 1) z^k, one step of the
+    method on y' = lambda y with z = h lambda"""
+    e = np.ones(len(b))
+    out, v = [1.0], e
+    for _ in range(len(b)):
+        out.append(float(b @ v))
+        v = A @ v
+    return out
+
+
+def test_rk4_rk6_table_order_conditions():
+    A, b, c = table("HH_RK4_A"), table("HH_RK4_B"), table("HH_RK4_C")
+    assert np.allclose(A.sum(axis=1), c, atol=5e-16, rtol=0)
+    for k in range(4):
+        assert abs(b @ c**k - 1.0 / (k + 1)) < 2e-16, k
+    assert abs(b @ (A @ c) - 1.0 / 6.0) < 2e-16
+    assert abs(b @ (c * (A @ c)) - 1.0 / 8.0) < 2e-16
+    assert abs(b @ (A @ c**2) - 1.0 / 12.0) < 2e-16
+    assert abs(b @ (A @ (A @ c)) - 1.0 / 24.0) < 2e-16
+    # classical RK4: R(z) is the degree-4 Taylor polynomial of exp
+    r = _stability_coeffs(A, b)
+    assert np.allclose(r, [1, 1, 1 / 2, 1 / 6, 1 / 24], rtol=1e-15, atol=0)
+    # the reference's RK6 tableau is Dormand-Prince 5(4)'s 5th-order formula:
+    # order-5 conditions hold, and its stability polynomial is the degree-5
+    # Taylor polynomial plus z^6/600 (the published DOPRI5 value)
+    A, b, c = table("HH_RK6_A"), table("HH_RK6_B"), table("HH_RK6_C")
+    assert A.shape == (7, 7) and b[6] == 0.0
+    assert np.allclose(A.sum(axis=1), c, atol=5e-15, rtol=0)
+    for k in range(5):
+        assert abs(b @ c**k - 1.0 / (k + 1)) < 5e-16, k
+    assert abs(b @ (A @ c) - 1.0 / 6.0) < 5e-16
+    assert abs(b @ (A @ c**3) - 1.0 / 20.0) < 5e-16
+    assert abs(b @ (A @ (A @ (A @ c))) - 1.0 / 120.0) < 5e-16
+    r = _stability_coeffs(A, b)
+    assert np.allclose(r, [1, 1, 1 / 2, 1 / 6, 1 / 24, 1 / 120, 1 / 600, 0], rtol=1e-13,
+                       atol=1e-16)
+
+
+def _linear_step_closed_form(coeffs, M, y0, h, n_steps):
+    """n steps of a method with stability polynomial ``coeffs`` on y' = M y"""
+    P = np.zeros_like(M)
+    Z = np.eye(len(M))
+    for c in coeffs:
+        P = P + c * Z
+        Z = Z @ (h * M)
+    y = np.asarray(y0, float)
+    for _ in range(n_steps):
+        y = P @ y
+    return y
+
+
+@pytest.mark.parametrize("order,coeffs", [
+    (4, [1, 1, 1 / 2, 1 / 6, 1 / 24]),
+    (6, [1, 1, 1 / 2, 1 / 6, 1 / 24, 1 / 120, 1 / 600])])
+def test_rk4_rk6_linear_ode_closed_form(oracle, order, coeffs):
+    # on a linear ODE one step is R(hM) y exactly; a coarse grid makes the
+    # truncation visible, so a wrong coefficient or stage fails here
+    M = [[0.0, 1.0], [-1.0, -0.1]]
+    sysm = oracle.System("linear", dim=2, A=M)
+    t = np.linspace(0.0, 3.0, 7)
+    st = oracle.rk_fixed(sysm, [1.0, 0.3], t, order=order)
+    want = _linear_step_closed_form(coeffs, np.array(M), [1.0, 0.3], t[1] - t[0], 6)
+    assert np.allclose(st[-1], want, rtol=1e-14, atol=1e-15)
+    exact = np.array([1.0, 0.3])
+    assert np.max(np.abs(st[-1] - want)) < 1e-3 * np.max(np.abs(st[-1] - exact) + 1e-300)
+
+
+@pytest.mark.parametrize("order,p", [(4, 4), (6, 5)])
+def test_rk4_rk6_convergence_order(oracle, order, p):
+    sysm = oracle.System("cr3bp6", mu=0.0)
+    y0 = _kepler_circular(0.5, 0.0, 0.0)
+    exact = _kepler_circular(0.5, 0.0, 1.0)
+    errs = []
+    for n in (41, 81, 161):
+        st = oracle.rk_fixed(sysm, y0, np.linspace(0.0, 1.0, n), order=order)
+        errs.append(np.max(np.abs(st[-1] - exact)))
+    for a, b in zip(errs, errs[1:]):
+        assert 2 ** (p - 0.5) < a / b < 2 ** (p + 0.7), (order, errs)
+
+
+@pytest.mark.parametrize("order,tol", [(4, 1e-4), (6, 1e-5)])
+def test_rk4_rk6_harmonic_oscillator_vs_scipy(oracle, order, tol):
+    # reference integrators/_tests/test_rk.py:264-310 (_RK4 / _RK6 rows)
+    sysm = oracle.System("linear", dim=2, A=[[0.0, 1.0], [-1.0, 0.0]])
+    t = np.linspace(0.0, 2 * math.pi, 2000)
+    st = oracle.rk_fixed(sysm, [1.0, 0.0], t, order=order)
+    sp = solve_ivp(lambda tt, y: [y[1], -y[0]], [0, t[-1]], [1.0, 0.0], method="DOP853",
+                   t_eval=t, rtol=1e-12, atol=1e-12)
+    assert np.sqrt(np.mean((st[:, 0] - sp.y[0])**2)) < tol
+    assert np.sqrt(np.mean((st[:, 1] - sp.y[1])**2)) < tol
+
+
+def test_rk_fixed_rejects_other_orders(oracle):
+    sysm = oracle.System("cr3bp6", mu=MU)
+    with pytest.raises(ValueError):
+        oracle.rk_fixed(sysm, np.zeros(6) + 0.8, np.linspace(0, 1, 5), order=5)
+
+
 # ---------------------------------------------------------------------------
 # pow
 # ---------------------------------------------------------------------------
