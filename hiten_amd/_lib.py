@@ -59,6 +59,8 @@ _SIGNATURES = {
     "hh_get_tuning": ([C.POINTER(Tuning)], _I),
     "hh_rk8_fixed": ([_I, _I, _P, _P, _P, _I, _P, _I, _ULL, _P, _P, _P, _I, _I, _P], _I),
     "hh_rk_fixed": ([_I, _I, _I, _P, _P, _P, _I, _P, _I, _ULL, _P, _P, _P, _I, _I, _P], _I),
+    "hh_rk_fixed_event": ([_I, _I, _P, _P, _P, _I, _P, _I, _ULL, _I, _D, _I, _D, _D, _P, _P, _P,
+                           _P, _P, _I, _P], _I),
     "hh_dop853_final": ([_I, _I, _P, _P, _P, _D, _D, _D, _D, _LL, _I, _ULL, _P, _P, _P, _P, _P,
                          _I, _I, _I, _P, _LL, _P, _P], _I),
     "hh_dop853_final_workspace": ([_I, _I, _I], _LL),

@@ -162,6 +162,47 @@ def rk_fixed(mu, y0, t0, tf, steps: int, *, order: int = 8, dim: int = 6, t_vals
 
 
 @dataclass
+class FixedEventResult:
+    hit: torch.Tensor            # (n,) bool
+    t_hit: torch.Tensor          # (n,)
+    y_hit: torch.Tensor          # (n, 6)
+    y_last: torch.Tensor         # (n, 6)
+
+
+def rk_fixed_event(mu, y0, t0, tf, steps: int, *, order: int = 8, t_vals=None,
+                   ev_index: int = 1, ev_offset: float = 0.0, direction: int = 0,
+                   xtol: float = 1e-12, gtol: float = 1e-12, forward: int = 1,
+                   flip_indices=None, block: int = 0, stream=None) -> FixedEventResult:
+    """Fixed-step RK (order 4, 6 or 8) until the first crossing of
+    y[ev_index] = ev_offset, refined by bisection on the step's cubic Hermite
+    interpolant (rk.py:655-718, 330-391); grid as rk_fixed.  Without a
+    crossing ``t_hit`` is the last grid time and ``y_hit`` the last state."""
+    if order not in FIXED_ORDERS:
+        raise ValueError("RK order must be 4, 6, or 8")
+    lib = _lib.load()
+    y = _soa(y0, 6)
+    n = y.shape[1]
+    mu_t = _vec(mu, n, "mu")
+    tv = None
+    if t_vals is not None:
+        tv = _vec(t_vals, len(t_vals), "t_vals")
+        steps = tv.shape[0]
+        t0_t = tf_t = None
+    else:
+        t0_t, tf_t = _vec(t0, n, "t0"), _vec(tf, n, "tf")
+    hit = torch.empty(n, dtype=torch.int32, device=y.device)
+    th = torch.empty(n, dtype=torch.float64, device=y.device)
+    yh, yl = torch.empty_like(y), torch.empty_like(y)
+    rc = lib.hh_rk_fixed_event(int(order), n, _ptr(mu_t), _ptr(t0_t), _ptr(tf_t), int(steps),
+                               _ptr(tv), int(forward), flip_mask(6, flip_indices), int(ev_index),
+                               float(ev_offset), int(direction), float(xtol), float(gtol), _ptr(y),
+                               _ptr(hit), _ptr(th), _ptr(yh), _ptr(yl), int(block),
+                               _lib.stream_handle(stream))
+    _lib.check(rc, "hh_rk_fixed_event")
+    return FixedEventResult(hit.bool(), th, yh.t(), yl.t())
+
+
+@dataclass
 class FinalBatch:
     """Device-resident inputs and outputs of one DOP853 final-value batch."""
     dim: int

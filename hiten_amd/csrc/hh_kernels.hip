@@ -321,6 +321,114 @@ void k_rk_fixed42(int n, const double* __restrict__ mu, const double* __restrict
     }
 }
 
+/* _hermite_eval_dense (rk.py:286-327) of one component */
+__device__ __forceinline__ double hermite1(double y0, double f0, double y1, double f1, double x,
+                                           double h) {
+    const double x2 = x * x;
+    const double x3 = x2 * x;
+    const double H00 = (2.0 * x3 - 3.0 * x2) + 1.0;
+    const double H10 = (x3 - 2.0 * x2) + x;
+    const double H01 = -2.0 * x3 + 3.0 * x2;
+    const double H11 = x3 - x2;
+    return ((H00 * y0 + H10 * (h * f0)) + H01 * y1) + H11 * (h * f1);
+}
+
+/* Fixed-step RK of tableau M until the plane event g = y[ev] - c, 6-d, one
+ * trajectory per lane: _integrate_fixed_rk_until_event (rk.py:655-718) with
+ * _hermite_refine_in_step (rk.py:330-391) over rk_run's grid.  Only the
+ * event component is interpolated inside the bisection; the state is formed
+ * at the returned abscissa (the reference's components are independent, so
+ * the bits are the same).  Without a crossing t_hit is the last grid time
+ * and y_hit = y_last = the last state.                                     */
+template <class M>
+__global__ __launch_bounds__(256) void k_rk_fixed_event6(
+    int n, const double* __restrict__ mu, const double* __restrict__ t0a,
+    const double* __restrict__ tfa, int steps, const double* __restrict__ t_vals, int fwd,
+    unsigned long long flip, int ev_index, double ev_off, int dir, double xtol, double gtol,
+    const double* __restrict__ y0, int* __restrict__ hit, double* __restrict__ t_hit,
+    double* __restrict__ y_hit, double* __restrict__ y_last) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Lane6 p;
+    p.mu = mu[i];
+    p.flipbits = lane_flip_bits(fwd, flip, 0, false);
+    p.flip_on = (fwd == -1) && flip != 0;
+    double y[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) y[d] = y0[(size_t)d * n + i];
+    const double t0 = t_vals ? t_vals[0] : t0a[i];
+    const double tf = t_vals ? t_vals[steps - 1] : tfa[i];
+    const double delta = tf - t0;
+    const double step = delta / (double)(steps - 1);
+    const bool zstep = (step == 0.0);
+    const bool constant = isclose(t0, tf);
+    int found = 0;
+    double th = grid_time(t_vals, steps - 1, steps, t0, tf, step, delta, zstep);
+    double yh[6], yl[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) yh[d] = yl[d] = y[d];
+    if (!constant) {
+        double f[6];
+        p.rhs(y, f);
+        double gp = pick6(y, ev_index) - ev_off;
+        double t_prev = grid_time(t_vals, 0, steps, t0, tf, step, delta, zstep);
+        for (int s_idx = 1; s_idx < steps; ++s_idx) {
+            const double t_s = grid_time(t_vals, s_idx, steps, t0, tf, step, delta, zstep);
+            const double h = t_s - t_prev;
+            double yn[6], fn[6];
+#pragma unroll
+            for (int d = 0; d < 6; ++d) yn[d] = y[d];
+            rk_step<M>(p, yn, h);
+            p.rhs(yn, fn);
+            const double gn = pick6(yn, ev_index) - ev_off;
+            if (event_crossed(gp, gn, dir)) {
+                const double ye0 = pick6(y, ev_index), fe0 = pick6(f, ev_index);
+                const double ye1 = pick6(yn, ev_index), fe1 = pick6(fn, ev_index);
+                double a = 0.0, b = 1.0, gl = ye0 - ev_off, x = -1.0;
+                for (int it = 0; it < 128; ++it) {
+                    const double mid = 0.5 * (a + b);
+                    const double gm = hermite1(ye0, fe0, ye1, fe1, mid, h) - ev_off;
+                    if (__builtin_fabs(gm) <= gtol) {
+                        x = mid;
+                        break;
+                    }
+                    if (crossed_direction(gl, gm, dir)) {
+                        b = mid;
+                    } else {
+                        a = mid;
+                        gl = gm;
+                    }
+                    if ((b - a) * __builtin_fabs(h) <= xtol) break;
+                }
+                if (x < 0.0) x = b;
+                found = 1;
+                th = t_prev + x * h;
+#pragma unroll
+                for (int d = 0; d < 6; ++d) {
+                    yh[d] = hermite1(y[d], f[d], yn[d], fn[d], x, h);
+                    yl[d] = yn[d];
+                }
+                break;
+            }
+#pragma unroll
+            for (int d = 0; d < 6; ++d) {
+                y[d] = yn[d];
+                f[d] = fn[d];
+                yh[d] = yl[d] = yn[d];
+            }
+            gp = gn;
+            t_prev = t_s;
+        }
+    }
+    hit[i] = found;
+    t_hit[i] = th;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        y_hit[(size_t)d * n + i] = yh[d];
+        y_last[(size_t)d * n + i] = yl[d];
+    }
+}
+
 /* k_rk_fixed6 with one trajectory per aligned 8-lane group, one component
  * per lane (Group6): rk_run's grid, steps and stores, per component.        */
 template <class M, int BLOCK>
@@ -1747,6 +1855,43 @@ int hh_rk_fixed(int order, int n, int dim, const double* mu, const double* t0, c
         launch_rk_fixed<RK8>(n, dim, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out,
                              traj_out, variant, block, st);
     return check_launch("hh_rk_fixed launch");
+}
+
+int hh_rk_fixed_event(int order, int n, const double* mu, const double* t0, const double* tf,
+                      int steps, const double* t_vals, int fwd, unsigned long long flip_mask,
+                      int ev_index, double ev_offset, int direction, double xtol, double gtol,
+                      const double* y0, int* hit, double* t_hit, double* y_hit, double* y_last,
+                      int block, void* stream) {
+    if (order != 4 && order != 6 && order != 8)
+        return fail(HH_ERR_ARGS, "hh_rk_fixed_event: order must be 4, 6 or 8");
+    if (n < 0) return fail(HH_ERR_ARGS, "hh_rk_fixed_event: n < 0");
+    if (n == 0) return HH_OK;
+    if (steps < 2) return fail(HH_ERR_ARGS, "hh_rk_fixed_event: steps must be >= 2");
+    if (ev_index < 0 || ev_index > 5)
+        return fail(HH_ERR_ARGS, "hh_rk_fixed_event: ev_index must be in 0..5");
+    if (!mu || !y0 || !hit || !t_hit || !y_hit || !y_last || (!t_vals && (!t0 || !tf)))
+        return fail(HH_ERR_ARGS, "hh_rk_fixed_event: null pointer");
+    if (block <= 0) block = 128;
+    if (block > 256 || block % 64)
+        return fail(HH_ERR_ARGS, "hh_rk_fixed_event: block must be 64..256, multiple of 64");
+    const double* t0p = t0 ? t0 : mu;
+    const double* tfp = tf ? tf : mu;
+    const int dir = direction > 0 ? 1 : (direction < 0 ? -1 : 0);
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 grid(blocks_for(n, block)), blk(block);
+    if (order == 4)
+        hipLaunchKernelGGL(k_rk_fixed_event6<RK4>, grid, blk, 0, st, n, mu, t0p, tfp, steps,
+                           t_vals, fwd, flip_mask, ev_index, ev_offset, dir, xtol, gtol, y0, hit,
+                           t_hit, y_hit, y_last);
+    else if (order == 6)
+        hipLaunchKernelGGL(k_rk_fixed_event6<RK6>, grid, blk, 0, st, n, mu, t0p, tfp, steps,
+                           t_vals, fwd, flip_mask, ev_index, ev_offset, dir, xtol, gtol, y0, hit,
+                           t_hit, y_hit, y_last);
+    else
+        hipLaunchKernelGGL(k_rk_fixed_event6<RK8>, grid, blk, 0, st, n, mu, t0p, tfp, steps,
+                           t_vals, fwd, flip_mask, ev_index, ev_offset, dir, xtol, gtol, y0, hit,
+                           t_hit, y_hit, y_last);
+    return check_launch("hh_rk_fixed_event launch");
 }
 
 int hh_rk8_fixed(int n, int dim, const double* mu, const double* t0, const double* tf, int steps,

@@ -206,8 +206,20 @@ class _FixedStepRK(_Integrator):
             return const
         mu, dim, fwd, flip = _system_params(system)
         if event_fn is not None:
-            raise NotImplementedError("fixed-step event search runs through DOP853 on the GPU; "
-                                      "use AdaptiveRK(order=8)")
+            # rk.py:474-513: scan the fixed steps, refine on the Hermite cubic
+            if dim != 6:
+                raise NotImplementedError("event search runs on the 6-d CR3BP field")
+            ev = _plane_event(event_fn)
+            direction = 0 if event_cfg is None else int(event_cfg.direction)
+            xtol = float(event_options.xtol if event_options is not None else 1e-12)
+            gtol = float(event_options.gtol if event_options is not None else 1e-12)
+            r = batch.rk_fixed_event(mu, y0[None, :], None, None, 0, order=self._p,
+                                     t_vals=t_vals, ev_index=ev.index, ev_offset=ev.offset,
+                                     direction=direction, xtol=xtol, gtol=gtol, forward=fwd,
+                                     flip_indices=flip)
+            t_end = float(r.t_hit[0]) if bool(r.hit[0]) else float(t_vals[-1])
+            return _Solution(times=np.array([t_vals[0], t_end]),
+                             states=np.vstack([y0, r.y_hit[0].cpu().numpy()]))
         fin, traj = batch.rk_fixed(mu, y0[None, :], None, None, 0, order=self._p, dim=dim,
                                    t_vals=t_vals, forward=fwd, flip_indices=flip,
                                    return_trajectory=True)

@@ -113,6 +113,23 @@ int hh_rk_fixed(int order, int n, int dim, const double* mu, const double* t0, c
                 const double* y0, double* y_out, double* traj_out, int variant, int block,
                 void* stream);
 
+/* Fixed-step RK of the given order (4, 6 or 8) on the 6-d field until the
+ * plane event g = y[ev_index] - ev_offset crosses zero in `direction`
+ * (0 either, +1 rising, -1 falling; g = 0 at a step's end counts), then
+ * bisection on the cubic Hermite interpolant of the step to xtol / gtol.
+ * Replaces: _FixedStepRK._integrate_fixed_rk_until_event (integrators/
+ * rk.py:655-718) with _hermite_refine_in_step (rk.py:330-391), reached by
+ * _FixedStepRK.integrate(..., event_fn=...) (rk.py:474-513).
+ *   grid        : as hh_rk_fixed (t_vals shared, else linspace(t0, tf, steps))
+ *   hit, t_hit  : [n]; without a crossing hit = 0 and t_hit = the last grid time
+ *   y0, y_hit, y_last : [6][n]; y_last is the state at the end of the step
+ *                that crossed (the last grid state without a crossing)  */
+int hh_rk_fixed_event(int order, int n, const double* mu, const double* t0, const double* tf,
+                      int steps, const double* t_vals, int fwd, unsigned long long flip_mask,
+                      int ev_index, double ev_offset, int direction, double xtol, double gtol,
+                      const double* y0, int* hit, double* t_hit, double* y_hit, double* y_last,
+                      int block, void* stream);
+
 /* Adaptive DOP853 from t0[i] to tf[i]; y_out is the dense-output value at
  * tf[i], i.e. the last row of _propagate_dynsys(..., steps=2,
  * method="adaptive") and the final Phi of _compute_stm.

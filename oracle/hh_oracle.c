@@ -351,6 +351,16 @@ static void hermite_eval(int n, const double *y0, const double *f0, const double
 int or_rk_fixed_event(or_sys *s, const double *y0, const double *t_vals, int n_t, int ev_index,
                       double ev_offset, int direction, double xtol, double gtol, int *hit,
                       double *t_hit, double *y_hit, double *y_last) {
+    return or_rk_fixed_event_order(8, s, y0, t_vals, n_t, ev_index, ev_offset, direction, xtol,
+                                   gtol, hit, t_hit, y_hit, y_last);
+}
+
+int or_rk_fixed_event_order(int order, or_sys *s, const double *y0, const double *t_vals,
+                            int n_t, int ev_index, double ev_offset, int direction, double xtol,
+                            double gtol, int *hit, double *t_hit, double *y_hit,
+                            double *y_last) {
+    const rk_tab *tb = rk_tableau(order);
+    if (!tb) return OR_ERR_ARGS;
     int n = s->dim;
     double k[RK8_S][DIM_MAX], y[DIM_MAX], yn[DIM_MAX], fp[DIM_MAX], fn[DIM_MAX], ym[DIM_MAX];
     memcpy(y, y0, sizeof(double) * n);
@@ -359,7 +369,7 @@ int or_rk_fixed_event(or_sys *s, const double *y0, const double *t_vals, int n_t
     for (int idx = 0; idx < n_t - 1; ++idx) {
         double tn = t_vals[idx];
         double h = t_vals[idx + 1] - tn;
-        rk8_step(s, tn, y, h, yn, k);
+        rk_step(tb, s, tn, y, h, yn, k);
         or_rhs(s, tn + h, yn, fn);
         double gn = yn[ev_index] - ev_offset;
         if (event_crossed(gp, gn, direction)) {

@@ -79,6 +79,10 @@ int or_rk_fixed_order(int order, or_sys *s, const double *y0, const double *t_va
 int or_rk_fixed_event(or_sys *s, const double *y0, const double *t_vals, int n_t,
                       int ev_index, double ev_offset, int direction, double xtol, double gtol,
                       int *hit, double *t_hit, double *y_hit, double *y_last);
+int or_rk_fixed_event_order(int order, or_sys *s, const double *y0, const double *t_vals,
+                            int n_t, int ev_index, double ev_offset, int direction, double xtol,
+                            double gtol, int *hit, double *t_hit, double *y_hit,
+                            double *y_last);
 
 /* DOP853 with dense output at t_eval (rk.py:2375-2549) */
 int or_dop853(or_sys *s, const double *y0, const double *t_eval, int m,

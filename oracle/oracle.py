@@ -86,6 +86,9 @@ def _declare(L):
     L.or_pow.restype = C.c_double
     L.or_linspace.argtypes = [C.c_double, C.c_double, C.c_int, _D]
     L.or_rk_fixed.argtypes = [C.POINTER(OrSys), _D, _D, C.c_int, _D, _D]
+    L.or_rk_fixed_event_order.argtypes = [C.c_int, C.POINTER(OrSys), _D, _D, C.c_int, C.c_int,
+                                          C.c_double, C.c_int, C.c_double, C.c_double,
+                                          C.POINTER(C.c_int), C.POINTER(C.c_double), _D, _D]
     L.or_rk_fixed_event.argtypes = [C.POINTER(OrSys), _D, _D, C.c_int, C.c_int, C.c_double,
                                     C.c_int, C.c_double, C.c_double, _I, _D, _D, _D]
     L.or_dop853.argtypes = [C.POINTER(OrSys), _D, _D, C.c_int, C.c_double, C.c_double,
@@ -212,16 +215,18 @@ def rk_fixed(system: System, y0, t_vals, derivs=False, order=8):
 
 
 def rk_fixed_event(system, y0, t_vals, ev_index, ev_offset=0.0, direction=0, xtol=1e-12,
-                   gtol=1e-12):
+                   gtol=1e-12, order=8):
     t_vals = _f64(t_vals)
     y0 = _f64(y0)
     hit = C.c_int(0)
     th = C.c_double(0)
     yh = np.empty(system.dim)
     yl = np.empty(system.dim)
-    lib().or_rk_fixed_event(C.byref(system.s), _p(y0), _p(t_vals), t_vals.size, ev_index,
-                            ev_offset, direction, xtol, gtol, C.byref(hit), C.byref(th), _p(yh),
-                            _p(yl))
+    rc = lib().or_rk_fixed_event_order(int(order), C.byref(system.s), _p(y0), _p(t_vals),
+                                       t_vals.size, ev_index, ev_offset, direction, xtol, gtol,
+                                       C.byref(hit), C.byref(th), _p(yh), _p(yl))
+    if rc != 0:
+        raise ValueError(f"or_rk_fixed_event_order: order {order} (error {rc})")
     return bool(hit.value), th.value, yh, yl
 
 

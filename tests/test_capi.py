@@ -92,3 +92,14 @@ def test_integrator_factories_follow_the_reference():
         I.AdaptiveRK()
     with pytest.raises(NotImplementedError):
         I.RungeKutta(order=45)
+
+
+def test_rk_fixed_event_arguments_checked_without_gpu():
+    from hiten_amd import _lib
+    lib = _lib.load(require_gpu=False)
+    rc = lib.hh_rk_fixed_event(7, 4, None, None, None, 10, None, 1, 0, 1, 0.0, 0, 1e-12, 1e-12,
+                               None, None, None, None, None, 0, None)
+    assert rc == 1 and b"order must be 4, 6 or 8" in lib.hh_last_error()
+    rc = lib.hh_rk_fixed_event(4, 4, None, None, None, 10, None, 1, 0, 9, 0.0, 0, 1e-12, 1e-12,
+                               None, None, None, None, None, 0, None)
+    assert rc == 1 and b"ev_index" in lib.hh_last_error()

@@ -113,6 +113,16 @@ def test_integrator_objects(gpu, oracle, seeds):
         sol = D._propagate_dynsys(sysm, x0, 0.0, 1.3, steps=50, method="fixed", order=order)
         same(sol.states, oracle.rk_fixed(oracle.System("cr3bp6", mu=MU), x0, tv, order=order),
              f"rk{order} propagate")
+    # fixed-step events through the integrator interface (rk.py:474-513)
+    for order in (4, 8):
+        ev = I.RungeKutta(order=order).integrate(sysm, x0, np.linspace(0.0, 4.0, 400),
+                                                 event_fn=I._g_y0,
+                                                 event_cfg=I.EventConfig(direction=-1))
+        hit, th, yh, _ = oracle.rk_fixed_event(oracle.System("cr3bp6", mu=MU), x0,
+                                               np.linspace(0.0, 4.0, 400), 1, 0.0, -1,
+                                               order=order)
+        assert ev.times[-1] == (th if hit else 4.0)
+        same(ev.states[-1], yh, f"rk{order} event state")
     # event through the integrator interface (rk.py:2256-2325)
     ev = I.RungeKutta(order=853, rtol=1e-12, atol=1e-12).integrate(
         sysm, x0, np.array([0.0, 4.0]), event_fn=I._g_y0, event_cfg=I.EventConfig(direction=-1))

@@ -340,6 +340,36 @@ def test_rk4_rk6_fixed_match_oracle(gpu, oracle, order, variant):
     assert_bitwise(got, want, f"rk{order} backward")
 
 
+@pytest.mark.parametrize("order", [4, 6, 8])
+def test_rk_fixed_event_matches_oracle(gpu, oracle, order):
+    say(f"hh_rk_fixed_event order {order}: 96 states, y = 0 falling, then x = 0.85 either "
+        "way backward with flips, over an explicit grid")
+    w = W.rk8_grid(n=96)
+    tv = np.linspace(0.0, 6.0, 900)
+    r = batch.rk_fixed_event(w["mu"], w["y0"], None, None, 0, order=order, t_vals=tv,
+                             ev_index=1, direction=-1, xtol=1e-13, gtol=1e-13)
+    hit, th = r.hit.cpu().numpy(), r.t_hit.cpu().numpy()
+    yh, yl = r.y_hit.cpu().numpy(), r.y_last.cpu().numpy()
+    sysm = oracle.System("cr3bp6", mu=w["mu"])
+    assert hit.any()
+    for i in range(96):
+        h, t, y, ylast = oracle.rk_fixed_event(sysm, w["y0"][i], tv, 1, 0.0, -1, xtol=1e-13,
+                                               gtol=1e-13, order=order)
+        assert bool(hit[i]) == h and th[i] == t, i
+        assert_bitwise(yh[i], y, f"rk{order} event state {i}")
+        if h:
+            assert_bitwise(yl[i], ylast, f"rk{order} event step end {i}")
+    r = batch.rk_fixed_event(w["mu"], w["y0"][:48], 0.0, 3.0, 500, order=order, ev_index=0,
+                             ev_offset=0.85, direction=0, forward=-1, flip_indices=slice(0, 6))
+    sysm = oracle.System("cr3bp6", mu=w["mu"], fwd=-1)
+    tv = np.linspace(0.0, 3.0, 500)
+    hit, th, yh = r.hit.cpu().numpy(), r.t_hit.cpu().numpy(), r.y_hit.cpu().numpy()
+    for i in range(48):
+        h, t, y, _ = oracle.rk_fixed_event(sysm, w["y0"][i], tv, 0, 0.85, 0, order=order)
+        assert bool(hit[i]) == h and th[i] == t, i
+        assert_bitwise(yh[i], y, f"rk{order} backward event {i}")
+
+
 @pytest.mark.parametrize("order", [4, 6])
 def test_rk4_rk6_fixed_42d(gpu, oracle, order):
     say(f"hh_rk_fixed order {order} dim=42 over an explicit grid")

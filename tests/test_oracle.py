@@ -483,12 +483,33 @@ def test_event_near_plane(oracle):
     assert hit and abs(t - 1e-9) < 1e-9 and abs(y[0] - 1.0) < 1e-10
 
 
-def test_event_fixed_rk_hermite_refine(oracle):
+@pytest.mark.parametrize("order", [4, 6, 8])
+def test_event_fixed_rk_hermite_refine(oracle, order):
     # rk.py:655-718 + 330-391: the cubic Hermite interpolant is exact for a
     # linear solution, so bisection lands on the crossing inside the step
     hit, t, y, _ = oracle.rk_fixed_event(_slope(oracle, 1.0), [0.0], np.linspace(0.0, 2.0, 21),
-                                         0, 1.05, +1)
+                                         0, 1.05, +1, order=order)
     assert hit and abs(t - 1.05) < 1e-12 and abs(y[0] - 1.05) < 1e-12
+    # the falling direction ignores a rising crossing; no hit ends on the grid
+    hit, t, y, yl = oracle.rk_fixed_event(_slope(oracle, 1.0), [0.0], np.linspace(0.0, 2.0, 21),
+                                          0, 1.05, -1, order=order)
+    assert not hit and t == 2.0 and abs(y[0] - 2.0) < 1e-12 and np.array_equal(y, yl)
+
+
+def test_event_fixed_rk_circular_orbit(oracle):
+    # mu = 0 circular orbit from y = 0 (x > 0) turning counter-clockwise: the
+    # first (falling) crossing of y = 0 is at half the synodic period (closed
+    # form), found by every fixed-step order to the accuracy of its steps and
+    # of the Hermite cubic between them
+    r = 0.5
+    w = r ** -1.5 - 1.0
+    sysm = oracle.System("cr3bp6", mu=0.0)
+    y0 = _kepler_circular(r, 0.0, 0.0)
+    tv = np.linspace(0.0, 1.5 * math.pi / w, 3000)
+    for order, tol in ((4, 1e-8), (6, 1e-10), (8, 5e-12)):
+        hit, t, y, _ = oracle.rk_fixed_event(sysm, y0, tv, 1, 0.0, -1, order=order)
+        assert hit and abs(t - math.pi / w) < tol, (order, t)
+        assert abs(y[1]) < 1e-11 and abs(y[0] + r) < tol
 
 
 def test_event_zero_at_step_end_counts_as_hit(oracle):
