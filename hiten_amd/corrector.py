@@ -76,6 +76,7 @@ class CorrectionProblem:
     min_alpha: float = 1e-4
     armijo_c: float = 0.1
     forward: int = 1
+    event_index: int = Y        # section y = 0 (Halo, Lyapunov); Z for vertical orbits
 
 
 def halo_problem(mu: float, x0, **kw) -> CorrectionProblem:
@@ -86,6 +87,13 @@ def halo_problem(mu: float, x0, **kw) -> CorrectionProblem:
 def lyapunov_problem(mu: float, x0, **kw) -> CorrectionProblem:
     """Lyapunov defaults (services/orbits.py:1238-1302)."""
     return CorrectionProblem(mu, np.asarray(x0, float), (VX, Z), (VY, VZ), (0.0, 0.0), None, **kw)
+
+
+def vertical_problem(mu: float, x0, **kw) -> CorrectionProblem:
+    """Vertical-orbit defaults (services/orbits.py:1429-1446): VX = Y = 0 at
+    the z = 0 crossing (_z_plane_crossing), controls VZ and VY."""
+    kw.setdefault("event_index", Z)
+    return CorrectionProblem(mu, np.asarray(x0, float), (VX, Y), (VZ, VY), (0.0, 0.0), None, **kw)
 
 
 @dataclass
@@ -135,8 +143,9 @@ def solve_delta_dense(J: np.ndarray, r: np.ndarray, cond_threshold: float = 1e8)
 class _Engine:
     """GPU evaluation of the operators for a set of orbits."""
 
-    def event_batch(self, mu: np.ndarray, states: np.ndarray, forward: int):
-        """_y_plane_crossing for each row: (t (n,), x_event (n, 6), ok (n,))."""
+    def event_batch(self, mu: np.ndarray, states: np.ndarray, forward: int, index: int = Y):
+        """_y_plane_crossing (the crossing of plane y[index] = 0) for each
+        row: (t (n,), x_event (n, 6), ok (n,))."""
         n = states.shape[0]
         t = np.full(n, np.nan)
         xe = np.full((n, 6), np.nan)
@@ -144,7 +153,8 @@ class _Engine:
         for m in np.unique(mu):
             sel = np.nonzero(mu == m)[0]
             hits = section_crossing_batch(rtbp_dynsys(float(m)), states[sel],
-                                          surface=_PlaneEvent("y", 0.0, None), forward=forward)
+                                          surface=_PlaneEvent("xyz"[index], 0.0, None),
+                                          forward=forward)
             for j, h in zip(sel, hits):
                 if h is not None:
                     t[j], xe[j], ok[j] = h.time, h.state, True
@@ -233,6 +243,7 @@ class _Bank:
         self.problems: list = []          # by row (None once released)
         self.mu = np.zeros(0)             # by row
         self.fwd = np.zeros(0, dtype=np.int64)
+        self.evi = np.zeros(0, dtype=np.int64)   # section plane coordinate by row
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
@@ -248,6 +259,8 @@ class _Bank:
             raise RuntimeError(f"{what} failed (code {rc})")
 
     def add(self, p: CorrectionProblem) -> int:
+        if int(p.event_index) not in (X, Y, Z):
+            raise ValueError("the section plane is x, y or z = 0 (event_index 0, 1 or 2)")
         res, ctrl = _i32(list(p.residual_indices)), _i32(list(p.control_indices))
         x0, tgt = _f64(p.x0), _f64(p.target)
         md = math.inf if p.max_delta is None else float(p.max_delta)
@@ -263,10 +276,12 @@ class _Bank:
                 cap = max(64, 2 * len(self.mu))
                 self.mu = np.concatenate([self.mu, np.zeros(cap - len(self.mu))])
                 self.fwd = np.concatenate([self.fwd, np.zeros(cap - len(self.fwd), np.int64)])
+                self.evi = np.concatenate([self.evi, np.zeros(cap - len(self.evi), np.int64)])
         else:
             self.problems[row] = p
         self.mu[row] = float(p.mu)
         self.fwd[row] = int(p.forward)
+        self.evi[row] = int(p.event_index)
         return row
 
     def release(self, row: int) -> None:
@@ -343,7 +358,7 @@ def _drive(bank: _Bank, eng: "_Engine") -> None:
     """Run every row of a bank to completion, one batched launch per
     request kind per round."""
     rows_all = np.arange(bank.n)
-    mu, fwd = bank.mu, bank.fwd
+    mu, fwd, evi = bank.mu, bank.fwd, bank.evi
     while True:
         st = bank.stages(rows_all)
         live = rows_all[st != bank.DONE]
@@ -352,10 +367,10 @@ def _drive(bank: _Bank, eng: "_Engine") -> None:
         sl = st[st != bank.DONE]
         ev = live[sl != bank.JAC]
         jac = live[sl == bank.JAC]
-        for f in np.unique(fwd[ev]):
-            rows = ev[fwd[ev] == f]
+        for f, e in sorted(set(zip(fwd[ev].tolist(), evi[ev].tolist()))):
+            rows = ev[(fwd[ev] == f) & (evi[ev] == e)]
             states, counts = bank.event_inputs(rows)
-            t, xe, ok = eng.event_batch(np.repeat(mu[rows], counts), states, int(f))
+            t, xe, ok = eng.event_batch(np.repeat(mu[rows], counts), states, int(f), index=int(e))
             bank.feed_events(rows, t, xe, ok)
         if jac.size:
             states, tev = bank.stm_inputs(jac)
@@ -393,13 +408,15 @@ PROBLEM_DTYPE = np.dtype([
     ("alpha_reduction", "f8"), ("min_alpha", "f8"), ("armijo_c", "f8"),
     ("residual_indices", "i4", 6), ("control_indices", "i4", 6), ("n_residuals", "i4"),
     ("n_controls", "i4"), ("halo", "i4"), ("forward", "i4"), ("max_attempts", "i4"),
-    ("reserved", "i4")], align=True)
+    ("event_plane", "i4")], align=True)
 RESULT_DTYPE = np.dtype([
     ("x", "f8", 6), ("residual_norm", "f8"), ("half_period", "f8"), ("iterations", "i4"),
     ("n_event_searches", "i4"), ("n_stm", "i4"), ("converged", "i4"), ("error", "i4"),
     ("n_steps", "i4"), ("done", "i4"), ("reserved", "i4")], align=True)
 HH_CORRECT_INVALID = 4
 _JMAX = 64
+#: hh_correction_t.event_plane of a section coordinate (HH_PLANE_Y / _X / _Z)
+EVENT_PLANE = {Y: 0, X: 1, Z: 2}
 
 
 def line_search_steps(p: CorrectionProblem) -> int:
@@ -413,10 +430,11 @@ def line_search_steps(p: CorrectionProblem) -> int:
 
 def device_supported(p: CorrectionProblem) -> bool:
     """Whether hh_correct takes the problem: a square 2x2 system (Halo,
-    Lyapunov) with at most 64 line-search steps."""
+    Lyapunov, vertical) on an x, y or z section with at most 64 line-search
+    steps."""
     return (len(p.residual_indices) == 2 and len(p.control_indices) == 2
             and all(0 <= int(i) <= 5 for i in (*p.residual_indices, *p.control_indices))
-            and line_search_steps(p) <= _JMAX)
+            and int(p.event_index) in EVENT_PLANE and line_search_steps(p) <= _JMAX)
 
 
 def pack_problems(problems: Sequence[CorrectionProblem]) -> np.ndarray:
@@ -441,6 +459,7 @@ def pack_problems(problems: Sequence[CorrectionProblem]) -> np.ndarray:
         r["halo"] = int(p.extra_jacobian == "halo")
         r["forward"] = int(p.forward)
         r["max_attempts"] = int(p.max_attempts)
+        r["event_plane"] = EVENT_PLANE[int(p.event_index)]
     return out
 
 

@@ -4,8 +4,9 @@
  * loop (the row functions of hh_newton.h: corrector/backends/newton.py:
  * 39-143, stepping/armijo.py:58-164) without returning to the host:
  *
- *   residual   _y_plane_crossing (poincare/singlehit/backend.py:237-280):
- *              the first window, then the fallback window (alignment
+ *   residual   _y_plane_crossing, or _z_plane_crossing / _x_plane_crossing
+ *              as the problem's event plane asks (poincare/singlehit/
+ *              backend.py:237-280, 315-371): the first window, then the fallback window (alignment
  *              propagation + event search), in the Group6 layout -- the
  *              sixteen 8-lane groups of waves 0 and 1 evaluate the residual,
  *              or all line-search trials of a step at once (in chunks of
@@ -79,6 +80,7 @@ struct CorrLds {
     int steps;    /* accepted integration steps so far                           */
     int more;     /* 1 while the line search is not yet decided                  */
     int fail;     /* the problem is not one this kernel takes                    */
+    int ev_index; /* coordinate of the section plane: 0 x, 1 y, 2 z              */
 };
 
 /* The Halo term's powers, correctly rounded: squares of coordinates of
@@ -190,15 +192,19 @@ __global__ void k_eval_solve2(int n, const double* __restrict__ J, const double*
     cond[i] = corr_cond2(Ji);
 }
 
-/* _y_plane_crossing of one state in this lane's 8-lane group (y: this
- * lane's component): the first window, then the fallback one, each an
- * optional alignment propagation to the window's start and an event search
- * over its length (singlehit/backend.py:202-280).  ok, t and (this lane's
- * component of) x_event as the batched path forms them.                   */
+/* _y_plane_crossing (or the crossing of plane ev_index) of one state in
+ * this lane's 8-lane group (y: this lane's component): the first window,
+ * then the fallback one, each an optional alignment propagation to the
+ * window's start and an event search over its length (singlehit/
+ * backend.py:202-280).  ok, t and (this lane's component of) x_event as the
+ * batched path forms them.                                                 */
 template <bool CANCEL>
 __device__ __forceinline__ bool corr_residual(const Group6& ga, const Group6& ge,
-                                              const CrossArgs& a, double y, int& ok, double& t,
-                                              double& xe, int& steps, const int* cancel) {
+                                              const CrossArgs& a, int ev_index, double y,
+                                              int& ok, double& t, double& xe, int& steps,
+                                              const int* cancel) {
+    EventParams ev = a.ev;
+    ev.index = ev_index;
     ok = 0;
     t = NAN;
     xe = NAN;
@@ -214,7 +220,7 @@ __device__ __forceinline__ bool corr_residual(const Group6& ga, const Group6& ge
         }
         if (!a.searched[w]) continue;
         EventOut6 o;
-        group6_event<CANCEL>(ge, ya, 0.0, a.span[w], a.sp_event, a.ev, o, cancel);
+        group6_event<CANCEL>(ge, ya, 0.0, a.span[w], a.sp_event, ev, o, cancel);
         steps += o.nacc;
         if (o.status == ST_CANCELLED) return false;
         const double t_rel = o.found ? o.t_hit : a.span[w];
@@ -399,7 +405,11 @@ __global__ __launch_bounds__(192) void k_correct(CorrArgs a) {
     if (tid == 0) {
         const hh_correction_t& pb = a.prob[i];
         L.fail = 1;
-        if (pb.n_residuals == 2 && pb.n_controls == 2)
+        /* the section's coordinate: y (default), x or z */
+        L.ev_index = pb.event_plane == HH_PLANE_X ? 0 : (pb.event_plane == HH_PLANE_Z ? 2 : 1);
+        const bool plane_ok = pb.event_plane == HH_PLANE_Y || pb.event_plane == HH_PLANE_X ||
+                              pb.event_plane == HH_PLANE_Z;
+        if (pb.n_residuals == 2 && pb.n_controls == 2 && plane_ok)
             L.fail = hh::newton_init(L.row, 2, 2, pb.mu, pb.x0, pb.residual_indices,
                                      pb.control_indices, pb.target, pb.halo, pb.forward,
                                      pb.max_attempts, pb.tol, pb.max_delta, pb.alpha_reduction,
@@ -436,8 +446,8 @@ __global__ __launch_bounds__(192) void k_correct(CorrArgs a) {
                     }
                     int ok, steps;
                     double t, xe;
-                    const bool in = corr_residual<true>(ga, ge, a.x, y, ok, t, xe, steps,
-                                                        &L.decided);
+                    const bool in = corr_residual<true>(ga, ge, a.x, L.ev_index, y, ok, t,
+                                                        xe, steps, &L.decided);
                     if (in && ge.act) L.xe[j][ge.d] = xe;
                     if (ge.d == 0) {
                         atomicAdd(&L.steps, steps);
@@ -573,7 +583,7 @@ __global__ __launch_bounds__(BLOCK) void k_plane_crossing(int n, const double* _
     const double y = ge.act ? y0[(size_t)d * n + i] : 0.0;
     int okv, steps;
     double tv, xv;
-    corr_residual<false>(ga, ge, x, y, okv, tv, xv, steps, nullptr);
+    corr_residual<false>(ga, ge, x, x.ev.index, y, okv, tv, xv, steps, nullptr);
     if (ge.act) y_hit[(size_t)d * n + i] = xv;
     if (d == 0) {
         ok[i] = okv;

@@ -305,7 +305,8 @@ class _EventBatch(_Pending):
     poincare/singlehit/backend.py:237-280 in one hh_plane_crossing launch,
     one state per 8-lane group."""
 
-    def __init__(self, bank: _Bank, rows: np.ndarray, forward: int, slot: _Slot):
+    def __init__(self, bank: _Bank, rows: np.ndarray, forward: int, slot: _Slot,
+                 ev_index: int = Y_INDEX):
         super().__init__(slot)
         self.bank, self.rows, self.forward = bank, rows, forward
         states, counts = bank.event_inputs(rows)
@@ -318,7 +319,7 @@ class _EventBatch(_Pending):
         fin[0], fin[1:7] = mu, states.T
         pk = self.pk = _Packed(fin, 7, 2, slot)
         lib = _lib.load()
-        rc = lib.hh_plane_crossing(k, pk.din(0), pk.din(1), int(forward), Y_INDEX, 0.0, 0,
+        rc = lib.hh_plane_crossing(k, pk.din(0), pk.din(1), int(forward), int(ev_index), 0.0, 0,
                                    1e-12, 1e-12, ts, span, ts2, span2, pk.iout(0), pk.fout(0),
                                    pk.fout(1), pk.iout(1), slot.sh)
         _lib.check(rc, "hh_plane_crossing")
@@ -732,9 +733,10 @@ class Server:
             self.ready[key] = []
             st = bank.stages(rows)
             ev = rows[st != bank.JAC]
-            for f in np.unique(bank.fwd[ev]):
-                self.inflight.append(_EventBatch(bank, ev[bank.fwd[ev] == f], int(f),
-                                                 self._slot()))
+            fwd, evi = bank.fwd[ev], bank.evi[ev]
+            for f, e in sorted(set(zip(fwd.tolist(), evi.tolist()))):
+                self.inflight.append(_EventBatch(bank, ev[(fwd == f) & (evi == e)], int(f),
+                                                 self._slot(), ev_index=int(e)))
             jac = rows[st == bank.JAC]
             if jac.size:
                 self.inflight.append(_StmBatch(bank, jac, self._slot()))

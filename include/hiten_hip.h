@@ -343,9 +343,16 @@ int hh_bank_release(void* bank, int row);
  * result is written when its correction ends, its `done` set to 1 last with
  * a system-scope release, so a host polling mapped memory can take it
  * before the launch completes (clear `done` before the launch).  error:
- * HH_BANK_* or HH_CORRECT_INVALID (not 2x2, an index outside 0..5, or more
- * than 64 line-search steps).                                             */
+ * HH_BANK_* or HH_CORRECT_INVALID (not 2x2, an index outside 0..5, an
+ * unknown event plane, or more than 64 line-search steps).
+ * event_plane selects the section of the residual: HH_PLANE_Y (0, the
+ * default of a zero-initialised record: _y_plane_crossing, Halo and
+ * Lyapunov orbits, services/orbits.py:863, 1254), HH_PLANE_X (1) or
+ * HH_PLANE_Z (2: _z_plane_crossing, vertical orbits, orbits.py:1445).     */
 #define HH_CORRECT_INVALID 4
+#define HH_PLANE_Y 0
+#define HH_PLANE_X 1
+#define HH_PLANE_Z 2
 typedef struct {
     double mu;
     double x0[6];
@@ -354,7 +361,7 @@ typedef struct {
     int residual_indices[6]; /* entries 0..n_residuals-1 */
     int control_indices[6];  /* entries 0..n_controls-1  */
     int n_residuals, n_controls;
-    int halo, forward, max_attempts, reserved;
+    int halo, forward, max_attempts, event_plane;
 } hh_correction_t;
 
 typedef struct {

@@ -452,13 +452,13 @@ class CorrectorEngine:
     (corrector/operators.py:298-307: 500 output points, Phi at the last one).
     """
 
-    def event_batch(self, mu, states, forward):
+    def event_batch(self, mu, states, forward, index=1):
         n = states.shape[0]
         t = np.full(n, np.nan)
         xe = np.full((n, 6), np.nan)
         ok = np.zeros(n, bool)
         for j in range(n):
-            th, st = plane_crossing(float(mu[j]), states[j], forward=forward)
+            th, st = plane_crossing(float(mu[j]), states[j], forward=forward, index=index)
             if th is not None:
                 t[j], xe[j], ok[j] = th, st, True
         return t, xe, ok
@@ -533,7 +533,8 @@ def correct_reference(p, engine=None):
 
     def residual(q):
         counts["ev"] += 1
-        t, xe, ok = eng.event_batch(mu, full(q)[None], p.forward)
+        t, xe, ok = eng.event_batch(mu, full(q)[None], p.forward,
+                                    index=int(getattr(p, "event_index", 1)))
         if not ok[0]:
             raise _NoCross()
         return xe[0][res] - tgt, t[0], xe[0]

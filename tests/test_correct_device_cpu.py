@@ -80,3 +80,27 @@ def test_pack_problems_and_validation():
     assert not Cr.device_supported(Cr.halo_problem(mu, np.zeros(6), min_alpha=1e-30))
     with pytest.raises(ValueError):
         Cr.pack_problems([bad])
+    # the section plane: y by default (a zero record field), z for vertical orbits
+    v = Cr.vertical_problem(mu, [0.84, 0, 0, 0, 0.006, 0.1387])
+    assert (v.residual_indices, v.control_indices, v.event_index) == ((3, 1), (5, 4), 2)
+    rec = Cr.pack_problems(ps + [v])
+    assert rec["event_plane"].tolist() == [0, 0, 2]
+    assert not Cr.device_supported(Cr.halo_problem(mu, np.zeros(6), event_index=4))
+
+
+def test_vertical_orbit_on_the_oracle():
+    """The oracle's sequential loop on the z = 0 section converges to a
+    periodic vertical orbit of L1 (the closed orbit pins the section's
+    coordinate and the residual / control choice)."""
+    from oracle import oracle as O
+    mu = W.MU_EM
+    w = O.correct_reference(Cr.vertical_problem(mu, [0.84, 0, 0, 0, 0.006, 0.1387],
+                                                max_attempts=25))
+    assert w["converged"] and w["residual_norm"] < 1e-12
+    x = w["x"]
+    assert x[1] == x[2] == x[3] == 0.0 and x[5] > 0.1
+    r = O.propagate(mu, x, 0.0, 2 * w["half_period"], steps=2)[1][-1]
+    assert np.max(np.abs(r - x)) < 1e-9
+    # the orbit leaves the xy-plane: at a quarter period z is far from 0
+    q = O.propagate(mu, x, 0.0, 0.5 * w["half_period"], steps=2)[1][-1]
+    assert abs(q[2]) > 0.02

@@ -138,3 +138,49 @@ def test_invalid_record_and_done_flags(gpu):
     want = Cr.correct_device([ps[0], ps[2]])
     for g, w in zip(good, want):
         assert np.array_equal(g.x_corrected, w.x_corrected)
+
+
+def _vertical_problems():
+    # seeds about a vertical orbit of L1 (x = 0.84): converged, Newton out of
+    # iterations and Armijo failures, each on the z = 0 section
+    return [Cr.vertical_problem(MU, [x0, 0, 0, 0, vy, vz], max_attempts=25)
+            for x0, vy, vz in ((0.84, 0.006, 0.1387), (0.84, 0.0, 0.1387), (0.84, 0.02, 0.1687),
+                               (0.835, 0.0, 0.1387), (0.845, 0.0, 0.1387),
+                               (0.86, 0.0, 0.1187))]
+
+
+def test_vertical_orbits_on_the_z_section(gpu, oracle):
+    ps = _vertical_problems()
+    say(f"vertical orbits (_z_plane_crossing): {len(ps)} problems, device vs bank vs oracle")
+    got = Cr.correct_device(ps)
+    bank = Cr.correct_batch(ps)
+    kinds = set()
+    for p, g, b in zip(ps, got, bank):
+        w = oracle.correct_reference(p)
+        # no Halo term: the device, the bank and the reference loop compute
+        # the same bits
+        assert np.array_equal(g.x_corrected, b.x_corrected)
+        assert g.iterations == b.iterations == w["iterations"]
+        assert g.error == b.error and g.converged == b.converged == w["converged"]
+        assert np.array_equal(b.x_corrected, w["x"])
+        kinds.add((g.converged, g.error.split(" ")[0]))
+        if g.converged:
+            assert g.half_period == b.half_period == w["half_period"]
+            # a periodic orbit: the state returns after two half periods
+            r = oracle.propagate(MU, g.x_corrected, 0.0, 2 * g.half_period, steps=2)[1][-1]
+            assert np.max(np.abs(r - g.x_corrected)) < 1e-9
+    assert {(True, ""), (False, "Newton"), (False, "Armijo")} <= kinds
+
+
+def test_unknown_event_plane_is_refused(gpu):
+    say("hh_correct: a record with an unknown event plane is refused")
+    rec = Cr.pack_problems(_vertical_problems()[:2])
+    assert rec["event_plane"].tolist() == [2, 2]
+    rec[0]["event_plane"] = 7
+    dev = torch.device("cuda:0")
+    prob = torch.from_numpy(rec.view(np.uint8).copy()).to(dev)
+    res = torch.zeros(2 * Cr.RESULT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    Cr.launch_device(prob.data_ptr(), res.data_ptr(), 2)
+    out = res.cpu().numpy().view(Cr.RESULT_DTYPE)
+    assert out["done"].tolist() == [1, 1]
+    assert out["error"][0] == Cr.HH_CORRECT_INVALID and out["error"][1] != Cr.HH_CORRECT_INVALID

@@ -175,3 +175,23 @@ def test_continuations_in_the_mix_equal_continue_family(gpu, kind):
         _same_correction(j.result, d)
     summ = S.summarize(jobs, 1.0, server.steps)
     assert summ["continuation"]["n"] == len(conts)
+
+
+@pytest.mark.parametrize("kind", ["device", "bank"])
+def test_sections_of_different_planes_in_one_loop(gpu, kind):
+    """Vertical-orbit corrections (z = 0 section) beside Halo ones (y = 0):
+    the loop keeps each problem on its own section."""
+    ps = [Cr.vertical_problem(W.MU_EM, [0.84, 0, 0, 0, vy, vz], max_attempts=25)
+          for vy, vz in ((0.006, 0.1387), (0.0, 0.1187), (0.02, 0.1687))]
+    ps += [Cr.halo_problem(W.MU_EM, [0.8234, 0, 0.0224, 0, 0.1343, 0], max_attempts=20),
+           Cr.halo_problem(W.MU_EM, [0.84, 0, 0.1, 0, 0.22, 0], max_attempts=20)]
+    say(f"serving loop ({kind}): 3 vertical and 2 Halo corrections")
+    server = S.Server(corrector=kind)
+    jobs = [server.submit("correction", p, arrival=0.0) for p in ps]
+    while server.busy():
+        server.step()
+    server.close()
+    direct = Cr.correct_device(ps) if kind == "device" else Cr.correct_batch(ps)
+    for j, d in zip(jobs, direct):
+        _same_correction(j.result, d)
+    assert jobs[0].result.converged and jobs[3].result.converged
