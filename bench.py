@@ -76,10 +76,34 @@ def cpu_baseline(w, n_threads: int, target_s: float = 12.0):
         if dt >= target_s / 4 or m >= n:
             break
         m = min(n, int(m * max(2.0, min(16.0, target_s / max(dt, 1e-3)))))
+    # one core on a part of the same sample (every k-th of its trajectories,
+    # k = the thread count, so that it takes about as long as the all-core run)
+    sub = idx[::max(1, n_threads)]
+    t = time.perf_counter()
+    _, na1, _, _ = O.batch_dop853_final(mu[sub], w["y0"][sub], w["t0"][sub], w["tf"][sub],
+                                        max_step=1e4, max_steps=w.get("max_steps", 10_000_000),
+                                        n_threads=1)
+    dt1 = time.perf_counter() - t
     return {"value": steps / spent, "unit": "trajectory-steps/s", "cores": n_threads,
             "kind": "port",
             "sample": f"{used} of {n} trajectories (every {max(1, n // used)}th), "
-                      f"{steps} accepted steps in {spent:.2f} s"}
+                      f"{steps} accepted steps in {spent:.2f} s",
+            "single_core": {"value": int(na1.sum()) / dt1, "cores": 1,
+                            "sample": f"{len(sub)} trajectories of the sample, "
+                                      f"{int(na1.sum())} accepted steps in {dt1:.2f} s"},
+            "cpu_model": cpu_model()}
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or '' where it is not readable."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
 
 
 def serving_run(seconds: float, rate: float = 2000.0, continuation_fraction: float = 0.0):
