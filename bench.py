@@ -240,15 +240,21 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
         hits = int(res.count.to(torch.int64).sum().item())
         ls = sum(ev) / len(ev)
         nbytes = n * m * 48 + m * 8
+        # one wave walking each trajectory (hh_synodic_detect), same hits
+        one = synodic.detect_arrays(times, states, two_pass=False, **kw)
+        _, ev1 = _timed(one.relaunch, steps, warmup)
+        out["one_wave_per_trajectory"] = {"ms_per_launch": sum(ev1) / len(ev1) * 1e3,
+                                          "kernel": "k_synodic_detect"}
         out.update(metric="trajectory samples scanned/s, section detection (cubic, "
                           "segment_refine 20)", value=n * m * steps / wall,
                    unit="samples/s", ms_per_step=wall / steps * 1e3, hits_per_pass=hits,
                    config={"workload": f"synodic: {n} x {m} samples of 6-d DOP853 arcs to "
                                        f"t = {tf}, plane y = 0, cubic Hermite + Newton 25, "
-                                       "segment_refine 20", "kernel": "k_synodic_detect"},
+                                       "segment_refine 20",
+                           "kernel": "k_synodic_scan + k_synodic_merge (+ k_synodic_whole)"},
                    roofline={"bound": "hbm", "achieved": nbytes / ls / 1e9, "peak": 8000.0,
                              "unit": "GB/s", "frac": nbytes / ls / 1e9 / 8000.0,
-                             "traffic": profiled_traffic("k_synodic_detect", "synodic")})
+                             "traffic": profiled_traffic("k_synodic_scan", "synodic")})
         if not no_cpu:
             from oracle import synodic as OS
             host = states[:2].cpu().numpy()

@@ -66,6 +66,9 @@ _SIGNATURES = {
     "hh_dop853_final_workspace": ([_I, _I, _I], _LL),
     "hh_synodic_detect": ([_I, _I, _P, _I, _P, _P, _P, _D, _I, _I, _I, _I, _I, _D, _D, _D, _I,
                            _I, _I, _P, _P, _P, _P, _P], _I),
+    "hh_synodic_workspace": ([_I, _I], _LL),
+    "hh_synodic_detect_ws": ([_I, _I, _P, _I, _P, _P, _P, _D, _I, _I, _I, _I, _I, _D, _D, _D,
+                              _I, _I, _I, _P, _P, _P, _P, _P, _LL, _P], _I),
     "hh_dop853_grid": ([_I, _I, _P, _P, _P, _I, _P, _D, _D, _D, _D, _LL, _I, _ULL, _P, _P, _P, _P,
                         _P, _I, _I, _P], _I),
     "hh_dop853_event": ([_I, _P, _P, _P, _D, _D, _D, _D, _LL, _I, _D, _I, _D, _D, _I, _ULL, _P,

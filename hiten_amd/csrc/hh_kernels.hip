@@ -2310,6 +2310,79 @@ int hh_synodic_detect(int n, int m, const double* times, int per_traj_times,
     return check_launch("hh_synodic_detect launch");
 }
 
+static int syn_parts(int m) { return m >= 2 ? (m - 2) / SYN_PART + 1 : 1; }
+
+long long hh_synodic_workspace(int n, int m) {
+    if (n <= 0 || m < 0) return 0;
+    const long long units = (long long)n * syn_parts(m);
+    return ws_align(units * 4) + units * SYN_CAP * (long long)sizeof(int2);
+}
+
+int hh_synodic_detect_ws(int n, int m, const double* times, int per_traj_times,
+                         const int* lengths, const double* states, const double* normal,
+                         double offset, int direction, int plane_i, int plane_j, int cubic,
+                         int segment_refine, double tol_on_surface, double dedup_time_tol,
+                         double dedup_point_tol, int max_hits, int newton_max_iter, int cap,
+                         int* count, double* hit_t, double* hit_x, double* hit_p,
+                         void* workspace, long long workspace_bytes, void* stream) {
+    if (n < 0 || m < 0) return fail(HH_ERR_ARGS, "hh_synodic_detect_ws: n < 0 or m < 0");
+    if (n == 0) return HH_OK;
+    if (!workspace || workspace_bytes < hh_synodic_workspace(n, m))
+        return fail(HH_ERR_ARGS, "hh_synodic_detect_ws: workspace missing or smaller than "
+                                 "hh_synodic_workspace(n, m)");
+    if ((reinterpret_cast<unsigned long long>(workspace) & 255) != 0)
+        return fail(HH_ERR_ARGS, "hh_synodic_detect_ws: workspace must be 256-byte aligned");
+    /* the arguments are checked as hh_synodic_detect checks them */
+    if (!times || !states || !normal || !count)
+        return fail(HH_ERR_ARGS, "hh_synodic_detect_ws: null pointer");
+    if (cap < 0 || (cap > 0 && (!hit_t || !hit_x || !hit_p)))
+        return fail(HH_ERR_ARGS, "hh_synodic_detect_ws: cap < 0 or null hit arrays");
+    if (direction < -1 || direction > 1)
+        return fail(HH_ERR_ARGS, "hh_synodic_detect_ws: direction must be -1, 0 or 1");
+    if (plane_i < 0 || plane_i > 5 || plane_j < 0 || plane_j > 5)
+        return fail(HH_ERR_ARGS, "hh_synodic_detect_ws: plane indices must be 0..5");
+    if (segment_refine < 0 || newton_max_iter < 0 || max_hits < 0)
+        return fail(HH_ERR_ARGS, "hh_synodic_detect_ws: negative segment_refine, "
+                                 "newton_max_iter or max_hits");
+    SynArgs a;
+    a.n = n;
+    a.m = m;
+    a.times = times;
+    a.per_traj_times = per_traj_times ? 1 : 0;
+    a.lengths = lengths;
+    a.states = states;
+    for (int d = 0; d < 6; ++d) a.nrm[d] = normal[d];
+    a.offset = offset;
+    a.direction = direction;
+    a.pi = plane_i;
+    a.pj = plane_j;
+    a.cubic = cubic ? 1 : 0;
+    a.refine = segment_refine;
+    a.newton_max_iter = newton_max_iter;
+    a.max_hits = max_hits;
+    a.cap = cap;
+    a.tol_on = tol_on_surface;
+    a.dedup_t = dedup_time_tol;
+    a.dedup_p = dedup_point_tol;
+    a.count = count;
+    a.hit_t = hit_t;
+    a.hit_x = hit_x;
+    a.hit_p = hit_p;
+    SynWs w;
+    w.parts = syn_parts(m);
+    const long long units = (long long)n * w.parts;
+    w.nlisted = static_cast<int*>(workspace);
+    w.listed = reinterpret_cast<int2*>(static_cast<char*>(workspace) + ws_align(units * 4));
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_synodic_scan<256>, dim3(blocks_for(units * 64, 256)), dim3(256), 0, st,
+                       a, w);
+    hipLaunchKernelGGL(k_synodic_merge<256>, dim3(blocks_for((long long)n * 64, 256)), dim3(256),
+                       0, st, a, w);
+    hipLaunchKernelGGL(k_synodic_whole<256>, dim3(blocks_for((long long)n * 64, 256)), dim3(256),
+                       0, st, a, w);
+    return check_launch("hh_synodic_detect_ws launch");
+}
+
 int hh_traj_filters(int n, int m, const double* mu, const double* states, double* min_r1,
                     double* min_r2, double* max_energy_err, void* stream) {
     if (n < 0 || m < 1) return fail(HH_ERR_ARGS, "hh_traj_filters: n < 0 or m < 1");

@@ -292,17 +292,10 @@ __device__ __forceinline__ void syn_state(const SynTraj& tr, const SynSeg& g, do
     }
 }
 
-/* Four waves per SIMD: 4,096 trajectories are one wave each and all of them
- * are resident at once (1,024 SIMDs), which hides the chunk loads better
- * than two waves per SIMD without spills (measured 0.60 against 0.75 ms on
- * 4,096 x 4,713 samples, same bits), though the 128-VGPR budget spills 66
- * registers (268 B per lane of scratch).                                   */
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)))
-void k_synodic_detect(SynArgs a) {
-    const int i = (int)((blockIdx.x * (unsigned)BLOCK + threadIdx.x) >> 6);
-    const int lane = threadIdx.x & 63;
-    if (i >= a.n) return;
+/* Trajectory i, whole, by one wave: its segments 64 at a time (the next
+ * chunk's samples in flight while this one is processed), candidates
+ * ordered by a wave scan and deduplicated by one uniform loop.             */
+__device__ __forceinline__ void syn_detect_wave(SynArgs a, int i, int lane) {
     SynTraj tr;
     tr.a = &a;
     tr.base = (size_t)i * a.m;
@@ -407,6 +400,251 @@ void k_synodic_detect(SynArgs a) {
             last_u = u;
             last_v = v;
             if (a.max_hits > 0 && kept >= a.max_hits) done = true;
+        }
+    }
+    if (lane == 0) a.count[i] = kept;
+}
+
+/* Four waves per SIMD: 4,096 trajectories are one wave each and all of them
+ * are resident at once (1,024 SIMDs), which hides the chunk loads better
+ * than two waves per SIMD without spills (measured 0.60 against 0.75 ms on
+ * 4,096 x 4,713 samples, same bits), though the 128-VGPR budget spills 66
+ * registers (268 B per lane of scratch).                                   */
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void k_synodic_detect(SynArgs a) {
+    const int i = (int)((blockIdx.x * (unsigned)BLOCK + threadIdx.x) >> 6);
+    if (i >= a.n) return;
+    syn_detect_wave(a, i, threadIdx.x & 63);
+}
+
+/* time and state of candidate c of segment g (backend.py:345-360 / 591-606) */
+__device__ __forceinline__ void syn_candidate(const SynTraj& tr, const SynSeg& g, int c,
+                                              double& th, double (&x)[6]) {
+    int c2 = 0;
+    const double s = syn_walk(*tr.a, g, c, &c2);
+    if (s < 0.0) {
+        th = g.t0;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) x[d] = tr.X(g.k, d);
+    } else {
+        th = (1.0 - s) * g.t0 + s * g.t1;
+        syn_state(tr, g, s, x);
+    }
+}
+
+/* The two-pass form.  Pass 1 scans every part of SYN_PART segments of every
+ * trajectory with a wave of its own and lists the segments that hold
+ * candidates; pass 2 refines those segments' candidates and deduplicates
+ * them per trajectory.  Candidates come from the same segment values, walk
+ * and refinement as in syn_detect_wave, so the hits are the same bits.     */
+constexpr int SYN_PART = 512;     /* segments per part (8 chunks of 64)        */
+constexpr int SYN_CAP = 32;       /* listed segments per part                  */
+constexpr int SYN_REC = 7;        /* doubles of a refined candidate: t, x[6]   */
+constexpr int SYN_LDS_REC = 96;   /* refined candidates a wave stages in LDS   */
+
+struct SynWs {
+    int parts;                    /* parts per trajectory                      */
+    int* nlisted;                 /* [n][parts] segments with candidates       */
+    int2* listed;                 /* [n][parts][SYN_CAP] (segment, candidates) */
+};
+
+/* Four waves per SIMD fit without spills (110 VGPRs); 5 and 6 spill and
+ * measured no faster.                                                     */
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void k_synodic_scan(SynArgs a, SynWs w) {
+    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= (long long)a.n * w.parts) return;
+    const int i = (int)(wave / w.parts);
+    const int p = (int)(wave % w.parts);
+    SynTraj tr;
+    tr.a = &a;
+    tr.base = (size_t)i * a.m;
+    tr.tbase = a.per_traj_times ? (size_t)i * a.m : 0;
+    tr.N = a.lengths ? min(a.lengths[i], a.m) : a.m;
+    const int nseg = tr.N - 1;
+    const int s0 = p * SYN_PART;
+    const int s1 = min(s0 + SYN_PART, nseg);
+    int2* listed = w.listed + ((size_t)i * w.parts + p) * SYN_CAP;
+    int found = 0;
+    auto extra = [&](int k0) {
+        return (lane == 0) ? k0 - 1 : ((lane == 62) ? k0 + 64 : ((lane == 63) ? k0 + 65 : -1));
+    };
+    double xo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, to = 0.0;
+    double xe[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, te = 0.0;
+    if (s0 < s1) {
+        if (s0 + lane < tr.N) tr.load(s0 + lane, xo, to);
+        const int jx = extra(s0);
+        if (jx >= 0 && jx < tr.N) tr.load(jx, xe, te);
+    }
+    for (int k0 = s0; k0 < s1; k0 += 64) {
+        const int k = k0 + lane;
+        const double gj = (k < tr.N) ? tr.G(xo) : 0.0;
+        const double tj = (k < tr.N) ? to : 0.0;
+        const int jx = extra(k0);
+        const bool ex = jx >= 0 && jx < tr.N;
+        const double gx = ex ? tr.G(xe) : 0.0;
+        const double tx = ex ? te : 0.0;
+        if (k0 + 64 < s1) {
+            if (k + 64 < tr.N) tr.load(k + 64, xo, to);
+            const int jn = extra(k0 + 64);
+            if (jn >= 0 && jn < tr.N) tr.load(jn, xe, te);
+        }
+        const double g_up1 = __shfl(gj, (lane + 1) & 63), t_up1 = __shfl(tj, (lane + 1) & 63);
+        const double g_up2 = __shfl(gj, (lane + 2) & 63), t_up2 = __shfl(tj, (lane + 2) & 63);
+        const double g_dn1 = __shfl(gj, (lane + 63) & 63), t_dn1 = __shfl(tj, (lane + 63) & 63);
+        const double g_x62 = __shfl(gx, 62), t_x62 = __shfl(tx, 62);
+        const double g_x63 = __shfl(gx, 63), t_x63 = __shfl(tx, 63);
+        const double gk1 = (lane < 63) ? g_up1 : g_x62, tk1 = (lane < 63) ? t_up1 : t_x62;
+        const double gk2 = (lane < 62) ? g_up2 : ((lane == 62) ? g_x62 : g_x63);
+        const double tk2 = (lane < 62) ? t_up2 : ((lane == 62) ? t_x62 : t_x63);
+        const double gkm1 = (lane > 0) ? g_dn1 : gx, tkm1 = (lane > 0) ? t_dn1 : tx;
+        int cnt = 0;
+        if (k < s1) {
+            const SynSeg g = syn_segment(a, tr.N, k, gkm1, gj, gk1, gk2, tkm1, tj, tk1, tk2);
+            syn_walk(a, g, -1, &cnt);
+        }
+        const unsigned long long has = __ballot(cnt > 0);
+        if (cnt > 0) {
+            const int slot = found + __popcll(has & ((1ULL << lane) - 1ULL));
+            if (slot < SYN_CAP) listed[slot] = make_int2(k, cnt);
+        }
+        found += __popcll(has);
+    }
+    if (lane == 0) w.nlisted[(size_t)i * w.parts + p] = found;
+}
+
+/* A trajectory with a part of more than SYN_CAP listed segments: detected
+ * whole, by one wave, after the merge pass.                               */
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void k_synodic_whole(SynArgs a, SynWs w) {
+    const int i = (int)((blockIdx.x * (unsigned)BLOCK + threadIdx.x) >> 6);
+    if (i >= a.n) return;
+    const int* nl = w.nlisted + (size_t)i * w.parts;
+    bool over = false;
+    for (int p = 0; p < w.parts; ++p) over = over || nl[p] > SYN_CAP;
+    if (over) syn_detect_wave(a, i, threadIdx.x & 63);
+}
+
+/* Pass 2: one wave per trajectory.  Lane l rebuilds listed segment l of a
+ * round of 64 from its samples k-1 .. k+2 (syn_segment, as pass 1 did);
+ * the candidates are then refined by their lane and deduplicated in
+ * segment order against the last kept hit (backend.py:369-440), as in
+ * syn_detect_wave.  A trajectory with a part of more than SYN_CAP listed
+ * segments is left to k_synodic_whole.                                     */
+/* Three waves per SIMD (167 VGPRs, no spills); four spill and measured the
+ * same.                                                                    */
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3)))
+void k_synodic_merge(SynArgs a, SynWs w) {
+    const int i = (int)((blockIdx.x * (unsigned)BLOCK + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= a.n) return;
+    const int* nl = w.nlisted + (size_t)i * w.parts;
+    bool over = false;
+    for (int p = 0; p < w.parts; ++p) over = over || nl[p] > SYN_CAP;
+    if (over) return;                 /* k_synodic_whole detects it */
+    __shared__ double lds[BLOCK / 64][SYN_LDS_REC * SYN_REC];
+    double* rec = lds[threadIdx.x >> 6];
+    SynTraj tr;
+    tr.a = &a;
+    tr.base = (size_t)i * a.m;
+    tr.tbase = a.per_traj_times ? (size_t)i * a.m : 0;
+    tr.N = a.lengths ? min(a.lengths[i], a.m) : a.m;
+    int kept = 0;
+    bool done = false;
+    double last_t = 0.0, last_u = 0.0, last_v = 0.0;
+    /* the listed segments of all parts, in order, 64 per round: entry e of
+     * the trajectory is entry e - (entries of the parts before) of its part */
+    int np = 0;
+    for (int p = 0; p < w.parts; ++p) np += nl[p];
+    const int2* listed_all = w.listed + (size_t)i * w.parts * SYN_CAP;
+    {
+        for (int r0 = 0; r0 < np && !done; r0 += 64) {
+            const int e = r0 + lane;
+            SynSeg g;
+            int cnt = 0;
+            if (e < np) {
+                int part = 0, idx = e;
+                while (idx >= nl[part]) idx -= nl[part++];
+                const int2 ent = listed_all[(size_t)part * SYN_CAP + idx];
+                const int k = ent.x;
+                cnt = ent.y;
+                const double gk = tr.G(k), tk = tr.T(k);
+                const double gk1 = tr.G(k + 1), tk1 = tr.T(k + 1);
+                const bool m1 = k >= 1, p2 = k + 2 < tr.N;
+                const double gkm1 = m1 ? tr.G(k - 1) : 0.0, tkm1 = m1 ? tr.T(k - 1) : 0.0;
+                const double gk2 = p2 ? tr.G(k + 2) : 0.0, tk2 = p2 ? tr.T(k + 2) : 0.0;
+                g = syn_segment(a, tr.N, k, gkm1, gk, gk1, gk2, tkm1, tk, tk1, tk2);
+            }
+            int inc = cnt;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(inc, o);
+                if (lane >= o) inc += v;
+            }
+            const int exc = inc - cnt;
+            const int total = __shfl(inc, 63);
+            /* every lane refines its segment's candidates at once, into the
+             * wave's LDS records in segment order (a round with more than
+             * SYN_LDS_REC candidates refines them one at a time instead) */
+            const bool staged = total <= SYN_LDS_REC;
+            if (staged) {
+                for (int c = 0; c < cnt; ++c) {
+                    double th, x[6];
+                    syn_candidate(tr, g, c, th, x);
+                    double* r = rec + (size_t)(exc + c) * SYN_REC;
+                    r[0] = th;
+#pragma unroll
+                    for (int d = 0; d < 6; ++d) r[1 + d] = x[d];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (int j = 0; j < total && !done; ++j) {
+                double th = 0.0, x[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                if (staged) {
+                    const double* r = rec + (size_t)j * SYN_REC;
+                    th = r[0];
+#pragma unroll
+                    for (int d = 0; d < 6; ++d) x[d] = r[1 + d];
+                } else {
+                    const unsigned long long own = __ballot(exc <= j && j < inc);
+                    const int src = __ffsll((long long)own) - 1;
+                    if (lane == src) syn_candidate(tr, g, j - exc, th, x);
+                    th = __shfl(th, src);
+#pragma unroll
+                    for (int d = 0; d < 6; ++d) x[d] = __shfl(x[d], src);
+                }
+                double u = x[0], v = x[0];
+#pragma unroll
+                for (int d = 1; d < 6; ++d) {
+                    u = (a.pi == d) ? x[d] : u;
+                    v = (a.pj == d) ? x[d] : v;
+                }
+                if (kept > 0) {
+                    if (__builtin_fabs(th - last_t) <= a.dedup_t) continue;
+                    const double du = u - last_u, dv = v - last_v;
+                    if ((du * du + dv * dv) <= (a.dedup_p * a.dedup_p)) continue;
+                }
+                if (lane == 0 && kept < a.cap) {
+                    const size_t o = (size_t)i * a.cap + kept;
+                    a.hit_t[o] = th;
+#pragma unroll
+                    for (int d = 0; d < 6; ++d) a.hit_x[o * 6 + d] = x[d];
+                    a.hit_p[o * 2] = u;
+                    a.hit_p[o * 2 + 1] = v;
+                }
+                ++kept;
+                last_t = th;
+                last_u = u;
+                last_v = v;
+                if (a.max_hits > 0 && kept >= a.max_hits) done = true;
+            }
+            /* the records are read by every lane before the next round writes */
+            __builtin_amdgcn_wave_barrier();
         }
     }
     if (lane == 0) a.count[i] = kept;

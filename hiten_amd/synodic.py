@@ -6,7 +6,7 @@ trajectories (times, states), find where they cross the affine section
 n . x = offset, refine each crossing (linear, or cubic Hermite + Newton,
 optionally on segment_refine sub-intervals), filter by direction,
 deduplicate, and return the hits with their 2-d section points.  One
-``hh_synodic_detect`` launch handles a whole batch; trajectories stored by
+``hh_synodic_detect_ws`` call (two launches) handles a whole batch; trajectories stored by
 ``hh_dop853_grid`` ([n][m][6] on the device) are read where they lie.
 
 Parity: hit counts and times equal the reference's; hit states agree to an
@@ -104,11 +104,14 @@ def detect_arrays(times, states, *, normal, offset: float = 0.0, plane_coords=("
                   tol_on_surface: float = 1e-12, dedup_time_tol: float = 1e-9,
                   dedup_point_tol: float = 1e-12, max_hits_per_traj: Optional[int] = None,
                   newton_max_iter: int = 4, direction: Optional[int] = None, lengths=None,
-                  cap: int = 32, stream=None) -> HitArrays:
+                  cap: int = 32, stream=None, two_pass: bool = True) -> HitArrays:
     """Detection for n trajectories of m samples: ``states`` (n, m, 6) and
     ``times`` (m,) shared or (n, m), as NumPy arrays or device tensors (read
     in place).  Every hit is kept: when a trajectory has more than ``cap``
-    the call is repeated with room for all of them."""
+    the call is repeated with room for all of them.  ``two_pass`` scans the
+    parts of each trajectory with waves of their own and merges them
+    (hh_synodic_detect_ws); otherwise one wave walks each trajectory
+    (hh_synodic_detect).  Both give the same hits, bit for bit."""
     lib = _lib.load()
     dev = batch._device()
     st = torch.as_tensor(states, dtype=torch.float64, device=dev).contiguous()
@@ -130,15 +133,24 @@ def detect_arrays(times, states, *, normal, offset: float = 0.0, plane_coords=("
     if interp_kind not in ("linear", "cubic"):
         raise ValueError("interp_kind must be 'linear' or 'cubic'")
 
+    if two_pass:
+        # workspace of the two-pass form (torch's caching allocator: 256-byte aligned)
+        wsb = int(lib.hh_synodic_workspace(n, m))
+        ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
+
     def launch(out: HitArrays) -> None:
-        rc = lib.hh_synodic_detect(
-            n, m, batch._ptr(tm), int(per), batch._ptr(ln), batch._ptr(st),
-            nrm.ctypes.data, float(offset), dirn, pi, pj, int(interp_kind == "cubic"),
-            int(segment_refine), float(tol_on_surface), float(dedup_time_tol),
-            float(dedup_point_tol), int(max_hits_per_traj or 0), int(newton_max_iter),
-            int(out.t.shape[1]), batch._ptr(out.count), batch._ptr(out.t), batch._ptr(out.x),
-            batch._ptr(out.p), _lib.stream_handle(stream))
-        _lib.check(rc, "hh_synodic_detect")
+        args = (n, m, batch._ptr(tm), int(per), batch._ptr(ln), batch._ptr(st),
+                nrm.ctypes.data, float(offset), dirn, pi, pj, int(interp_kind == "cubic"),
+                int(segment_refine), float(tol_on_surface), float(dedup_time_tol),
+                float(dedup_point_tol), int(max_hits_per_traj or 0), int(newton_max_iter),
+                int(out.t.shape[1]), batch._ptr(out.count), batch._ptr(out.t),
+                batch._ptr(out.x), batch._ptr(out.p))
+        if two_pass:
+            rc = lib.hh_synodic_detect_ws(*args, ws.data_ptr(), wsb, _lib.stream_handle(stream))
+            _lib.check(rc, "hh_synodic_detect_ws")
+        else:
+            rc = lib.hh_synodic_detect(*args, _lib.stream_handle(stream))
+            _lib.check(rc, "hh_synodic_detect")
 
     while True:
         cap = max(int(cap), 1)

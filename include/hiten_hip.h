@@ -250,6 +250,22 @@ int hh_synodic_detect(int n, int m, const double* times, int per_traj_times,
                       double dedup_point_tol, int max_hits, int newton_max_iter, int cap,
                       int* count, double* hit_t, double* hit_x, double* hit_p, void* stream);
 
+/* hh_synodic_detect in two passes: every part of 512 segments of every
+ * trajectory is scanned by a wave of its own, which lists the segments
+ * holding candidates in the workspace; then one wave per trajectory refines
+ * and deduplicates their candidates in segment order -- the same hits, bit
+ * for bit.  A trajectory with more than 32 such segments in a part is
+ * detected whole by a third launch, as hh_synodic_detect does.  workspace: device memory of hh_synodic_workspace(n, m)
+ * bytes, 256-byte aligned, owned by the caller (no memory is allocated).  */
+long long hh_synodic_workspace(int n, int m);
+int hh_synodic_detect_ws(int n, int m, const double* times, int per_traj_times,
+                         const int* lengths, const double* states, const double* normal,
+                         double offset, int direction, int plane_i, int plane_j, int cubic,
+                         int segment_refine, double tol_on_surface, double dedup_time_tol,
+                         double dedup_point_tol, int max_hits, int newton_max_iter, int cap,
+                         int* count, double* hit_t, double* hit_x, double* hit_p,
+                         void* workspace, long long workspace_bytes, void* stream);
+
 /* Filters of a dense-output state grid states[n][m][6] (row-major per
  * trajectory), as the manifold computation applies them to every arc:
  * min_r1[i], min_r2[i] = minimum distance to the primaries, NaN if any

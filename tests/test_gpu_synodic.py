@@ -91,6 +91,51 @@ def test_matches_oracle_on_cr3bp_grids(gpu, opt, plane):
     assert total > 0
 
 
+@pytest.mark.parametrize("opt", OPTS, ids=[str(i) for i in range(len(OPTS))])
+def test_two_pass_equals_one_wave(gpu, opt):
+    """hh_synodic_detect_ws (parts scanned by waves of their own, then merged)
+    against hh_synodic_detect (one wave walks a trajectory): the same hits."""
+    say(f"synodic two-pass vs one wave per trajectory, 64 grids, {opt}")
+    t, states = _cr3bp_grid()
+    kw = dict(normal=synodic.axis_normal("y"), **opt)
+    a = synodic.hits_from_arrays(synodic.detect_arrays(t, states, two_pass=False, **kw))
+    b = synodic.hits_from_arrays(synodic.detect_arrays(t, states, two_pass=True, **kw))
+    assert [len(h) for h in a] == [len(h) for h in b]
+    for ha, hb in zip(a, b):
+        for u, v in zip(ha, hb):
+            assert u.time == v.time and np.array_equal(u.state, v.state)
+            assert np.array_equal(u.point2d, v.point2d)
+
+
+def test_two_pass_crowded_parts_fall_back(gpu):
+    """Trajectories crossing the plane every 10 samples list more than 32
+    segments in a part of 512: those are detected whole; the result is the
+    one-wave kernel's and the oracle's."""
+    say("synodic two-pass: crowded parts (51 crossings per 512 segments)")
+    m = 1500
+    t = np.linspace(0.0, 15.0, m)
+    n = 4
+    states = np.zeros((n, m, 6))
+    for i in range(n):
+        w = 2 * np.pi / (0.2 * (1.0 + 0.1 * i))          # a crossing every ~10 samples
+        states[i, :, 0] = 0.8 + 0.01 * np.cos(0.3 * t)
+        states[i, :, 1] = 0.01 * np.sin(w * t + 0.1 * i)
+        states[i, :, 4] = 0.01 * w * np.cos(w * t + 0.1 * i)
+    states[1, :, 1] = 0.01 + 0.001 * np.sin(t)            # never crosses
+    kw = dict(normal=synodic.axis_normal("y"), interp_kind="cubic", segment_refine=3,
+              newton_max_iter=10)
+    a = synodic.hits_from_arrays(synodic.detect_arrays(t, states, two_pass=False, **kw))
+    b = synodic.hits_from_arrays(synodic.detect_arrays(t, states, two_pass=True, **kw))
+    assert len(b[0]) > 100 and len(b[1]) == 0
+    for i in range(n):
+        want = S.detect(t, states[i], normal=synodic.axis_normal("y"), plane_idx=(1, 4),
+                        interp_kind="cubic", segment_refine=3, newton_max_iter=10)
+        assert len(a[i]) == len(b[i]) == len(want), i
+        for u, v, wv in zip(a[i], b[i], want):
+            assert u.time == v.time == wv[0]
+            assert np.array_equal(u.state, v.state)
+
+
 def test_per_trajectory_times_lengths_and_small_cap(gpu):
     say("hh_synodic_detect: per-trajectory times, ragged lengths, cap overflow re-run")
     t, states = _cr3bp_grid(n=16, m=800)
