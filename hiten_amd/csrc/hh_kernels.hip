@@ -2371,8 +2371,9 @@ int hh_synodic_detect_ws(int n, int m, const double* times, int per_traj_times,
     SynWs w;
     w.parts = syn_parts(m);
     const long long units = (long long)n * w.parts;
-    w.nlisted = static_cast<int*>(workspace);
-    w.listed = reinterpret_cast<int2*>(static_cast<char*>(workspace) + ws_align(units * 4));
+    char* base = static_cast<char*>(workspace);
+    w.nlisted = reinterpret_cast<int*>(base);
+    w.listed = reinterpret_cast<int2*>(base + ws_align(units * 4));
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(k_synodic_scan<256>, dim3(blocks_for(units * 64, 256)), dim3(256), 0, st,
                        a, w);

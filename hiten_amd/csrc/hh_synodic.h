@@ -449,16 +449,9 @@ struct SynWs {
     int2* listed;                 /* [n][parts][SYN_CAP] (segment, candidates) */
 };
 
-/* Four waves per SIMD fit without spills (110 VGPRs); 5 and 6 spill and
- * measured no faster.                                                     */
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)))
-void k_synodic_scan(SynArgs a, SynWs w) {
-    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    if (wave >= (long long)a.n * w.parts) return;
-    const int i = (int)(wave / w.parts);
-    const int p = (int)(wave % w.parts);
+/* Pass 1 for part p of trajectory i (one wave).                           */
+__device__ __forceinline__ void syn_scan_part(SynArgs a, const SynWs& w, int i, int p,
+                                              int lane) {
     SynTraj tr;
     tr.a = &a;
     tr.base = (size_t)i * a.m;
@@ -516,6 +509,16 @@ void k_synodic_scan(SynArgs a, SynWs w) {
     if (lane == 0) w.nlisted[(size_t)i * w.parts + p] = found;
 }
 
+/* Four waves per SIMD fit without spills (110 VGPRs); 5 and 6 spill and
+ * measured no faster.                                                     */
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void k_synodic_scan(SynArgs a, SynWs w) {
+    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    if (wave >= (long long)a.n * w.parts) return;
+    syn_scan_part(a, w, (int)(wave / w.parts), (int)(wave % w.parts), threadIdx.x & 63);
+}
+
 /* A trajectory with a part of more than SYN_CAP listed segments: detected
  * whole, by one wave, after the merge pass.                               */
 template <int BLOCK>
@@ -535,20 +538,12 @@ void k_synodic_whole(SynArgs a, SynWs w) {
  * segment order against the last kept hit (backend.py:369-440), as in
  * syn_detect_wave.  A trajectory with a part of more than SYN_CAP listed
  * segments is left to k_synodic_whole.                                     */
-/* Three waves per SIMD (167 VGPRs, no spills); four spill and measured the
- * same.                                                                    */
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3)))
-void k_synodic_merge(SynArgs a, SynWs w) {
-    const int i = (int)((blockIdx.x * (unsigned)BLOCK + threadIdx.x) >> 6);
-    const int lane = threadIdx.x & 63;
-    if (i >= a.n) return;
+__device__ __forceinline__ void syn_merge_traj(SynArgs a, const SynWs& w, int i, int lane,
+                                               double* rec) {
     const int* nl = w.nlisted + (size_t)i * w.parts;
     bool over = false;
     for (int p = 0; p < w.parts; ++p) over = over || nl[p] > SYN_CAP;
     if (over) return;                 /* k_synodic_whole detects it */
-    __shared__ double lds[BLOCK / 64][SYN_LDS_REC * SYN_REC];
-    double* rec = lds[threadIdx.x >> 6];
     SynTraj tr;
     tr.a = &a;
     tr.base = (size_t)i * a.m;
@@ -648,6 +643,17 @@ void k_synodic_merge(SynArgs a, SynWs w) {
         }
     }
     if (lane == 0) a.count[i] = kept;
+}
+
+/* Three waves per SIMD (167 VGPRs, no spills); four spill and measured the
+ * same.                                                                    */
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3)))
+void k_synodic_merge(SynArgs a, SynWs w) {
+    const int i = (int)((blockIdx.x * (unsigned)BLOCK + threadIdx.x) >> 6);
+    if (i >= a.n) return;
+    __shared__ double lds[BLOCK / 64][SYN_LDS_REC * SYN_REC];
+    syn_merge_traj(a, w, i, threadIdx.x & 63, lds[threadIdx.x >> 6]);
 }
 
 }  // namespace hh
