@@ -201,18 +201,34 @@ def result_tuple(r: "_corr.CorrectionResult", prediction) -> tuple:
             {"period": float(2.0 * r.half_period)})
 
 
+#: what a prediction whose correction raised counts as (pc.py:85-90)
+FAILED = (None, float("nan"), False, {})
+
+
+def make_problems(make_problem, predictions) -> list:
+    """The correction problem of each prediction, or None where building it
+    raised (the reference's corrector call raising, pc.py:85-90)."""
+    out = []
+    for pred in predictions:
+        try:
+            out.append(make_problem(pred))
+        except Exception:
+            out.append(None)
+    return out
+
+
 def _correct_all(problems, predictions, corrector):
-    """result_tuple per prediction.  A callable ``corrector`` is called on
-    the list of predictions instead."""
+    """result_tuple per prediction (FAILED where its problem is None).  A
+    callable ``corrector`` is called on the list of predictions instead."""
     if callable(corrector):
         return list(corrector(predictions))
-    if corrector == "device":
-        res = _corr.correct_device(problems)
-    elif corrector == "bank":
-        res = _corr.correct_batch(problems)
-    else:
+    if corrector not in ("device", "bank"):
         raise ValueError("corrector must be 'device' or 'bank'")
-    return [result_tuple(r, pred) for r, pred in zip(res, predictions)]
+    live = [p for p in problems if p is not None]
+    res = iter(_corr.correct_device(live) if corrector == "device"
+               else _corr.correct_batch(live))
+    return [FAILED if p is None else result_tuple(next(res), pred)
+            for p, pred in zip(problems, predictions)]
 
 
 def continue_families(seeds: Sequence, make_problem: Callable[[np.ndarray], "_corr.CorrectionProblem"],
@@ -242,7 +258,8 @@ def continue_families(seeds: Sequence, make_problem: Callable[[np.ndarray], "_co
         wins = [f.window(window) for f in live]
         preds = [p for w in wins for _, p in w]
         problems = ([] if callable(corrector) else
-                    [f.make_problem(p) for f, w in zip(live, wins) for _, p in w])
+                    [q for f, w in zip(live, wins) for q in make_problems(f.make_problem,
+                                                                         [p for _, p in w])])
         results = _correct_all(problems, preds, corrector)
         o = 0
         for f, w in zip(live, wins):

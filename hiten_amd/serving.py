@@ -503,11 +503,17 @@ class _ContRun:
         self.win = self.fam.window(int(self.job.request.window))
         self.res = [None] * len(self.win)
         self.fam.corrections += len(self.win)
-        for k, (_, pred) in enumerate(self.win):
-            sub = Job(-1, "correction", self.fam.make_problem(pred), server.now())
+        probs = _cont.make_problems(self.fam.make_problem, [pred for _, pred in self.win])
+        for k, p in enumerate(probs):
+            if p is None:                     # its correction raised: not converged
+                self.res[k] = _cont.FAILED
+                continue
+            sub = Job(-1, "correction", p, server.now())
             sub.admitted = sub.arrival
             sub.parent = (self, self.round, k)
             server._enqueue_correction(sub)
+        if self.fam.decided(self.res) and self not in server.cont_dirty:
+            server.cont_dirty.append(self)    # decided without a launch
 
     def on_result(self, rnd: int, k: int, result: CorrectionResult) -> bool:
         """Store a result; False if its round is over (or the family is)."""

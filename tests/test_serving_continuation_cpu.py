@@ -31,6 +31,7 @@ def _as_tuple(pred):
 class _FakeServer:
     def __init__(self):
         self.queue = []
+        self.cont_dirty = []
 
     def now(self):
         return 0.0
@@ -48,6 +49,10 @@ def _serve(req, rng):
     run.start_round(fake)
     done = False
     while not done:
+        if fake.cont_dirty:                   # decided without a launch
+            fake.cont_dirty = []
+            done = run.advance(fake)
+            continue
         assert fake.queue, "a continuation waits for results that never come"
         k = int(rng.integers(1, len(fake.queue) + 1))
         pick = sorted(rng.choice(len(fake.queue), size=k, replace=False), reverse=True)
@@ -133,3 +138,35 @@ def test_stream_with_continuations_keeps_the_two_kind_draws():
     assert p.max_attempts == 10 and p.tol == 1e-12
     with pytest.raises(ValueError):
         W.serving_stream(duration=0.01, continuation_fraction=0.2)
+
+
+def _raising_make(x):
+    """a problem builder that raises in a band of z (where the reference's
+    corrector call would raise, pc.py:85-90)"""
+    if 0.035 < float(x[2]) < 0.05:
+        raise ValueError("no problem here")
+    return np.asarray(x, float).copy()
+
+
+def _raising_corrector(pred):
+    _raising_make(pred)
+    return _as_tuple(pred)
+
+
+@pytest.mark.parametrize("window", [1, 3])
+def test_failing_problems_count_as_rejections(window):
+    seed = np.array([0.83, 0.0, 0.01, 0.0, 0.15, 0.0])
+    kw = dict(step=np.array([0.01]), state_indices=[2], stepper="natural",
+              max_retries_per_step=3)
+    fam, info = OC.run(seed, _raising_corrector, max_members=8, **kw)
+    req = S.ContinuationRequest(seed, _raising_make, np.array([0.01]), max_members=8,
+                                max_retries_per_step=3, state_indices=[2], window=window)
+    got = _serve(req, np.random.default_rng(window))
+    assert (got.accepted_count, got.rejected_count, got.iterations) == (
+        info["accepted_count"], info["rejected_count"], info["iterations"])
+    for a, b in zip(got.family, fam):
+        assert np.array_equal(a, b)
+    assert info["rejected_count"] > 0
+    # the batched API treats them the same way (problems built on the host)
+    probs = Ct.make_problems(_raising_make, [np.array([0, 0, 0.04, 0, 0, 0.0]), seed])
+    assert probs[0] is None and probs[1] is not None
