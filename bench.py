@@ -192,7 +192,7 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
                    roofline={"bound": "mfma", "achieved": fl / ls / 1e12,
                              "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                              "frac": fl / ls / 1e12 / FP64_PEAK_TFLOPS,
-                             "traffic": None})
+                             "traffic": profiled_traffic("k_rk_fixed6_group", "rk8_grid")})
         if not no_cpu:
             from oracle import oracle as O
             k = 256
