@@ -60,16 +60,24 @@ double or_pow(double x, double y) { return hh_pow(x, y); }
 /* vector fields                                                          */
 /* ---------------------------------------------------------------------- */
 
+/* Integer powers as numba lowers them (mode < 2: v*v and v*(v*v)) or as
+ * Python evaluates ** on NumPy scalars (mode 2: the C library's pow). */
+/* Called through a volatile pointer: gcc folds pow(v, 2.0) into v * v, which
+ * the C library's pow does not always equal. */
+static double (*volatile libm_pow)(double, double) = pow;
+static double sqm(double v, int mode) { return mode == 2 ? libm_pow(v, 2.0) : v * v; }
+static double cbm(double v, int mode) { return mode == 2 ? libm_pow(v, 3.0) : v * (v * v); }
+
 /* dynamics/rtbp.py:30-74, expression by expression. */
-void or_crtbp_accel(const double *st, double mu, double *out) {
+static void crtbp_accel_m(const double *st, double mu, int mode, double *out) {
     double x = st[0], y = st[1], z = st[2], vx = st[3], vy = st[4], vz = st[5];
     double xm = x + mu;
-    double r1 = sqrt((xm * xm + y * y) + z * z);                      /* :67 */
+    double r1 = sqrt((sqm(xm, mode) + sqm(y, mode)) + sqm(z, mode));  /* :67 */
     double one_mu = 1.0 - mu;
     double xr = x - one_mu;
-    double r2 = sqrt((xr * xr + y * y) + z * z);                      /* :68 */
-    double r1c = r1 * (r1 * r1);                                      /* r1**3 */
-    double r2c = r2 * (r2 * r2);
+    double r2 = sqrt((sqm(xr, mode) + sqm(y, mode)) + sqm(z, mode));  /* :68 */
+    double r1c = cbm(r1, mode);                                       /* r1**3 */
+    double r2c = cbm(r2, mode);
     double xm1 = (x - 1.0) + mu;                                      /* x - 1 + mu */
     double ax = ((2.0 * vy + x) - (one_mu * xm) / r1c) - (mu * xm1) / r2c;   /* :70 */
     double ay = ((-2.0 * vx + y) - (one_mu * y) / r1c) - (mu * y) / r2c;     /* :71 */
@@ -78,6 +86,8 @@ void or_crtbp_accel(const double *st, double mu, double *out) {
     out[3] = ax; out[4] = ay; out[5] = az;
 }
 
+void or_crtbp_accel(const double *st, double mu, double *out) { crtbp_accel_m(st, mu, 0, out); }
+
 static double p15(double x, int libm) { return libm ? pow(x, 1.5) : hh_pow15(x); }
 static double p25(double x, int libm) { return libm ? pow(x, 2.5) : hh_pow25(x); }
 
@@ -85,15 +95,15 @@ static double p25(double x, int libm) { return libm ? pow(x, 2.5) : hh_pow25(x);
 void or_jacobian_crtbp(double x, double y, double z, double mu, int libm, double *F) {
     double mu2 = 1.0 - mu;
     double xm = x + mu, xr = x - mu2;
-    double r2 = (xm * xm + y * y) + z * z;                            /* :117 */
-    double R2 = (xr * xr + y * y) + z * z;                            /* :118 */
+    double r2 = (sqm(xm, libm) + sqm(y, libm)) + sqm(z, libm);      /* :117 */
+    double R2 = (sqm(xr, libm) + sqm(y, libm)) + sqm(z, libm);      /* :118 */
     double r3 = p15(r2, libm), r5 = p25(r2, libm);
     double R3 = p15(R2, libm), R5 = p25(R2, libm);
-    double omgxx = ((1.0 + ((mu2 / r5) * 3.0) * (xm * xm)) + ((mu / R5) * 3.0) * (xr * xr))
+    double omgxx = ((1.0 + ((mu2 / r5) * 3.0) * sqm(xm, libm)) + ((mu / R5) * 3.0) * sqm(xr, libm))
                    - (mu2 / r3 + mu / R3);                            /* :124-127 */
-    double omgyy = ((1.0 + ((mu2 / r5) * 3.0) * (y * y)) + ((mu / R5) * 3.0) * (y * y))
+    double omgyy = ((1.0 + ((mu2 / r5) * 3.0) * sqm(y, libm)) + ((mu / R5) * 3.0) * sqm(y, libm))
                    - (mu2 / r3 + mu / R3);                            /* :129-132 */
-    double omgzz = ((0.0 + ((mu2 / r5) * 3.0) * (z * z)) + ((mu / R5) * 3.0) * (z * z))
+    double omgzz = ((0.0 + ((mu2 / r5) * 3.0) * sqm(z, libm)) + ((mu / R5) * 3.0) * sqm(z, libm))
                    - (mu2 / r3 + mu / R3);                            /* :134-137 */
     double omgxy = (3.0 * y) * ((mu2 * xm) / r5 + (mu * xr) / R5);    /* :139 */
     double omgxz = (3.0 * z) * ((mu2 * xm) / r5 + (mu * xr) / R5);    /* :140 */
@@ -120,8 +130,8 @@ void or_var_equations(const double *P, double mu, int libm, double *out) {
         }
     double mu2 = 1.0 - mu;
     double xm = x + mu, xr = x - mu2;
-    double r2 = (xm * xm + y * y) + z * z;                            /* :228 */
-    double R2 = (xr * xr + y * y) + z * z;
+    double r2 = (sqm(xm, libm) + sqm(y, libm)) + sqm(z, libm);      /* :228 */
+    double R2 = (sqm(xr, libm) + sqm(y, libm)) + sqm(z, libm);
     double r3 = p15(r2, libm), R3 = p15(R2, libm);                    /* :230-231 */
     double ax = ((x - mu2 * (xm / r3)) - mu * (xr / R3)) + 2.0 * vy;  /* :233-236 */
     double ay = ((y - mu2 * (y / r3)) - mu * (y / R3)) - 2.0 * vx;    /* :237-240 */
@@ -135,7 +145,7 @@ void or_rhs(or_sys *s, double t, const double *y, double *dy) {
     (void)t;
     s->nfev++;
     if (s->kind == OR_KIND_CR3BP6) {
-        or_crtbp_accel(y, s->mu, dy);
+        crtbp_accel_m(y, s->mu, s->libm_pow, dy);
     } else if (s->kind == OR_KIND_CR3BP42) {
         or_var_equations(y, s->mu, s->libm_pow, dy);
     } else {
@@ -177,6 +187,35 @@ double or_sumsq(const double *v, int dim) {
     return ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + p[6]);
 }
 
+/* x . x as NumPy's np.dot / np.linalg.norm form it on the reference's host
+ * when its source runs as plain Python (mode 2 only): OpenBLAS 0.3.29's
+ * ddot with the SkylakeX kernel this container's NumPy selects -- the first
+ * n & -16 terms in fused multiply-adds over 4 x 8 lanes (32 at a time), the
+ * rest of those 4 x 4 lanes, a fixed reduction tree, then the remaining
+ * terms one fused multiply-add each.  Matches np.dot bit for bit on random
+ * vectors of length 6 and 42 in this container. */
+static double sumsq_numpy(const double *v, int n) {
+    int n1 = n & -16, i = 0;
+    double a5[4][8] = {{0}}, acc[4][4] = {{0}};
+    for (; i < (n1 & ~31); i += 32)
+        for (int r = 0; r < 4; ++r)
+            for (int l = 0; l < 8; ++l) a5[r][l] = fma(v[i + 8 * r + l], v[i + 8 * r + l], a5[r][l]);
+    for (int r = 0; r < 4; ++r)
+        for (int l = 0; l < 4; ++l) acc[r][l] = a5[r][l] + a5[r][l + 4];
+    for (; i < n1; i += 8)
+        for (int r = 0; r < 2; ++r)
+            for (int l = 0; l < 4; ++l) acc[r][l] = fma(v[i + 4 * r + l], v[i + 4 * r + l], acc[r][l]);
+    double a[4];
+    for (int l = 0; l < 4; ++l) a[l] = ((acc[0][l] + acc[1][l]) + acc[2][l]) + acc[3][l];
+    double dot = (a[0] + a[2]) + (a[1] + a[3]);
+    for (int j = n1; j < n; ++j) dot = fma(v[j], v[j], dot);
+    return dot;
+}
+
+static double sumsq_m(const double *v, int n, int mode) {
+    return mode == 2 ? sumsq_numpy(v, n) : or_sumsq(v, n);
+}
+
 /* utils.py:13-39 */
 static int event_crossed(double gp, double gn, int dir) {
     if (dir == 0) return (gp < 0.0 && gn > 0.0) || (gp > 0.0 && gn < 0.0) || (gn == 0.0);
@@ -215,12 +254,14 @@ static double adjust_step_to_endpoint(double t, double h, double t_end) {
 }
 
 /* utils.py:215-255, order = 8 (DOP853 _p) */
-static double pi_accept_factor(double err, double err_prev, double order) {
+static double ctl_pow(double x, double y, int mode) { return mode == 2 ? libm_pow(x, y) : hh_pow(x, y); }
+
+static double pi_accept_factor(double err, double err_prev, double order, int mode) {
     double beta = 1.0 / (order + 1.0);
     double alpha = 0.4 * beta;
     double f;
-    if (err_prev < 0.0) f = (err == 0.0) ? 10.0 : 0.9 * hh_pow(err, -beta);
-    else f = (err == 0.0) ? 10.0 : (0.9 * hh_pow(err, -beta)) * hh_pow(err_prev, alpha);
+    if (err_prev < 0.0) f = (err == 0.0) ? 10.0 : 0.9 * ctl_pow(err, -beta, mode);
+    else f = (err == 0.0) ? 10.0 : (0.9 * ctl_pow(err, -beta, mode)) * ctl_pow(err_prev, alpha, mode);
     if (!(f == f)) f = 10.0;
     if (f < 0.2) f = 0.2;
     if (f > 10.0) f = 10.0;
@@ -228,11 +269,11 @@ static double pi_accept_factor(double err, double err_prev, double order) {
 }
 
 /* utils.py:258-287 */
-static double pi_reject_factor(double err, double order) {
+static double pi_reject_factor(double err, double order, int mode) {
     double err_exp = 1.0 / order;
     double f;
     if (err <= 0.0) f = 0.2;
-    else f = 0.9 * hh_pow(err, -err_exp);
+    else f = 0.9 * ctl_pow(err, -err_exp, mode);
     if (!(f == f)) f = 0.2;
     if (f < 0.2) f = 0.2;
     if (f > 10.0) f = 10.0;
@@ -447,16 +488,26 @@ static void dop853_step(or_sys *s, double t, const double *y, double h, double (
     for (int d = 0; d < n; ++d) { err5[d] *= h; err3[d] *= h; }
 }
 
+/* Diagnostic record of one adaptive attempt: (t, h) as the step is taken,
+ * its error norm and whether it was accepted. */
+static void trace_attempt(or_sys *s, double t, double h, double err, int accepted) {
+    if (s->trace && s->trace_len < s->trace_cap) {
+        double *r = s->trace + 4 * s->trace_len;
+        r[0] = t; r[1] = h; r[2] = err; r[3] = accepted ? 1.0 : 0.0;
+    }
+    s->trace_len++;
+}
+
 /* rk.py:2457-2467: the scaled error norm (note |h| applied a second time). */
 static double dop853_err_norm(int n, const double *y, const double *y_high, const double *err5,
-                              const double *err3, double h, double rtol, double atol) {
+                              const double *err3, double h, double rtol, double atol, int mode) {
     double e5s[DIM_MAX], e3s[DIM_MAX];
     for (int d = 0; d < n; ++d) {
         double sc = atol + rtol * np_max(fabs(y[d]), fabs(y_high[d]));
         e5s[d] = err5[d] / sc;
         e3s[d] = err3[d] / sc;
     }
-    double e5 = or_sumsq(e5s, n), e3 = or_sumsq(e3s, n);
+    double e5 = sumsq_m(e5s, n, mode), e3 = sumsq_m(e3s, n, mode);
     if (e5 == 0.0 && e3 == 0.0) return 0.0;
     double denom = e5 + 0.01 * e3;
     return (fabs(h) * e5) / sqrt(denom * (double)n);
@@ -464,7 +515,7 @@ static double dop853_err_norm(int n, const double *y, const double *y_high, cons
 
 /* rk.py:2443-2448 */
 static double dop853_initial_step(int n, const double *y, const double *f0, double rtol,
-                                  double atol, double min_step, double max_step) {
+                                  double atol, double min_step, double max_step, int mode) {
     double a[DIM_MAX], b[DIM_MAX];
     for (int d = 0; d < n; ++d) {
         double sc = atol + rtol * fabs(y[d]);
@@ -472,8 +523,8 @@ static double dop853_initial_step(int n, const double *y, const double *f0, doub
         b[d] = f0[d] / sc;
     }
     double sn = sqrt((double)n);
-    double d0 = sqrt(or_sumsq(a, n)) / sn;
-    double d1 = sqrt(or_sumsq(b, n)) / sn;
+    double d0 = sqrt(sumsq_m(a, n, mode)) / sn;
+    double d1 = sqrt(sumsq_m(b, n, mode)) / sn;
     return select_initial_step(d0, d1, min_step, max_step);
 }
 
@@ -594,7 +645,7 @@ int or_dop853(or_sys *s, const double *y0, const double *t_eval, int m, double r
     or_rhs(s, t, y, f0);
     int rc = nodes_push(&nd, t, y, f0, k, 0);
     if (rc) { nodes_free(&nd); return rc; }
-    double h = dop853_initial_step(n, y, f0, rtol, atol, min_step, max_step);
+    double h = dop853_initial_step(n, y, f0, rtol, atol, min_step, max_step, s->libm_pow);
     double err_prev = -1.0;
     long long nacc = 0, nrej = 0;
     int status = OR_OK;
@@ -603,7 +654,8 @@ int or_dop853(or_sys *s, const double *y0, const double *t_eval, int m, double r
         h = clamp_step(h, max_step, min_step);
         h = adjust_step_to_endpoint(t, h, tf);
         dop853_step(s, t, y, h, k, yh, e5, e3);
-        double err = dop853_err_norm(n, y, yh, e5, e3, h, rtol, atol);
+        double err = dop853_err_norm(n, y, yh, e5, e3, h, rtol, atol, s->libm_pow);
+        trace_attempt(s, t, h, err, err <= 1.0);
         if (err != err) { status = OR_ERR_NONFINITE; break; }
         if (err <= 1.0) {
             double t_new = t + h;
@@ -612,11 +664,11 @@ int or_dop853(or_sys *s, const double *y0, const double *t_eval, int m, double r
             if (rc) { nodes_free(&nd); return rc; }
             t = t_new;
             memcpy(y, yh, sizeof(double) * n);
-            h = h * pi_accept_factor(err, err_prev, 8.0);
+            h = h * pi_accept_factor(err, err_prev, 8.0, s->libm_pow);
             err_prev = err;
             nacc++;
         } else {
-            h = h * pi_reject_factor(err, 8.0);
+            h = h * pi_reject_factor(err, 8.0, s->libm_pow);
             h = clamp_step(h, max_step, min_step);
             nrej++;
         }
@@ -703,7 +755,7 @@ int or_dop853_event(or_sys *s, const double *y0, double t0, double tmax, double 
     memcpy(y, y0, sizeof(double) * n);
     or_rhs(s, t, y, fc);
     double gp = y[ev_index] - ev_offset;
-    double h = dop853_initial_step(n, y, fc, rtol, atol, min_step, max_step);
+    double h = dop853_initial_step(n, y, fc, rtol, atol, min_step, max_step, s->libm_pow);
     double err_prev = -1.0;
     long long nacc = 0, nrej = 0;
     int status = OR_OK;
@@ -713,7 +765,8 @@ int or_dop853_event(or_sys *s, const double *y0, double t0, double tmax, double 
         h = clamp_step(h, max_step, min_step);
         h = adjust_step_to_endpoint(t, h, tmax);
         dop853_step(s, t, y, h, k, yh, e5, e3);
-        double err = dop853_err_norm(n, y, yh, e5, e3, h, rtol, atol);
+        double err = dop853_err_norm(n, y, yh, e5, e3, h, rtol, atol, s->libm_pow);
+        trace_attempt(s, t, h, err, err <= 1.0);
         if (err != err) { status = OR_ERR_NONFINITE; break; }
         if (err <= 1.0) {
             double t_new = t + h;
@@ -732,10 +785,10 @@ int or_dop853_event(or_sys *s, const double *y0, double t0, double tmax, double 
             memcpy(y, yh, sizeof(double) * n);
             memcpy(fc, fn, sizeof(double) * n);
             gp = gn;
-            h = h * pi_accept_factor(err, err_prev, 8.0);
+            h = h * pi_accept_factor(err, err_prev, 8.0, s->libm_pow);
             err_prev = err;
         } else {
-            h = h * pi_reject_factor(err, 8.0);
+            h = h * pi_reject_factor(err, 8.0, s->libm_pow);
             h = clamp_step(h, max_step, min_step);
             nrej++;
         }
@@ -781,7 +834,7 @@ int or_batch_rk_final(int order, int n, const double *mu, const double *y0, doub
         } else {
 #pragma omp for schedule(dynamic, 4)
             for (int i = 0; i < n; ++i) {
-                or_sys s = {OR_KIND_CR3BP6, 6, mu[i], 1, 0, 0, 0, 0, 0};
+                or_sys s = {OR_KIND_CR3BP6, 6, mu[i], 1, 0, 0, 0, 0, 0, 0, 0, 0};
                 or_rk_fixed_order(order, &s, y0 + (size_t)i * 6, tv, steps, states, NULL);
                 memcpy(y_out + (size_t)i * 6, states + (size_t)(steps - 1) * 6, sizeof(double) * 6);
             }
@@ -801,7 +854,7 @@ int or_batch_dop853_final(int n, int dim, const double *mu, const double *y0, co
     double min_step = 10.0 * 2.220446049250313e-16;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int i = 0; i < n; ++i) {
-        or_sys s = {kind, dim, mu[i], 1, 0, 0, 0, 0, 0};
+        or_sys s = {kind, dim, mu[i], 1, 0, 0, 0, 0, 0, 0, 0, 0};
         double te[2] = {t0[i], tf[i]};
         double yo[2 * DIM_MAX];
         or_stats st = {0, 0, 0};
@@ -824,7 +877,7 @@ int or_batch_dop853_event(int n, const double *mu, const double *y0, const doubl
     double min_step = 10.0 * 2.220446049250313e-16;
 #pragma omp parallel for schedule(dynamic, 4)
     for (int i = 0; i < n; ++i) {
-        or_sys s = {OR_KIND_CR3BP6, 6, mu[i], 1, 0, 0, 0, 0, 0};
+        or_sys s = {OR_KIND_CR3BP6, 6, mu[i], 1, 0, 0, 0, 0, 0, 0, 0, 0};
         double ylast[6];
         or_stats st = {0, 0, 0};
         or_dop853_event(&s, y0 + (size_t)i * 6, 0.0, tmax[i], rtol, atol, max_step, min_step,
@@ -845,7 +898,7 @@ double or_dop853_single_step(or_sys *s, double t, const double *y, double h, dou
     dop853_step(s, t, y, h, k, y_high, err5, err3);
     if (k_out)
         for (int r = 0; r < DOP_S + 1; ++r) memcpy(k_out + r * s->dim, k[r], sizeof(double) * s->dim);
-    return dop853_err_norm(s->dim, y, y_high, err5, err3, h, rtol, atol);
+    return dop853_err_norm(s->dim, y, y_high, err5, err3, h, rtol, atol, s->libm_pow);
 }
 
 /* common/energy.py:51-54: r = (...)**0.5 (the correctly rounded root) */

@@ -43,10 +43,19 @@ typedef struct {
     double mu;            /* CR3BP mass parameter                             */
     int fwd;              /* _DirectedSystem direction: +1 or -1              */
     uint64_t flip;        /* bit i: negate component i when fwd == -1         */
-    int libm_pow;         /* 1: rhs42 uses C library pow (reference literal)  */
+    int libm_pow;         /* 0: correctly rounded x**1.5, x**2.5 and controller
+                           *    powers, x**2 -> x*x, x**3 -> x*(x*x) (numba);
+                           * 1: rhs42's x**1.5, x**2.5 through the C library
+                           *    pow (numba calls it for them);
+                           * 2: every ** through the C library pow, as the
+                           *    reference's source runs as plain Python on
+                           *    NumPy scalars (fields and step controller)   */
     const double *lin_a;  /* LINEAR: dy = lin_a (dim x dim) @ y + lin_b       */
     const double *lin_b;
     long long nfev;       /* right-hand side evaluations (diagnostic)         */
+    double *trace;        /* NULL, or trace_cap rows (t, h, err, accepted)    */
+    long long trace_cap;  /* of the adaptive drivers' attempts, in order      */
+    long long trace_len;  /* rows written (counts past trace_cap too)         */
 } or_sys;
 
 typedef struct {

@@ -42,6 +42,9 @@ class OrSys(C.Structure):
         ("lin_a", C.POINTER(C.c_double)),
         ("lin_b", C.POINTER(C.c_double)),
         ("nfev", C.c_longlong),
+        ("trace", C.POINTER(C.c_double)),
+        ("trace_cap", C.c_longlong),
+        ("trace_len", C.c_longlong),
     ]
 
 
@@ -134,7 +137,7 @@ class System:
     """
 
     def __init__(self, kind="cr3bp6", mu=0.0, dim=None, fwd=1, flip=None, libm_pow=False,
-                 A=None, b=None):
+                 A=None, b=None, trace_cap=0):
         self.kind = {"cr3bp6": KIND_CR3BP6, "cr3bp42": KIND_CR3BP42, "linear": KIND_LINEAR}[kind]
         if dim is None:
             dim = {KIND_CR3BP6: 6, KIND_CR3BP42: 42}.get(self.kind)
@@ -148,9 +151,25 @@ class System:
             for i in idx:
                 mask |= 1 << int(i)
         self.s = OrSys(self.kind, self.dim, float(mu), 1 if fwd >= 0 else -1, mask,
-                       1 if libm_pow else 0,
+                       int(libm_pow),
                        _p(self._A) if self._A is not None else None,
                        _p(self._b) if self._b is not None else None, 0)
+        self._trace = np.zeros((int(trace_cap), 4)) if trace_cap else None
+        if self._trace is not None:
+            self.s.trace = _p(self._trace)
+            self.s.trace_cap = int(trace_cap)
+
+    def trace(self):
+        """Rows (t, h, err, accepted) of every adaptive attempt since the last
+        call (a diagnostic of the drivers' step sequence); resets the record."""
+        if self._trace is None:
+            raise ValueError("System built without trace_cap")
+        n = int(self.s.trace_len)
+        if n > self._trace.shape[0]:
+            raise ValueError(f"trace overflow: {n} attempts, capacity {self._trace.shape[0]}")
+        out = self._trace[:n].copy()
+        self.s.trace_len = 0
+        return out
 
     def rhs(self, y, t=0.0):
         y = _f64(y)
@@ -310,16 +329,17 @@ def compute_stm(mu, x0, tf, steps=2000, forward=1, method="adaptive", order=8, *
     return PHI[:, 36:42], times, PHI[-1, :36].reshape(6, 6), PHI
 
 
-def _cross_event_driven(mu, state0, t0, tmax, forward, index=1, offset=0.0, direction=0):
+def _cross_event_driven(mu, state0, t0, tmax, forward, index=1, offset=0.0, direction=0,
+                        libm_pow=0):
     """poincare/singlehit/backend.py:164-235."""
     t_start = float(t0)
     if t_start <= 0.0:
         t_start = 1e-12
     _, st = propagate(mu, state0, 0.0, t_start, forward=forward, steps=2, method="adaptive",
-                      order=8)
+                      order=8, libm_pow=libm_pow)
     y_start = st[-1].copy()
     span = float(max(0.0, tmax - t_start))
-    sysm = System("cr3bp6", mu=mu)
+    sysm = System("cr3bp6", mu=mu, libm_pow=libm_pow)
     times = np.array([0.0, span])
     if np.isclose(times[0], times[-1]):
         t_rel, y_hit = span, y_start
@@ -334,11 +354,14 @@ def _cross_event_driven(mu, state0, t0, tmax, forward, index=1, offset=0.0, dire
 
 
 def plane_crossing(mu, x0, forward=1, t_guess=None, t_window=None, index=1, offset=0.0,
-                   direction=0, t0_offset=0.15):
+                   direction=0, t0_offset=0.15, libm_pow=0):
     """poincare/singlehit/backend.py:237-280 (_cross) as used by _y_plane_crossing.
 
-    Returns (t_hit, state) or (None, None).
+    Returns (t_hit, state) or (None, None).  ``libm_pow`` selects the
+    arithmetic of every integration (see ``System``).
     """
+    ced = lambda a, b: _cross_event_driven(mu, x0, a, b, forward, index, offset,  # noqa: E731
+                                           direction, libm_pow)
     x0 = _f64(x0)
     if t_guess is not None and t_window is not None and t_window > 0.0:
         t0 = float(max(t_guess - 0.5 * t_window, 0.0))
@@ -350,14 +373,13 @@ def plane_crossing(mu, x0, forward=1, t_guess=None, t_window=None, index=1, offs
         half = float(np.pi) * 0.5
         t0 = float(max(t_start - half, 0.0))
         tmax = float(t0 + 2.0 * half)
-    hit = _cross_event_driven(mu, x0, t0, tmax, forward, index, offset, direction)
+    hit = ced(t0, tmax)
     if hit is None and t_guess is not None:
         span = float(t_window if (t_window is not None and t_window > 0.0) else 0.1)
         while span < float(np.pi):
             span *= 2.0
             t0_try = float(max(t_guess - 0.5 * span, 0.0))
-            hit = _cross_event_driven(mu, x0, t0_try, t0_try + span, forward, index, offset,
-                                      direction)
+            hit = ced(t0_try, t0_try + span)
             if hit is not None:
                 break
     if hit is None:
@@ -367,8 +389,7 @@ def plane_crossing(mu, x0, forward=1, t_guess=None, t_window=None, index=1, offs
             t_start = float(np.pi / 2.0 - t0_offset)
             if t_start < 0.0:
                 t_start = 0.0
-        hit = _cross_event_driven(mu, x0, t_start, t_start + float(np.pi), forward, index,
-                                  offset, direction)
+        hit = ced(t_start, t_start + float(np.pi))
     return hit if hit is not None else (None, None)
 
 
