@@ -506,7 +506,7 @@ def main():
         },
         "accepted_steps_per_pass": steps_per_pass,
         "rejected_steps_per_pass": rejected,
-        "status_counts": {str(k): int((status == k).sum()) for k in (0, 1, 2)},
+        "status_counts": {str(k): int((status == k).sum()) for k in (0, 1, 2, 4)},
         "launch_s": launches,
         "plain": {
             "value": plain_value,

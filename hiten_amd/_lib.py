@@ -24,7 +24,7 @@ HH_VARIANT_PERSISTENT = 1
 HH_VARIANT_WIDE = 2
 HH_VARIANT_PAIR = 3
 HH_VARIANT_PERSISTENT_PAIR = 4
-HH_STATUS_OK, HH_STATUS_MAXSTEPS, HH_STATUS_NONFINITE = 0, 1, 2
+HH_STATUS_OK, HH_STATUS_MAXSTEPS, HH_STATUS_NONFINITE, HH_STATUS_DROPPED = 0, 1, 2, 4
 ABI_VERSION = 2
 
 

@@ -66,6 +66,8 @@ constexpr int ST_OK = 0;
 constexpr int ST_MAXSTEPS = 1;
 constexpr int ST_NONFINITE = 2;
 constexpr int ST_CANCELLED = 3;   /* stopped by its caller; the result is not used */
+constexpr int ST_DROPPED = 4;     /* lost by the persistent scheduler (ruled out by its
+                                     queue capacity; reported rather than silent)     */
 
 __device__ __forceinline__ double np_max(double a, double b) {
     /* numpy.maximum: NaN-propagating */

@@ -1249,9 +1249,15 @@ void k_dop853_final42_persist(Final42Args a) {
  * library on the stream before each launch):
  *   ctr[0] fresh-work counter        ctr[1] resume slots claimed (tail)
  *   ctr[2] resume slots taken (head) ctr[3] trajectories not yet finished
- *   ctr[4] spills that found no slot (an error; never happens for
- *          cap >= 15 n + 64: every pairing or wave-wide run finishes one
- *          trajectory and spills at most 15)
+ *   ctr[4] spills that found no slot (an error that the capacity rules
+ *          out; such a trajectory would end with status HH_STATUS_DROPPED)
+ *   ctr[5] slots taken by waves that hand their work away to sleep beside
+ *          a reserved SIMD (at most SLEEP_BUDGET(n); past it they keep
+ *          working instead)
+ *   Capacity: every pairing or wave-wide run finishes one trajectory and
+ *   spills at most 15 (the other 7 groups of its wave, 8 of the partner
+ *   wave), so those spills take at most 15 n slots; the sleepers' spills are
+ *   bounded by their own budget; cap = 15 n + 64 + SLEEP_BUDGET(n).
  *   rq[cap]        trajectory ids of the resume queue, -1 until written
  *   spill[89][n]   resumable state of a spilled trajectory: y (42, entry
  *                  6*b + d of Lane42 block b), k0 (42), t, h, err_prev, nacc,
@@ -1306,7 +1312,10 @@ struct SimdReserve {
     }
 };
 
-__host__ __device__ inline long long resume_cap(long long n) { return 15 * n + 64; }
+__host__ __device__ inline long long sleep_budget(long long n) { return 16 * n + 64; }
+__host__ __device__ inline long long resume_cap(long long n) {
+    return 15 * n + 64 + sleep_budget(n);
+}
 __host__ __device__ inline long long ws_align(long long bytes) { return (bytes + 255) / 256 * 256; }
 
 /* The trajectory a pair takes over, in Wave42's block/entry order */
@@ -1350,7 +1359,12 @@ __device__ __forceinline__ void spill42(const Final42Args& a, const ResumeArgs& 
         if (slot < r.cap) {
             __hip_atomic_store(&r.rq[slot], traj, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
+            /* no slot (ruled out by the capacity above): the trajectory ends
+             * here and says so instead of leaving its outputs unwritten */
             atomicAdd(&r.ctr[4], 1);
+            if (a.status) a.status[traj] = ST_DROPPED;
+            if (a.nacc) a.nacc[traj] = s.nacc;
+            if (a.nrej) a.nrej[traj] = s.nrej;
             atomicSub(&r.ctr[3], 1);
         }
     }
@@ -1521,9 +1535,24 @@ void k_dop853_final42_resume(Final42Args a, ResumeArgs r) {
         }
         /* a long run holds this SIMD: hand this wave's work to the others and
          * sleep until it is released (or a pair claim or the end wakes it) */
-        if (r.exclusive > 0 &&
+        bool sleep = r.exclusive > 0 &&
             __builtin_amdgcn_readfirstlane(__hip_atomic_load(&r.simd_tab[key], __ATOMIC_RELAXED,
-                                                             __HIP_MEMORY_SCOPE_AGENT)) != 0) {
+                                                             __HIP_MEMORY_SCOPE_AGENT)) != 0;
+        if (sleep && live) {
+            /* the slots this wave's spill takes come out of the sleepers'
+             * budget; with the budget spent the wave keeps its work */
+            const int cnt = __popcll(live);
+            int ok = 1;
+            if (lane == 0) {
+                const int old = atomicAdd(&r.ctr[5], cnt);
+                if ((long long)old + cnt > sleep_budget(a.n)) {
+                    atomicSub(&r.ctr[5], cnt);
+                    ok = 0;
+                }
+            }
+            sleep = __builtin_amdgcn_readfirstlane(__shfl(ok, 0)) != 0;
+        }
+        if (sleep) {
             spill42(a, r, traj >= 0, traj, b, grp, lane, s);
             traj = -1;
             while (__builtin_amdgcn_readfirstlane(

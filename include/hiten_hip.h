@@ -34,6 +34,10 @@ extern "C" {
 #define HH_STATUS_OK 0
 #define HH_STATUS_MAXSTEPS 1
 #define HH_STATUS_NONFINITE 2
+/* hh_dop853_final, variant HH_VARIANT_PERSISTENT_PAIR: a trajectory its
+ * scheduler could not requeue (the workspace capacity rules this out; the
+ * status exists so that such a loss is never silent).  y_out is undefined. */
+#define HH_STATUS_DROPPED 4
 
 #define HH_VARIANT_PLAIN 0
 #define HH_VARIANT_PERSISTENT 1

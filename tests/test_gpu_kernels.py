@@ -241,7 +241,7 @@ def test_dop853_42d_wave_wide_backward_flip(gpu, oracle, heavy_threshold, varian
 
 @pytest.mark.parametrize("heavy,pair,max_steps,exclusive", [
     (3, 0, 200_000, 0), (3, 3, 200_000, 0), (3, 3, 60, 0), (3, 60, 200_000, 0),
-    (0, 0, 200_000, 0), (3, 0, 200_000, 3), (3, 60, 200_000, 3)])
+    (0, 0, 200_000, 0), (3, 0, 200_000, 3), (3, 60, 200_000, 3), (3, 3, 200_000, 1)])
 def test_dop853_42d_persistent_pair_resume_queue(gpu, heavy_threshold, heavy, pair, max_steps,
                                                  exclusive):
     """Many moves: every trajectory flagged after 3 attempts sends the other
