@@ -19,6 +19,7 @@ import torch  # noqa: F401  (must precede the library load, see module docstring
 from .build import LIB
 
 HH_OK = 0
+HH_ERR_ARGS, HH_ERR_HIP = 1, 2
 HH_VARIANT_PLAIN = 0
 HH_VARIANT_PERSISTENT = 1
 HH_VARIANT_WIDE = 2
@@ -64,6 +65,8 @@ _SIGNATURES = {
     "hh_dop853_final": ([_I, _I, _P, _P, _P, _D, _D, _D, _D, _LL, _I, _ULL, _P, _P, _P, _P, _P,
                          _I, _I, _I, _P, _LL, _P, _P], _I),
     "hh_dop853_final_workspace": ([_I, _I, _I], _LL),
+    "hh_dop853_final_static": ([_I, _I, _P, _P, _P, _D, _D, _D, _D, _LL, _I, _ULL, _P, _P, _P, _P,
+                                _P, _I, _I, _I, _P, _P], _I),
     "hh_synodic_detect": ([_I, _I, _P, _I, _P, _P, _P, _D, _I, _I, _I, _I, _I, _D, _D, _D, _I,
                            _I, _I, _P, _P, _P, _P, _P], _I),
     "hh_synodic_workspace": ([_I, _I], _LL),

@@ -1312,11 +1312,16 @@ struct SimdReserve {
     }
 };
 
-__host__ __device__ inline long long sleep_budget(long long n) { return 16 * n + 64; }
-__host__ __device__ inline long long resume_cap(long long n) {
+__host__ __device__ constexpr long long sleep_budget(long long n) { return 16 * n + 64; }
+__host__ __device__ constexpr long long resume_cap(long long n) {
     return 15 * n + 64 + sleep_budget(n);
 }
-__host__ __device__ inline long long ws_align(long long bytes) { return (bytes + 255) / 256 * 256; }
+__host__ __device__ constexpr long long ws_align(long long bytes) { return (bytes + 255) / 256 * 256; }
+/* Workspace bytes of the pair-tail persistent form for n trajectories */
+__host__ __device__ constexpr long long pair_workspace(long long n) {
+    return ws_align((RQ_CTR + resume_cap(n)) * 4) + ws_align((long long)SPILL_W * n * 8) +
+           (long long)SIMD_TAB * 4;
+}
 
 /* The trajectory a pair takes over, in Wave42's block/entry order */
 struct PairHandoff {
@@ -1966,9 +1971,7 @@ static int resident_grid(const void* kernel, int block, int cap_per_cu, int* gri
 long long hh_dop853_final_workspace(int n, int dim, int variant) {
     if (n <= 0 || dim != 42) return 0;
     if (variant == HH_VARIANT_PERSISTENT) return 256;
-    if (variant == HH_VARIANT_PERSISTENT_PAIR)
-        return ws_align((RQ_CTR + resume_cap(n)) * 4) + ws_align((long long)SPILL_W * n * 8) +
-               (long long)SIMD_TAB * 4;
+    if (variant == HH_VARIANT_PERSISTENT_PAIR) return pair_workspace(n);
     return 0;
 }
 
@@ -2092,6 +2095,62 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
         default: hipLaunchKernelGGL(k_dop853_final42_persist<512>, dim3(grid), dim3(512), 0, st, a); break;
     }
     return check_launch("hh_dop853_final(42, persistent) launch");
+}
+
+/* ---- hh_dop853_final without a caller workspace ------------------------- */
+
+/* The workspace of the persistent forms, held by the library: the
+ * persistent form's counter and the pair-tail form's counters, resume queue,
+ * spill area and SIMD table for up to HH_STATIC_PAIR_MAX trajectories.
+ * Each GPU the library runs on holds its own copy until the process ends.
+ * In host code `scratch` names a host-side stand-in: the device address
+ * comes from hipGetSymbolAddress (static_scratch below), once per device.  */
+constexpr long long STATIC_SCRATCH_BYTES = pair_workspace(HH_STATIC_PAIR_MAX);
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+
+/* This device's address of `scratch` (looked up on the first call made with
+ * the device current, then kept per device).                              */
+static int static_scratch(void** out) {
+    constexpr int MAX_DEVICES = 64;
+    static void* ptr[MAX_DEVICES] = {};
+    static std::mutex mu;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(HH_ERR_ARGS, "device index beyond 63");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return fail(HH_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+        ptr[dev] = p;
+    }
+    *out = ptr[dev];
+    return HH_OK;
+}
+
+int hh_dop853_final_static(int n, int dim, const double* mu, const double* t0, const double* tf,
+                           double rtol, double atol, double max_step, double min_step,
+                           long long max_steps, int fwd, unsigned long long flip_mask,
+                           const double* y0, double* y_out, int* status, int* n_acc, int* n_rej,
+                           int variant, int block, int grid, const int* order, void* stream) {
+    if (n < 0) return fail(HH_ERR_ARGS, "hh_dop853_final_static: n < 0");
+    if (variant == HH_VARIANT_PERSISTENT_PAIR && n > HH_STATIC_PAIR_MAX)
+        return fail(HH_ERR_ARGS, "hh_dop853_final_static: the persistent pair variant takes at "
+                                 "most HH_STATIC_PAIR_MAX (16384) trajectories; use "
+                                 "hh_dop853_final with a workspace for more");
+    const long long need = hh_dop853_final_workspace(n, dim, variant);
+    if (need > STATIC_SCRATCH_BYTES)
+        return fail(HH_ERR_ARGS, "hh_dop853_final_static: workspace beyond the static scratch");
+    void* ws = nullptr;
+    if (need > 0) {
+        const int rc = static_scratch(&ws);
+        if (rc != HH_OK) return rc;
+    }
+    return hh_dop853_final(n, dim, mu, t0, tf, rtol, atol, max_step, min_step, max_steps, fwd,
+                           flip_mask, y0, y_out, status, n_acc, n_rej, variant, block, grid, ws,
+                           need > 0 ? STATIC_SCRATCH_BYTES : 0, order, stream);
 }
 
 int hh_dop853_grid(int n, int dim, const double* mu, const double* t0, const double* tf, int m,

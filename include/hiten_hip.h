@@ -183,6 +183,24 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
  * variants that need none.  No counterpart in the reference.              */
 long long hh_dop853_final_workspace(int n, int dim, int variant);
 
+/* hh_dop853_final without a workspace argument: the persistent variants take
+ * their counters, resume queue and spill area from a device array the
+ * library holds on each GPU (13,959,936 bytes: the pair-tail form's
+ * workspace for HH_STATIC_PAIR_MAX trajectories), and the call
+ * allocates no memory -- for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Every other argument, the variants and
+ * the results are hh_dop853_final's.  HH_VARIANT_PERSISTENT_PAIR takes at
+ * most HH_STATIC_PAIR_MAX trajectories (more: HH_ERR_ARGS); the persistent
+ * variant takes any n.  The array is reset on the stream before each
+ * launch; calls that overlap on one GPU share it and race.              */
+#define HH_STATIC_PAIR_MAX 16384
+int hh_dop853_final_static(int n, int dim, const double* mu, const double* t0, const double* tf,
+                           double rtol, double atol, double max_step, double min_step,
+                           long long max_steps, int fwd, unsigned long long flip_mask,
+                           const double* y0, double* y_out, int* status, int* n_acc, int* n_rej,
+                           int variant, int block, int grid, const int* order, void* stream);
+
 /* Adaptive DOP853 from t0[i] to tf[i] with dense output at the m points of
  * numpy.linspace(t0[i], tf[i], m), or at t_eval[0..m-1] when t_eval is given
  * (shared by all trajectories; then t0[i] = t_eval[0] and tf[i] =
