@@ -37,6 +37,19 @@ def profile_path(kernel: str, workload: str = "persistent_262k") -> str:
     return os.path.join(ROOT, "profiles", f"{kernel}__{workload}.md")
 
 
+def profiled_value(kernel: str, key: str, workload: str = "persistent_262k"):
+    """A `key = value` line of the committed rocprofv3 summary of ``kernel``
+    on ``workload`` (profiles/), or None without one."""
+    try:
+        with open(profile_path(kernel, workload)) as f:
+            for line in f:
+                if line.startswith(key + " ="):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def profiled_traffic(kernel: str, workload: str = "persistent_262k"):
     """HBM bytes per launch of ``kernel`` on ``workload`` from its committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated), or
@@ -106,24 +119,38 @@ def cpu_model() -> str:
     return ""
 
 
-def serving_run(seconds: float, rate: float = 2000.0, continuation_fraction: float = 0.0):
+def _halo_seeds():
+    """Corrected Halo orbits (from serving-distributed guesses, corrected
+    before any clock starts): seeds of continuation jobs and of the
+    converging correction stream."""
+    import numpy as np
+    from hiten_amd import corrector as Cr, workloads as W
+    rng = np.random.default_rng(W.SEED + 8)
+    guesses = [Cr.halo_problem(W.MU_EM, [rng.uniform(0.80, 0.88), 0,
+                                         rng.uniform(-0.2, 0.2), 0,
+                                         rng.uniform(-0.3, 0.3), 0], max_attempts=20)
+               for _ in range(64)]
+    return [r.x_corrected for r in Cr.correct_device(guesses) if r.converged][:16]
+
+
+def serving_run(seconds: float, rate: float = 2000.0, continuation_fraction: float = 0.0,
+                converging: bool = False):
     """The serving stream: Poisson arrivals at ``rate``, 70% corrections and
     30% manifold fans of 32-2048 trajectories, through hiten_amd.serving.
     With ``continuation_fraction`` that share of the corrections becomes
     continuation jobs instead (Halo families of 4 members, z step 1e-3,
-    retry window 2, seeded by orbits corrected before the clock starts)."""
-    import numpy as np
-    from hiten_amd import corrector as Cr, serving as S, workloads as W
+    retry window 2, seeded by orbits corrected before the clock starts).
+    With ``converging`` the corrections' guesses are corrected Halo orbits
+    moved by 1e-4..1e-3 in (x, z, vy), so that most converge to 1e-12 (the
+    default stream's guesses are the configuration's random states, most of
+    which do not converge within 10 iterations)."""
+    from hiten_amd import serving as S, workloads as W
     kw = {}
     if continuation_fraction > 0:
-        rng = np.random.default_rng(W.SEED + 8)
-        guesses = [Cr.halo_problem(W.MU_EM, [rng.uniform(0.80, 0.88), 0,
-                                             rng.uniform(-0.2, 0.2), 0,
-                                             rng.uniform(-0.3, 0.3), 0], max_attempts=20)
-                   for _ in range(64)]
-        seeds = [r.x_corrected for r in Cr.correct_device(guesses) if r.converged][:16]
         kw = dict(correction_fraction=0.7 - continuation_fraction,
-                  continuation_fraction=continuation_fraction, continuation_seeds=seeds)
+                  continuation_fraction=continuation_fraction, continuation_seeds=_halo_seeds())
+    if converging:
+        kw["correction_seeds"] = _halo_seeds()
     reqs = W.serving_stream(duration=seconds, rate=rate, **kw)
     server = S.Server()
     # warm the kernels and the pinned-memory pool on a few requests first
@@ -134,7 +161,11 @@ def serving_run(seconds: float, rate: float = 2000.0, continuation_fraction: flo
     server.close()
     out = S.summarize(jobs, wall, server.steps)
     out.update(requests=len(reqs), rate_per_s=rate, seconds_of_arrivals=seconds,
-               max_queue=server.max_queue, corrector=server.corrector)
+               max_queue=server.max_queue, corrector=server.corrector,
+               stream="converging guesses (corrected Halo orbits + 1e-4..1e-3)" if converging
+               else "random guesses (x ~ U(0.80, 0.88), z ~ U(-0.2, 0.2), vy ~ U(-0.3, 0.3))")
+    conv = out["correction"]["outcomes"].get("converged") if "correction" in out else None
+    out["converged_correction_p99_ms"] = conv["p99_ms"] if conv else None
     return out
 
 
@@ -189,7 +220,7 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
                    config={"workload": "rk8_grid: 1024 x 6-d, RK8, 2000 grid points to 2 pi, "
                                        "mu=0.012150585609624", "kernel": best,
                            "by_kernel": {k: v["value"] for k, v in res.items()}},
-                   roofline={"bound": "mfma", "achieved": fl / ls / 1e12,
+                   roofline={"bound": "fp64_valu", "achieved": fl / ls / 1e12,
                              "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                              "frac": fl / ls / 1e12 / FP64_PEAK_TFLOPS,
                              "traffic": profiled_traffic("k_rk_fixed6_group", "rk8_grid")})
@@ -219,7 +250,7 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
                    ms_per_step=ls * 1e3,
                    config={"workload": "stm_uneven: 4096 x 42-d, DOP853 rtol=atol=1e-12, "
                                        "t_f ~ U(2.5, 3.5)", "kernel": best},
-                   roofline={"bound": "mfma", "achieved": acc * FLOPS_PER_STEP_42 / ls / 1e12,
+                   roofline={"bound": "fp64_valu", "achieved": acc * FLOPS_PER_STEP_42 / ls / 1e12,
                              "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                              "frac": acc * FLOPS_PER_STEP_42 / ls / 1e12 / FP64_PEAK_TFLOPS,
                              "traffic": None},
@@ -348,7 +379,7 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
                    config={"workload": "manifold_fan: 64 seeds x 512 directions (32,768), "
                                        "DOP853 1e-12, y = 0 with vy < 0, bisection 1e-13, "
                                        "t_max ~ U(2, 8)", "kernel": f"k_dop853_event6 ({best})"},
-                   roofline={"bound": "mfma", "achieved": fl / ls / 1e12,
+                   roofline={"bound": "fp64_valu", "achieved": fl / ls / 1e12,
                              "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                              "frac": fl / ls / 1e12 / FP64_PEAK_TFLOPS, "traffic": None})
         if not no_cpu:
@@ -471,7 +502,12 @@ def main():
     same = bool(torch.equal(b.y, y_persist))
     plain_value = steps_per_pass * args.plain_steps / pwall if args.plain_steps else None
 
-    achieved = steps_per_pass * FLOPS_PER_STEP_42 / launch_s / 1e12
+    # every attempt, accepted or rejected, evaluates the same 12 stages
+    attempts = steps_per_pass + rejected
+    achieved = attempts * FLOPS_PER_STEP_42 / launch_s / 1e12
+    # FP64 FLOPs the VALU executed, counted by SQ_INSTS_VALU_FLOPS_FP64 (+ _TRANS)
+    # in the committed rocprofv3 counter run of this kernel on this batch
+    counted = profiled_value(kernel, "fp64_flop_per_launch")
     out = {
         "metric": METRIC,
         "value": value,
@@ -497,12 +533,18 @@ def main():
         },
         "autotune": tune_info,
         "roofline": {
-            "bound": "mfma",
+            "bound": "fp64_valu",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": profiled_traffic(kernel),
+            "work": "algorithmic: (accepted + rejected attempts) x 8,622 FLOP",
+            "achieved_accepted_only": steps_per_pass * FLOPS_PER_STEP_42 / launch_s / 1e12,
+            "counted_fp64_flop_per_launch": counted,
+            "counted_tflops": (counted / launch_s / 1e12) if counted else None,
+            "counted_frac": (counted / launch_s / 1e12 / FP64_PEAK_TFLOPS) if counted else None,
+            "valu_busy_frac": profiled_value(kernel, "valu_busy_frac"),
         },
         "accepted_steps_per_pass": steps_per_pass,
         "rejected_steps_per_pass": rejected,
@@ -517,6 +559,7 @@ def main():
     }
     if args.serve_seconds > 0:
         out["serving"] = serving_run(args.serve_seconds)
+        out["serving_converging"] = serving_run(args.serve_seconds, converging=True)
         out["serving_with_continuations"] = serving_run(args.serve_seconds,
                                                         continuation_fraction=0.1)
     if not args.no_cpu:

@@ -135,14 +135,19 @@ class _Family:
 
     def window(self, w: int):
         """The next w predictions of the retry chain (fewer if the retries
-        run out first): [(step used, prediction)]."""
+        run out first): [(step used, prediction, step after its rejection)].
+        The shrink policy is applied once per entry here, speculatively, and
+        consume() takes the steps from this chain, so the steps recorded are
+        the ones the corrections used (a policy must therefore depend only on
+        the step it is given, as the reference's default halving does)."""
         out, stp = [], self.step_vec
         last = self.family[-1]
         for k in range(w):
             if self.attempt + k > self.max_retries:
                 break
-            out.append((np.asarray(stp, dtype=float), self.predict(last, stp)))
-            stp = self.on_reject(stp)
+            nxt = self.on_reject(stp)
+            out.append((np.asarray(stp, dtype=float), self.predict(last, stp), nxt))
+            stp = nxt
         return out
 
     @staticmethod
@@ -159,7 +164,7 @@ class _Family:
 
     def consume(self, window, results) -> None:
         """Replay the sequential loop over a window's corrections."""
-        for (stp, prediction), res in zip(window, results):
+        for (stp, prediction, nxt), res in zip(window, results):
             self.iterations += 1
             corrected, res_norm, converged, aux = res
             if converged:
@@ -178,7 +183,7 @@ class _Family:
                 return
             self.rejected += 1
             self.attempt += 1
-            self.step_vec = self.on_reject(self.step_vec)
+            self.step_vec = nxt
             if self.attempt > self.max_retries:
                 self.failed = True
                 return
@@ -256,10 +261,10 @@ def continue_families(seeds: Sequence, make_problem: Callable[[np.ndarray], "_co
         if not live:
             break
         wins = [f.window(window) for f in live]
-        preds = [p for w in wins for _, p in w]
+        preds = [p for w in wins for _, p, _ in w]
         problems = ([] if callable(corrector) else
                     [q for f, w in zip(live, wins) for q in make_problems(f.make_problem,
-                                                                         [p for _, p in w])])
+                                                                         [p for _, p, _ in w])])
         results = _correct_all(problems, preds, corrector)
         o = 0
         for f, w in zip(live, wins):

@@ -503,7 +503,7 @@ class _ContRun:
         self.win = self.fam.window(int(self.job.request.window))
         self.res = [None] * len(self.win)
         self.fam.corrections += len(self.win)
-        probs = _cont.make_problems(self.fam.make_problem, [pred for _, pred in self.win])
+        probs = _cont.make_problems(self.fam.make_problem, [pred for _, pred, _ in self.win])
         for k, p in enumerate(probs):
             if p is None:                     # its correction raised: not converged
                 self.res[k] = _cont.FAILED
@@ -841,16 +841,45 @@ def to_request(r: dict):
     return "fan", FanRequest(r["mu"], np.asarray(r["y0"], float), np.asarray(r["tmax"], float))
 
 
+def correction_outcome(result) -> str:
+    """converged / max_iterations (Newton stopped at max_attempts) /
+    line_search (Armijo found no productive step) / no_crossing (no section
+    crossing for an iterate) / other, from a CorrectionResult."""
+    if result is None:
+        return "other"
+    if result.converged:
+        return "converged"
+    err = result.error or ""
+    if err.startswith("Newton did not converge"):
+        return "max_iterations"
+    if err.startswith("Armijo"):
+        return "line_search"
+    if err.startswith("no section crossing"):
+        return "no_crossing"
+    return "other"
+
+
+def _percentiles(lat_ms: np.ndarray) -> dict:
+    return dict(n=int(lat_ms.size), p50_ms=float(np.percentile(lat_ms, 50)),
+                p99_ms=float(np.percentile(lat_ms, 99)), max_ms=float(lat_ms.max()))
+
+
 def summarize(jobs: List[Job], wall: float, steps: int) -> dict:
-    """Latency percentiles (ms) per kind and overall, and throughput."""
+    """Latency percentiles (ms) per kind and overall, the corrections' by
+    outcome, and throughput."""
     out = {}
     for kind in ("correction", "continuation", "fan", None):
         lat = np.array([j.latency for j in jobs if kind is None or j.kind == kind]) * 1e3
         if lat.size == 0:
             continue
         key = kind or "all"
-        out[key] = dict(n=int(lat.size), p50_ms=float(np.percentile(lat, 50)),
-                        p99_ms=float(np.percentile(lat, 99)), max_ms=float(lat.max()))
+        out[key] = _percentiles(lat)
+    corr = [j for j in jobs if j.kind == "correction"]
+    if corr:
+        by = {}
+        for j in corr:
+            by.setdefault(correction_outcome(j.result), []).append(j.latency * 1e3)
+        out["correction"]["outcomes"] = {k: _percentiles(np.array(v)) for k, v in sorted(by.items())}
     out["trajectory_steps_per_s"] = steps / wall if wall > 0 else None
     out["wall_s"] = wall
     return out

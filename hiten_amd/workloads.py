@@ -122,13 +122,18 @@ def serving_stream(duration: float = 1.0, rate: float = 2000.0, seed: int = SEED
                    correction_fraction: float = 0.7, fan_min: int = 32, fan_max: int = 2048,
                    eps: float = 1e-6, continuation_fraction: float = 0.0,
                    continuation_seeds=None, members: int = 4, window: int = 2,
-                   z_step: float = 1e-3):
+                   z_step: float = 1e-3, correction_seeds=None,
+                   perturbation=(1e-4, 1e-3)):
     """Poisson request stream for the serving loop.  Draw order per request:
     inter-arrival time ~ Exp(rate), u ~ U(0, 1) for the kind (correction if
     u < ``correction_fraction``, continuation if below that plus
     ``continuation_fraction``, else fan), then
       correction:   x ~ U(0.80, 0.88), z ~ U(-0.2, 0.2), vy ~ U(-0.3, 0.3)
                     (a Halo-style single-shooting problem, <= 10 iterations);
+                    or, with ``correction_seeds`` (corrected Halo orbits), a
+                    seed drawn from them moved by a perturbation of size
+                    log-uniform in ``perturbation`` along a Gaussian
+                    direction in (x, z, vy): guesses that converge;
       continuation: a seed orbit drawn from ``continuation_seeds`` (corrected
                     Halo orbits), the sign of a natural-parameter step of
                     ``z_step`` in z; ``members`` members, retry window
@@ -150,6 +155,15 @@ def serving_stream(duration: float = 1.0, rate: float = 2000.0, seed: int = SEED
             break
         u = rng.uniform()
         if u < correction_fraction:
+            if correction_seeds is not None:
+                x0 = np.asarray(correction_seeds[int(rng.integers(len(correction_seeds)))],
+                                float).copy()
+                size = math.exp(rng.uniform(math.log(perturbation[0]),
+                                            math.log(perturbation[1])))
+                g = rng.normal(size=3)
+                x0[[0, 2, 4]] += size * g / np.linalg.norm(g)
+                out.append(dict(t=t, kind="correction", mu=MU_EM, x0=x0))
+                continue
             x = rng.uniform(0.80, 0.88)
             z = rng.uniform(-0.2, 0.2)
             vy = rng.uniform(-0.3, 0.3)

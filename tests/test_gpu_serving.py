@@ -114,6 +114,62 @@ def test_alone_equals_mixed(mixed_run):
     server.close()
 
 
+def test_alone_equals_mixed_full_config(gpu):
+    """0.5 s of the configuration's stream (2k req/s, 70% corrections, fans of
+    the full 32-2048 range) through the loop with the device corrector: every
+    correction equals correct_device on its problem, every fan equals its
+    event search alone, and a sample (the largest fans among them) equals the
+    same request run alone through the loop."""
+    reqs = W.serving_stream(duration=0.5, rate=2000.0)
+    sizes = [len(r["y0"]) for r in reqs if r["kind"] == "fan"]
+    say(f"serving loop (device): {len(reqs)} requests over 0.5 s, fans {min(sizes)}-{max(sizes)}")
+    assert max(sizes) > 1024 and min(sizes) < 64
+    server = S.Server(corrector="device")
+    jobs = server.run(reqs, time_limit=120.0)
+    assert all(not math.isnan(j.finished) for j in jobs)
+    corr = [j for j in jobs if j.kind == "correction"]
+    fans = [j for j in jobs if j.kind == "fan"]
+    for j, d in zip(corr, Cr.correct_device([j.request for j in corr])):
+        _same_correction(j.result, d)
+    for j in fans:
+        r = j.request
+        ev = batch.dop853_event(r.mu, r.y0, 0.0, r.tmax, ev_index=1, direction=-1,
+                                xtol=r.xtol, gtol=r.gtol)
+        _same_fan(j.result, S.FanResult(ev.hit.cpu().numpy(), ev.t_hit.cpu().numpy(),
+                                        ev.y_hit.cpu().numpy(), ev.n_accepted.cpu().numpy()))
+    big = sorted(range(len(reqs)), key=lambda i: -len(reqs[i].get("y0", ())))[:3]
+    pick = big + [i for i, r in enumerate(reqs) if r["kind"] == "correction"][::97]
+    say(f"alone vs mixed: requests {pick}")
+    for i in pick:
+        alone = server.run([dict(reqs[i], t=0.0)], time_limit=30.0)[0]
+        if alone.kind == "correction":
+            _same_correction(alone.result, jobs[i].result)
+        else:
+            _same_fan(alone.result, jobs[i].result)
+    server.close()
+    summ = S.summarize(jobs, 1.0, server.steps)
+    assert sum(v["n"] for v in summ["correction"]["outcomes"].values()) == len(corr)
+
+
+def test_converging_stream_converges(gpu):
+    """Guesses near corrected Halo orbits (the bench's converging stream)
+    converge to the tolerance, and the loop says so per outcome."""
+    guesses = [Cr.halo_problem(W.MU_EM, [0.8234, 0, 0.0224, 0, 0.1343, 0], max_attempts=20),
+               Cr.halo_problem(W.MU_EM, [0.84, 0, 0.1, 0, 0.22, 0], max_attempts=20)]
+    seeds = [r.x_corrected for r in Cr.correct_device(guesses)]
+    reqs = W.serving_stream(duration=0.05, rate=2000.0, fan_max=256, correction_seeds=seeds)
+    say(f"converging stream: {len(reqs)} requests")
+    server = S.Server(corrector="device")
+    jobs = server.run(reqs, time_limit=60.0)
+    server.close()
+    corr = [j for j in jobs if j.kind == "correction"]
+    conv = [j for j in corr if j.result.converged]
+    assert len(conv) >= 0.9 * len(corr)
+    assert all(j.result.residual_norm < 1e-12 for j in conv)
+    summ = S.summarize(jobs, 1.0, server.steps)
+    assert summ["correction"]["outcomes"]["converged"]["n"] == len(conv)
+
+
 def test_hbm_budget_admission(gpu):
     reqs = _stream()
     largest = max(S.Server.fan_bytes(len(r["y0"])) for r in reqs if r["kind"] == "fan")
