@@ -198,10 +198,18 @@ def solve_delta_stack(J: np.ndarray, r: np.ndarray, cond_threshold: float = 1e8)
 
 
 HH_BANK_MAX_ITERATIONS = 2
+HH_BANK_LINE_SEARCH = 3
 _ERRORS = {
     1: "no section crossing for the current state",
-    3: "Armijo line search failed to find a productive step.",
 }
+
+
+def _line_search_error(k: int) -> str:
+    """The reference's ConvergenceError text when the Armijo step finds no
+    productive step at Newton iteration k (backends/newton.py:125-130 around
+    stepping/armijo.py:160-164)."""
+    return (f"Step strategy failed to produce an update at iter {k}: "
+            "Armijo line search failed to find a productive step.")
 
 
 def _i32(a) -> np.ndarray:
@@ -348,6 +356,8 @@ class _Bank:
         if err.value == HH_BANK_MAX_ITERATIONS:
             msg = (f"Newton did not converge after {p.max_attempts} iterations "
                    f"(|R|={rn.value:.2e}).")
+        elif err.value == HH_BANK_LINE_SEARCH:
+            msg = _line_search_error(k.value)
         else:
             msg = _ERRORS.get(err.value, "")
         return CorrectionResult(bool(ok.value), x, rn.value, k.value, half_period=hp.value,
@@ -474,6 +484,8 @@ def results_from_records(rec: np.ndarray, problems: Sequence[CorrectionProblem]
                    f"(|R|={float(r['residual_norm']):.2e}).")
         elif err == HH_CORRECT_INVALID:
             msg = "invalid correction problem"
+        elif err == HH_BANK_LINE_SEARCH:
+            msg = _line_search_error(int(r["iterations"]))
         else:
             msg = _ERRORS.get(err, "")
         out.append(CorrectionResult(bool(r["converged"]), np.array(r["x"], dtype=float),
@@ -521,7 +533,5 @@ def correct(problem: CorrectionProblem, device: bool = False, **kw) -> Correctio
     ``device=True`` runs the whole loop in one launch (correct_device)."""
     r = correct_device([problem])[0] if device else correct_batch([problem], **kw)[0]
     if not r.converged:
-        if "Armijo" in r.error:
-            raise ConvergenceError(f"Step strategy failed to produce an update: {r.error}")
         raise ConvergenceError(r.error)
     return r

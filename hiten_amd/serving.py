@@ -852,7 +852,7 @@ def correction_outcome(result) -> str:
     err = result.error or ""
     if err.startswith("Newton did not converge"):
         return "max_iterations"
-    if err.startswith("Armijo"):
+    if "Armijo line search failed" in err:
         return "line_search"
     if err.startswith("no section crossing"):
         return "no_crossing"

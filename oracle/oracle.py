@@ -473,19 +473,23 @@ class CorrectorEngine:
     (corrector/operators.py:298-307: 500 output points, Phi at the last one).
     """
 
+    def __init__(self, libm_pow=0):
+        self.libm_pow = libm_pow       # arithmetic of every integration (see System)
+
     def event_batch(self, mu, states, forward, index=1):
         n = states.shape[0]
         t = np.full(n, np.nan)
         xe = np.full((n, 6), np.nan)
         ok = np.zeros(n, bool)
         for j in range(n):
-            th, st = plane_crossing(float(mu[j]), states[j], forward=forward, index=index)
+            th, st = plane_crossing(float(mu[j]), states[j], forward=forward, index=index,
+                                    libm_pow=self.libm_pow)
             if th is not None:
                 t[j], xe[j], ok[j] = th, st, True
         return t, xe, ok
 
     def stm_batch(self, mu, states, t_event):
-        return np.stack([compute_stm(float(m), s, te, steps=500)[2]
+        return np.stack([compute_stm(float(m), s, te, steps=500, libm_pow=self.libm_pow)[2]
                          for m, s, te in zip(mu, states, t_event)])
 
 
@@ -607,5 +611,7 @@ def correct_reference(p, engine=None):
             if best_alpha > 0:
                 x_new = best_x
             else:
-                return out(False, x, rn, k, "Armijo line search failed to find a productive step.")
+                return out(False, x, rn, k, "Step strategy failed to produce an update at iter "
+                                            f"{k}: Armijo line search failed to find a productive "
+                                            "step.")
         x = x_new

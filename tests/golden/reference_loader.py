@@ -120,7 +120,101 @@ def _stand_ins():
 
 _STAND_INS = ("hiten.utils.log_config", "hiten.algorithms.dynamics.hamiltonian",
               "hiten.algorithms.integrators.symplectic",
-              "hiten.algorithms.poincare.core.backend")
+              "hiten.algorithms.poincare.core.backend",
+              "hiten.algorithms.types.core (corrector only: the framework base classes "
+              "_HitenBaseBackend, whose lifecycle hooks return None as the reference's do, "
+              "_DomainPayload, _BackendCall and _HitenBaseInterface)")
+
+
+def _core_stand_in():
+    import dataclasses
+    from types import MappingProxyType
+    core = types.ModuleType("hiten.algorithms.types.core")
+
+    class _HitenBaseBackend:
+        def __init__(self, *a, **k):
+            pass
+
+        def on_iteration(self, k, x, r_norm):
+            return None
+
+        def on_accept(self, x, *, iterations, residual_norm):
+            return None
+
+        def on_failure(self, x, *, iterations, residual_norm):
+            return None
+
+        def on_success(self, x, *, iterations, residual_norm):
+            return None
+
+    @dataclasses.dataclass(frozen=True)
+    class _DomainPayload:
+        data: dict = dataclasses.field(default_factory=dict)
+
+        def __post_init__(self):
+            object.__setattr__(self, "data", MappingProxyType(dict(self.data)))
+
+    class _BackendCall:
+        def __init__(self, *a, **k):
+            self.args, self.kwargs = a, k
+
+    class _HitenBaseInterface:
+        def __init__(self, *a, **k):
+            pass
+
+    core._HitenBaseBackend = _HitenBaseBackend
+    core._DomainPayload = _DomainPayload
+    core._BackendCall = _BackendCall
+    core._HitenBaseInterface = _HitenBaseInterface
+    sys.modules[core.__name__] = core
+
+
+def load_corrector():
+    """load(singlehit=True) plus the Newton/Armijo corrector
+    (corrector/types.py, protocols.py, stepping/*.py, backends/base.py,
+    backends/newton.py, operators.py) and types/exceptions.py."""
+    m = load(singlehit=True)
+    for p in ("hiten.algorithms.corrector", "hiten.algorithms.corrector.stepping",
+              "hiten.algorithms.corrector.backends"):
+        _package(p)
+    _core_stand_in()
+    C = f"{REF}/corrector"
+    m["exceptions"] = _load("hiten.algorithms.types.exceptions", f"{REF}/types/exceptions.py")
+    m["ctypes"] = _load("hiten.algorithms.corrector.types", f"{C}/types.py")
+    m["cprotocols"] = _load("hiten.algorithms.corrector.protocols", f"{C}/protocols.py")
+    m["norm"] = _load("hiten.algorithms.corrector.stepping.norm", f"{C}/stepping/norm.py")
+    m["sbase"] = _load("hiten.algorithms.corrector.stepping.base", f"{C}/stepping/base.py")
+    m["plain"] = _load("hiten.algorithms.corrector.stepping.plain", f"{C}/stepping/plain.py")
+    m["armijo"] = _load("hiten.algorithms.corrector.stepping.armijo", f"{C}/stepping/armijo.py")
+    stepping = _load("hiten.algorithms.corrector.stepping", f"{C}/stepping/__init__.py")
+    m["stepping"] = stepping
+    m["cbase"] = _load("hiten.algorithms.corrector.backends.base", f"{C}/backends/base.py")
+    m["newton"] = _load("hiten.algorithms.corrector.backends.newton", f"{C}/backends/newton.py")
+    m["operators"] = _load("hiten.algorithms.corrector.operators", f"{C}/operators.py")
+    return m
+
+
+def reference_function(path, qualname, namespace):
+    """Compile one function of a reference file that cannot be imported
+    whole (its module pulls in the whole package): the function's own source
+    text, taken from the file at run time, executed in ``namespace``."""
+    import ast
+    src = open(path).read()
+    tree = ast.parse(src)
+    parts = qualname.split(".")
+    body = tree.body
+    node = None
+    for name in parts:
+        node = next(n for n in body if isinstance(n, (ast.ClassDef, ast.FunctionDef))
+                    and n.name == name)
+        body = node.body
+    fn_src = ast.get_source_segment(src, node)
+    lines = fn_src.splitlines()
+    indent = len(lines[0]) - len(lines[0].lstrip())
+    code = "\n".join(l[indent:] if l.strip() else "" for l in lines)
+    ns = dict(namespace)
+    exec(compile(code, path, "exec"), ns)
+    return ns[parts[-1]]
 
 
 def load(singlehit: bool = False):

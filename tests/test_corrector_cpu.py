@@ -53,4 +53,4 @@ def test_batched_corrector_equals_sequential_reference(oracle):
                                                        and math.isnan(w["half_period"]))
         outcomes.add((g.converged, g.error.split(" ")[0]))
     # converged, max-iterations and line-search failures are all exercised
-    assert (True, "") in outcomes and (False, "Newton") in outcomes and (False, "Armijo") in outcomes
+    assert (True, "") in outcomes and (False, "Newton") in outcomes and (False, "Step") in outcomes
