@@ -254,3 +254,30 @@ def load(singlehit: bool = False):
         m["singlehit"] = _load("hiten.algorithms.poincare.singlehit.backend",
                                f"{REF}/poincare/singlehit/backend.py")
     return m
+
+
+def load_continuation():
+    """load_corrector() plus the predictor-corrector continuation
+    (continuation/types.py, stepping/*.py, backends/base.py, backends/pc.py)."""
+    m = load_corrector()
+    for p in ("hiten.algorithms.continuation", "hiten.algorithms.continuation.stepping",
+              "hiten.algorithms.continuation.stepping.np",
+              "hiten.algorithms.continuation.stepping.sc",
+              "hiten.algorithms.continuation.backends"):
+        _package(p)
+    K = f"{REF}/continuation"
+    m["ktypes"] = _load("hiten.algorithms.continuation.types", f"{K}/types.py")
+    m["support"] = _load("hiten.algorithms.continuation.stepping.support",
+                         f"{K}/stepping/support.py")
+    m["kbase"] = _load("hiten.algorithms.continuation.stepping.base", f"{K}/stepping/base.py")
+    m["kplain"] = _load("hiten.algorithms.continuation.stepping.plain", f"{K}/stepping/plain.py")
+    m["np_step"] = _load("hiten.algorithms.continuation.stepping.np.base",
+                         f"{K}/stepping/np/base.py")
+    m["sc_step"] = _load("hiten.algorithms.continuation.stepping.sc.base",
+                         f"{K}/stepping/sc/base.py")
+    m["kstepping"] = _load("hiten.algorithms.continuation.stepping",
+                           f"{K}/stepping/__init__.py")
+    m["kbackend"] = _load("hiten.algorithms.continuation.backends.base", f"{K}/backends/base.py")
+    m["pc"] = _load("hiten.algorithms.continuation.backends.pc", f"{K}/backends/pc.py")
+    return m
+

@@ -8,17 +8,10 @@ import pytest
 
 from hiten_amd import continuation as Ct
 from oracle import continuation as OC
+from tests import golden_continuation_cases as GC
 
 
-def _mock(pred):
-    """Converges unless the first coordinate's 4th decimal digit is 7..9;
-    raises (a failed correction) when it is 6; the corrected state moves."""
-    pred = np.asarray(pred, dtype=float)
-    digit = int(abs(pred[0]) * 1e4) % 10
-    if digit == 6:
-        raise RuntimeError("no crossing")
-    corrected = pred + 1e-4 * np.sin(np.arange(6) + pred[0])
-    return corrected, float(np.linalg.norm(corrected - pred)), digit < 7, {"period": 3.0 + pred[0]}
+_mock = GC.mock
 
 
 def _batched_mock(preds):
@@ -31,17 +24,8 @@ def _batched_mock(preds):
     return out
 
 
-SEEDS = [np.array([0.8234, 0.0, 0.0224, 0.0, 0.1343, 0.0]),
-         np.array([0.84, 0.0, 0.1, 0.0, 0.22, 0.0]),
-         np.array([0.81, 0.0, -0.03, 0.0, 0.2, 0.0])]
-CASES = [
-    dict(step=np.array([1e-3]), state_indices=[0], stepper="natural"),
-    dict(step=np.array([3.3e-3, 1e-3]), state_indices=[0, 2], stepper="natural",
-         shrink_policy=lambda s: s * 0.3),
-    dict(step=np.array([2e-3]), state_indices=[2], stepper="secant"),
-    dict(step=np.full(6, 1e-3), state_indices=None, stepper="secant", step_min=5e-4),
-    dict(step=np.array([7e-3]), state_indices=[0], stepper="natural", max_retries_per_step=1),
-]
+SEEDS = GC.MOCK_SEEDS
+CASES = GC.MOCK_CASES
 
 
 @pytest.mark.parametrize("case", range(len(CASES)))

@@ -70,3 +70,28 @@ def test_rejections_shrink_the_step(gpu):
         info["accepted_count"], info["rejected_count"], info["iterations"])
     for a, b in zip(got.family, fam):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("kind", ["device", "bank"])
+def test_halo_families_against_the_reference_loop(gpu, kind):
+    """The batched continuation on the GPU correctors against the Halo
+    families the reference's own loop and corrector grew
+    (tests/golden/continuation_golden.npz): the same accepted, rejected and
+    iteration counts, members and aux periods to 1e-10 (the kernels compute
+    in numba's arithmetic, the vectors in Python's)."""
+    from tests import golden_continuation_cases as GC
+    from tests.test_golden_continuation import G, _fam
+    seeds = list(G["halo_seeds"])
+    for ci, case in enumerate(GC.HALO_CASES):
+        kw = {k: v for k, v in case.items() if k != "max_members"}
+        say(f"continuation ({kind}) vs the reference loop: case {ci}")
+        got = Ct.continue_families(seeds, _make, max_members=case["max_members"], window=2,
+                                   corrector=kind, **kw)
+        for si, g in enumerate(got):
+            ref = _fam(f"halo case {ci} seed {si}")
+            assert (g.accepted_count, g.rejected_count, g.iterations) == ref["counts"]
+            assert len(g.family) == len(ref["family"])
+            for a, b in zip(g.family, ref["family"]):
+                assert np.abs(np.asarray(a) - b).max() <= 1e-10
+            per = np.array([a["period"] for a in g.aux])
+            assert np.abs(per - ref["periods"]).max() <= 1e-10
