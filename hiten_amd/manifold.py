@@ -29,6 +29,7 @@ from typing import List, Literal, Optional, Tuple
 
 import numpy as np
 
+from .trajectory import Trajectory
 from . import batch
 from .dynamics import _compute_stm, variational_dynsys
 from .energy import traj_filters
@@ -194,10 +195,9 @@ class ManifoldResult:
                 self.attempts)
 
     @property
-    def trajectories(self) -> List[Tuple[np.ndarray, np.ndarray]]:
-        """(times, states) of each kept arc, as Manifold.compute wraps them in
-        Trajectory objects (system/manifold.py)."""
-        return list(zip(self.times_list, self.states_list))
+    def trajectories(self) -> List[Trajectory]:
+        """Each kept arc as a Trajectory (types/services/manifold.py:207-213)."""
+        return [Trajectory(t, x) for t, x in zip(self.times_list, self.states_list)]
 
     def to_df(self):
         """Manifold.to_df (system/manifold.py:373-387): one row per sample,

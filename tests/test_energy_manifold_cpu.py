@@ -133,7 +133,8 @@ def test_manifold_to_df_rows_and_columns(tmp_path):
     states = [rng.standard_normal((5, 6)), rng.standard_normal((3, 6))]
     res = ManifoldResult(states_list=states, times_list=times, successes=2, attempts=2)
     rows = []
-    for i, (t, x) in enumerate(res.trajectories):
+    for i, traj in enumerate(res.trajectories):
+        t, x = traj.times, traj.states
         for j in range(x.shape[0]):
             rows.append([i, t[j], *x[j, :6]])
     want = pd.DataFrame(rows, columns=["trajectory_id", "time", "x", "y", "z", "vx", "vy", "vz"])
@@ -145,3 +146,31 @@ def test_manifold_to_df_rows_and_columns(tmp_path):
     back = pd.read_csv(path)
     assert back.shape == (8, 8) and back["trajectory_id"].tolist() == [0] * 5 + [1] * 3
     assert ManifoldResult().to_df().shape == (0, 8)
+
+
+def test_trajectory_container_as_the_reference_defines_it():
+    """Trajectory (types/states.py:613-830): validation, views, (time, state)
+    access, slicing, reversal and from_solution without a copy."""
+    from hiten_amd.integrators import _Solution
+    from hiten_amd.trajectory import Trajectory
+    t = np.linspace(0.0, -1.0, 5)
+    x = np.arange(30.0).reshape(5, 6)
+    tr = Trajectory(t, x)
+    assert (tr.n_samples, tr.dim, len(tr)) == (5, 6, 5)
+    assert tr.t0 == 0.0 and tr.tf == -1.0 and tr.duration == -1.0
+    t2, x2 = tr[2]
+    assert t2 == t[2] and np.array_equal(x2, x[2])
+    sub = tr[1:4]
+    assert isinstance(sub, Trajectory) and np.array_equal(sub.times, t[1:4])
+    assert [p[0] for p in tr] == list(t)
+    rv = tr.reversed()
+    assert np.array_equal(rv.times, t[::-1]) and np.array_equal(rv.states, x[::-1])
+    sol = _Solution(t, x)
+    fs = Trajectory.from_solution(sol)
+    assert fs.times is not None and np.shares_memory(fs.states, x)
+    assert "decreasing" in repr(tr)
+    for bad in ((t[:1], x[:1]), (np.array([0.0, 1.0, 1.0]), x[:3]), (t, x[:4]), (t[None], x)):
+        with pytest.raises(ValueError):
+            Trajectory(*bad)
+    with pytest.raises(ValueError):
+        tr.vector_at(0)
