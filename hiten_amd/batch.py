@@ -429,12 +429,13 @@ def eval_rhs(mu, y, dim: int = 6, stream=None) -> torch.Tensor:
 
 
 def eval_math(x, y, stream=None) -> torch.Tensor:
-    """(5, n): sqrt(x), x/y, x**1.5, x**2.5, x**y as the kernels compute them."""
+    """(6, n): sqrt(x), x/y, x**1.5, x**2.5, x**y as the kernels compute them
+    and the kernels' own square root (hh_dev_sqrt) of x."""
     lib = _lib.load()
     xt = _vec(x, len(x), "x")
     yt = _vec(y, len(x), "y")
     n = xt.shape[0]
-    out = torch.empty((5, n), dtype=torch.float64, device=xt.device)
+    out = torch.empty((6, n), dtype=torch.float64, device=xt.device)
     _lib.check(lib.hh_eval_math(n, _ptr(xt), _ptr(yt), _ptr(out), _lib.stream_handle(stream)),
                "hh_eval_math")
     return out

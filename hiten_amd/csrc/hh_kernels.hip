@@ -1768,6 +1768,7 @@ __global__ void k_eval_math(int n, const double* __restrict__ x, const double* _
     out[(size_t)2 * n + i] = hh_pow15(a);
     out[(size_t)3 * n + i] = hh_pow25(a);
     out[(size_t)4 * n + i] = hh_pow(a, b);
+    out[(size_t)5 * n + i] = hh_dev_sqrt(a);
 }
 
 int check_launch(const char* what) {

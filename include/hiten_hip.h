@@ -245,7 +245,9 @@ int hh_eval_rhs(int n, int dim, const double* mu, const double* y, double* out, 
 
 /* Arithmetic probe used by the parity tests: out[0][i] = sqrt(x[i]),
  * out[1][i] = x[i] / y[i], out[2][i] = x[i]^1.5, out[3][i] = x[i]^2.5,
- * out[4][i] = x[i]^y[i] as the kernels compute them.  out : [5][n].
+ * out[4][i] = x[i]^y[i] as the kernels compute them, out[5][i] = the
+ * kernels' own square root of x[i] (the guarded lean form the stage loops
+ * use).  out : [6][n].
  * No counterpart in the reference (it checks the arithmetic the reference
  * obtains from libm: np.sqrt, '/', and '**' in rtbp.py / utils.py).       */
 int hh_eval_math(int n, const double* x, const double* y, double* out, void* stream);

@@ -13,6 +13,8 @@ continuation loop, restated literally, one prediction at a time.
 
 ``corrector(prediction)`` returns (corrected, residual, converged, aux) or
 raises; the loop is the checker of hiten_amd.continuation's batched form.
+Pinned by golden vectors the reference's own pc.py loop wrote
+(tests/test_golden_continuation.py): identical families, counters and steps.
 """
 from __future__ import annotations
 

@@ -29,9 +29,18 @@
  * rk.py:2452, 2762): an attempt count of max_steps stops with
  * OR_ERR_MAXSTEPS, and a NaN error norm stops with OR_ERR_NONFINITE; the
  * state returned is the last accepted one.
- * The step-size sequence (accept/reject pattern) of or_dop853 is pinned only
- * by this restatement of integrators/utils.py and rk.py:2443-2484: no golden
- * sequence exists outside the reference's numba runtime.
+ * Golden vectors written by the reference's own code (tests/golden/, read by
+ * tests/test_golden_integrators.py): with libm_pow = 2 -- every ** through
+ * the C library's pow and dot products in NumPy's OpenBLAS order, the
+ * arithmetic the reference's source has when it runs as plain Python --
+ * or_dop853 (final values, grids, forward and backward, 42-d and 6-d),
+ * or_dop853_event and or_rk_fixed reproduce the reference's outputs and
+ * every DOP853 attempt's t, h, error norm and accept/reject decision bit for
+ * bit, which pins the initial step, the PI controller, the clamps, the
+ * endpoint adjustment, the dense output and the event bisection.  In the
+ * default arithmetic (numba's lowering, libm_pow = 0) the step counts and
+ * accept/reject sequences are the same and the outputs agree to the
+ * rounding sensitivity of each case.
  */
 #include "hh_oracle.h"
 

@@ -12,6 +12,16 @@ in Python on top of the C integrators:
 * ``compute_stm``    dynamics/rtbp.py:258-340 (_compute_stm)
 * ``plane_crossing`` poincare/singlehit/backend.py:164-280
                      (_SingleHitBackend._cross / _cross_event_driven)
+* ``correct_reference`` corrector/backends/newton.py:39-143 with
+                     stepping/armijo.py:58-164 on the operators of
+                     corrector/operators.py:319-452
+
+Pins beyond the closed forms of tests/test_oracle.py: golden vectors written
+by the reference's own code (tests/golden/).  With ``libm_pow=2`` (the
+reference's source run as plain Python) ``propagate``, ``compute_stm``,
+``plane_crossing`` (both windows) and ``correct_reference`` reproduce them
+bit for bit, iteration counts and event-search counts included
+(tests/test_golden_integrators.py, tests/test_golden_corrector.py).
 """
 
 from __future__ import annotations

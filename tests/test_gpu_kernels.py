@@ -57,6 +57,41 @@ def test_math_probe_matches_cpu(gpu, oracle):
     assert_bitwise(out[2], np.array([oracle.pow15(v) for v in x]), "pow15")
     assert_bitwise(out[3], np.array([oracle.pow25(v) for v in x]), "pow25")
     assert_bitwise(out[4], np.array([oracle.powg(a, b) for a, b in zip(x, y)]), "pow")
+    assert_bitwise(out[5], np.sqrt(x), "hh_dev_sqrt")
+
+
+def test_device_sqrt_and_powers_over_the_whole_range(gpu, oracle):
+    """The kernels' square root (hh_dev_sqrt: lean form for normal x >=
+    2^-767, the compiler's sqrt below) and x**1.5, x**2.5 built on it, bit for
+    bit against the correctly rounded host results over the whole double
+    range: log-uniform over normal numbers, the 2^-767 switch-over and its
+    neighbours, near-collision squared distances, subnormals, zeros,
+    infinities and NaN."""
+    say("hh_eval_math: hh_dev_sqrt and powers over the whole range")
+    rng = np.random.default_rng(17)
+    n = 1 << 17
+    x = np.exp2(rng.uniform(-1022.0, 1023.0, n))
+    edge = np.ldexp(1.0, -767)
+    around = [edge]
+    for _ in range(64):
+        around.append(np.nextafter(around[-1], np.inf))
+    lo = [edge]
+    for _ in range(64):
+        lo.append(np.nextafter(lo[-1], 0.0))
+    near = np.exp(rng.uniform(math.log(1e-30), math.log(1e-8), 4096))  # near-collision r^2
+    specials = [0.0, -0.0, np.inf, -np.inf, np.nan, -1.0, 5e-324, 2.2250738585072014e-308,
+                1.7976931348623157e308, np.ldexp(1.0, -1074) * 3, np.ldexp(1.0, -1000)]
+    sub = np.ldexp(rng.uniform(1.0, 2.0, 1024), -1070)
+    x = np.concatenate([x, around, lo, near, specials, sub, np.ldexp(1.0, np.arange(-1074, 1024))])
+    y = np.full(x.size, 0.5)
+    out = batch.eval_math(x, y).cpu().numpy()
+    with np.errstate(invalid="ignore"):
+        want = np.sqrt(x)
+    assert np.array_equal(out[5], want, equal_nan=True), "hh_dev_sqrt"
+    assert np.array_equal(out[0], want, equal_nan=True), "sqrt"
+    fin = np.isfinite(x) & (x >= 0)
+    assert_bitwise(out[2][fin], np.array([oracle.pow15(v) for v in x[fin]]), "pow15")
+    assert_bitwise(out[3][fin], np.array([oracle.pow25(v) for v in x[fin]]), "pow25")
 
 
 @pytest.mark.parametrize("dim", [6, 42])
