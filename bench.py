@@ -453,7 +453,7 @@ def main():
     # or as given on the command line
     tune_info = None
     if args.variant == "auto":
-        res = autotune.tune_final(b, reps=1, max_trials=args.tune_trials, use_cache=False,
+        res = autotune.tune_final(b, reps=3, max_trials=args.tune_trials, use_cache=False,
                                   **step_kw)
         cfg = res.best
         tune_info = {"trials": len(res.trials),
