@@ -217,8 +217,8 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
         ls = res[best]["launch_s"]
         out.update(metric="FP64 trajectory-steps/s, fixed-step RK8 6-d", value=res[best]["value"],
                    ms_per_step=res[best]["wall"] / steps * 1e3,
-                   config={"workload": "rk8_grid: 1024 x 6-d, RK8, 2000 grid points to 2 pi, "
-                                       "mu=0.012150585609624", "kernel": best,
+                   config={"workload": "rk8_grid: 1024 x 6-d, RK8, 1999 steps of a 2000-point "
+                                       "grid to 2 pi, final state out, mu=0.012150585609624", "kernel": best,
                            "by_kernel": {k: v["value"] for k, v in res.items()}},
                    roofline={"bound": "fp64_valu", "achieved": fl / ls / 1e12,
                              "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -267,6 +267,9 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
         kw = dict(normal=synodic.axis_normal("y"), plane_coords=("y", "vy"),
                   interp_kind="cubic", segment_refine=20, newton_max_iter=25)
         res = synodic.detect_arrays(times, states, **kw)
+        # a pass takes about 0.3 ms: time at least 50 of them
+        steps = max(steps, 50)
+        out["steps"] = steps
         wall, ev = _timed(res.relaunch, steps, warmup)
         hits = int(res.count.to(torch.int64).sum().item())
         ls = sum(ev) / len(ev)
@@ -282,7 +285,7 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
                    config={"workload": f"synodic: {n} x {m} samples of 6-d DOP853 arcs to "
                                        f"t = {tf}, plane y = 0, cubic Hermite + Newton 25, "
                                        "segment_refine 20",
-                           "kernel": "k_synodic_scan + k_synodic_merge (+ k_synodic_whole)"},
+                           "kernel": "k_synodic_scan + k_synodic_refine + k_synodic_merge"},
                    roofline={"bound": "hbm", "achieved": nbytes / ls / 1e9, "peak": 8000.0,
                              "unit": "GB/s", "frac": nbytes / ls / 1e9 / 8000.0,
                              "traffic": profiled_traffic("k_synodic_scan", "synodic")})

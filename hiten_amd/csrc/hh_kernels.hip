@@ -2404,7 +2404,9 @@ static int syn_parts(int m) { return m >= 2 ? (m - 2) / SYN_PART + 1 : 1; }
 long long hh_synodic_workspace(int n, int m) {
     if (n <= 0 || m < 0) return 0;
     const long long units = (long long)n * syn_parts(m);
-    return ws_align(units * 4) + units * SYN_CAP * (long long)sizeof(int2);
+    const long long cap_list = (units + SYN_LISTS - 1) / SYN_LISTS * SYN_CAP;
+    return SYN_LISTS * 128 + ws_align(units * 4) + ws_align(SYN_LISTS * cap_list * 16) +
+           units * SYN_CAP * SYN_REC * (long long)sizeof(double);
 }
 
 int hh_synodic_detect_ws(int n, int m, const double* times, int per_traj_times,
@@ -2460,15 +2462,27 @@ int hh_synodic_detect_ws(int n, int m, const double* times, int per_traj_times,
     SynWs w;
     w.parts = syn_parts(m);
     const long long units = (long long)n * w.parts;
+    if (units * SYN_CAP > 0x7fffffffLL)
+        return fail(HH_ERR_ARGS, "hh_synodic_detect_ws: n * parts too large");
+    const long long cap_list = (units + SYN_LISTS - 1) / SYN_LISTS * SYN_CAP;
+    w.cap_list = (int)cap_list;
     char* base = static_cast<char*>(workspace);
-    w.nlisted = reinterpret_cast<int*>(base);
-    w.listed = reinterpret_cast<int2*>(base + ws_align(units * 4));
+    w.nlist = reinterpret_cast<int*>(base);
+    w.ncand = reinterpret_cast<int*>(base + SYN_LISTS * 128);
+    w.list = reinterpret_cast<int4*>(base + SYN_LISTS * 128 + ws_align(units * 4));
+    w.rec = reinterpret_cast<double*>(base + SYN_LISTS * 128 + ws_align(units * 4) +
+                                      ws_align(SYN_LISTS * cap_list * 16));
     hipStream_t st = (hipStream_t)stream;
+    /* the list counters are cleared by a kernel, so the sequence can be
+     * captured in a graph */
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(SYN_LISTS * 32), 0, st, w.nlist, 0,
+                       SYN_LISTS * 32);
     hipLaunchKernelGGL(k_synodic_scan<256>, dim3(blocks_for(units * 64, 256)), dim3(256), 0, st,
                        a, w);
+    hipLaunchKernelGGL(k_synodic_refine<256>,
+                       dim3(min(blocks_for(units * SYN_CAP, 256), 256)), dim3(256), 0, st,
+                       a, w);
     hipLaunchKernelGGL(k_synodic_merge<256>, dim3(blocks_for((long long)n * 64, 256)), dim3(256),
-                       0, st, a, w);
-    hipLaunchKernelGGL(k_synodic_whole<256>, dim3(blocks_for((long long)n * 64, 256)), dim3(256),
                        0, st, a, w);
     return check_launch("hh_synodic_detect_ws launch");
 }

@@ -28,6 +28,7 @@
 #define HH_SYNODIC_H
 
 #include "hh_device.h"
+#include "hh_wave.h"
 
 namespace hh {
 
@@ -265,6 +266,78 @@ __device__ __forceinline__ double syn_walk(const SynArgs& a, const SynSeg& g, in
     return 0.0;
 }
 
+/* syn_walk for one candidate (want >= 0) with the sub-interval located
+ * first and refined after the walk, the same operations in the same order:
+ * lanes of a wave refining candidates of different segments then run one
+ * refinement loop together instead of one per sub-interval.               */
+__device__ __forceinline__ double syn_walk_deferred(const SynArgs& a, const SynSeg& g, int want,
+                                                    int* count) {
+    int c = 0;
+    if (a.refine <= 0) return syn_walk(a, g, want, count);
+    const bool left = syn_on_surface(a, g);
+    if (left) {
+        if (want == 0) {
+            *count = 1;
+            return -1.0;
+        }
+        ++c;
+    }
+    const int r = a.refine;
+    const double step = 1.0 / (double)(r + 1);
+    double g_prev = 0.0;
+    bool have_prev = false;
+    bool found = false;
+    double f_slo = 0.0, f_shi = 0.0, f_glo = 0.0, f_ghi = 0.0;
+    for (int m = 0; m <= r; ++m) {
+        const double s_lo = (double)m * step;
+        const double s_hi = (double)(m + 1) * step;
+        if (s_hi > 1.0 + 1e-15) break;
+        if (left && m == 0) continue;
+        /* the value at s_lo is the previous sub-interval's value at s_hi:
+         * the same expression of the same s, so the same bits            */
+        const double g_lo = have_prev ? g_prev : syn_interp(a, g, s_lo);
+        const double g_hi = syn_interp(a, g, s_hi);
+        g_prev = g_hi;
+        have_prev = true;
+        if (!syn_crosses(g_lo, g_hi, a.direction)) continue;
+        if (c == want) {
+            found = true;
+            f_slo = s_lo;
+            f_shi = s_hi;
+            f_glo = g_lo;
+            f_ghi = g_hi;
+            break;
+        }
+        ++c;
+    }
+    if (!found) {
+        *count = c;
+        return 0.0;
+    }
+    double s;
+    if (f_glo == f_ghi) {
+        s = 0.5 * (f_slo + f_shi);
+    } else {
+        /* Python's min(1.0, max(0.0, alpha)): a NaN gives 0.0 */
+        double al = f_glo / (f_glo - f_ghi);
+        al = (0.0 < al) ? al : 0.0;
+        al = (al < 1.0) ? al : 1.0;
+        s = f_slo + al * (f_shi - f_slo);
+    }
+    if (a.cubic && g.dt > 0.0) {
+        for (int it = 0; it < a.newton_max_iter; ++it) {
+            const double f = syn_hermite(s, g.gk, g.gk1, g.d0, g.d1, g.dt);
+            const double df = syn_hermite_der(s, g.gk, g.gk1, g.d0, g.d1, g.dt);
+            if (df == 0.0) break;
+            s = s - f / df;
+            if (s < f_slo) { s = f_slo; break; }
+            if (s > f_shi) { s = f_shi; break; }
+        }
+    }
+    *count = c + 1;
+    return s;
+}
+
 /* hit state at parameter s of segment k (backend.py:345-360 / 591-606) */
 __device__ __forceinline__ void syn_state(const SynTraj& tr, const SynSeg& g, double s,
                                           double (&x)[6]) {
@@ -418,40 +491,74 @@ void k_synodic_detect(SynArgs a) {
     syn_detect_wave(a, i, threadIdx.x & 63);
 }
 
-/* time and state of candidate c of segment g (backend.py:345-360 / 591-606) */
-__device__ __forceinline__ void syn_candidate(const SynTraj& tr, const SynSeg& g, int c,
-                                              double& th, double (&x)[6]) {
+/* time and state of candidate c of segment g (backend.py:345-360 / 591-606)
+ * from the segment's samples k-1 .. k+2 already in registers (xm1, x2 and
+ * tm1, tp2 are read only where syn_state reads them): syn_state's
+ * expressions on the same values                                          */
+__device__ __forceinline__ void syn_candidate(const SynArgs& a, const SynSeg& g, int N, int c,
+                                              const double (&xm1)[6], const double (&x0)[6],
+                                              const double (&x1)[6], const double (&x2)[6],
+                                              double tm1, double tp2, double& th,
+                                              double (&x)[6]) {
     int c2 = 0;
-    const double s = syn_walk(*tr.a, g, c, &c2);
+    const double s = syn_walk_deferred(a, g, c, &c2);
+    const int k = g.k;
     if (s < 0.0) {
         th = g.t0;
 #pragma unroll
-        for (int d = 0; d < 6; ++d) x[d] = tr.X(g.k, d);
+        for (int d = 0; d < 6; ++d) x[d] = x0[d];
     } else {
         th = (1.0 - s) * g.t0 + s * g.t1;
-        syn_state(tr, g, s, x);
+        if (a.cubic && g.dt > 0.0 && k >= 1 && k + 2 < N) {
+            const double oms = 1.0 - s;
+            const double h00 = (1.0 + 2.0 * s) * (oms * oms);
+            const double h10 = s * (oms * oms);
+            const double h01 = (s * s) * (3.0 - 2.0 * s);
+            const double h11 = (s * s) * (s - 1.0);
+#pragma unroll
+            for (int d = 0; d < 6; ++d) {
+                const double dx0 = (x1[d] - xm1[d]) / (g.t1 - tm1);
+                const double dx1 = (x2[d] - x0[d]) / (tp2 - g.t0);
+                x[d] = ((h00 * x0[d] + (h10 * dx0) * g.dt) + h01 * x1[d]) + (h11 * dx1) * g.dt;
+            }
+        } else {
+#pragma unroll
+            for (int d = 0; d < 6; ++d) x[d] = x0[d] + s * (x1[d] - x0[d]);
+        }
     }
 }
 
-/* The two-pass form.  Pass 1 scans every part of SYN_PART segments of every
- * trajectory with a wave of its own and lists the segments that hold
- * candidates; pass 2 refines those segments' candidates and deduplicates
- * them per trajectory.  Candidates come from the same segment values, walk
- * and refinement as in syn_detect_wave, so the hits are the same bits.     */
+/* The multi-pass form.  Pass 1 scans every part of SYN_PART segments of
+ * every trajectory with a wave of its own and appends its candidates, with
+ * their places in segment order, to compact lists; pass 2 refines the
+ * listed candidates, one per lane of a dense grid; pass 3 deduplicates
+ * each trajectory's refined candidates in order.  Candidates come from the
+ * same segment values, walk and refinement as in syn_detect_wave, so the
+ * hits are the same bits.                                                 */
 constexpr int SYN_PART = 512;     /* segments per part (8 chunks of 64)        */
-constexpr int SYN_CAP = 32;       /* listed segments per part                  */
+constexpr int SYN_CAP = 32;       /* refined candidates recorded per part      */
 constexpr int SYN_REC = 7;        /* doubles of a refined candidate: t, x[6]   */
-constexpr int SYN_LDS_REC = 96;   /* refined candidates a wave stages in LDS   */
+constexpr int SYN_LISTS = 8;      /* candidate lists (part index % SYN_LISTS),
+                                     so the appends of a pass meet on 8
+                                     counters                                  */
 
 struct SynWs {
     int parts;                    /* parts per trajectory                      */
-    int* nlisted;                 /* [n][parts] segments with candidates       */
-    int2* listed;                 /* [n][parts][SYN_CAP] (segment, candidates) */
+    int* nlist;                   /* [SYN_LISTS * 32] entries of each list (one
+                                     128-byte line per counter; zeroed per call) */
+    int4* list;                   /* [SYN_LISTS][cap_list] (part index, segment,
+                                     candidate, place) per candidate            */
+    int cap_list;                 /* entries per list                          */
+    int* ncand;                   /* [n][parts] candidates of the part (more
+                                     than SYN_CAP: counted, not listed)        */
+    double* rec;                  /* [n][parts][SYN_REC][SYN_CAP] refined
+                                     candidates, component-major per part      */
 };
 
-/* Pass 1 for part p of trajectory i (one wave).                           */
+/* Pass 1 for part p of trajectory i (one wave); `seg` is the wave's LDS
+ * list of candidates (SYN_CAP entries).                                   */
 __device__ __forceinline__ void syn_scan_part(SynArgs a, const SynWs& w, int i, int p,
-                                              int lane) {
+                                              int lane, int4* seg) {
     SynTraj tr;
     tr.a = &a;
     tr.base = (size_t)i * a.m;
@@ -460,8 +567,8 @@ __device__ __forceinline__ void syn_scan_part(SynArgs a, const SynWs& w, int i, 
     const int nseg = tr.N - 1;
     const int s0 = p * SYN_PART;
     const int s1 = min(s0 + SYN_PART, nseg);
-    int2* listed = w.listed + ((size_t)i * w.parts + p) * SYN_CAP;
-    int found = 0;
+    const int unit = i * w.parts + p;
+    int found = 0;        /* candidates so far */
     auto extra = [&](int k0) {
         return (lane == 0) ? k0 - 1 : ((lane == 62) ? k0 + 64 : ((lane == 63) ? k0 + 65 : -1));
     };
@@ -499,161 +606,161 @@ __device__ __forceinline__ void syn_scan_part(SynArgs a, const SynWs& w, int i, 
             const SynSeg g = syn_segment(a, tr.N, k, gkm1, gj, gk1, gk2, tkm1, tj, tk1, tk2);
             syn_walk(a, g, -1, &cnt);
         }
-        const unsigned long long has = __ballot(cnt > 0);
-        if (cnt > 0) {
-            const int slot = found + __popcll(has & ((1ULL << lane) - 1ULL));
-            if (slot < SYN_CAP) listed[slot] = make_int2(k, cnt);
-        }
-        found += __popcll(has);
-    }
-    if (lane == 0) w.nlisted[(size_t)i * w.parts + p] = found;
-}
-
-/* Four waves per SIMD fit without spills (110 VGPRs); 5 and 6 spill and
- * measured no faster.                                                     */
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)))
-void k_synodic_scan(SynArgs a, SynWs w) {
-    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
-    if (wave >= (long long)a.n * w.parts) return;
-    syn_scan_part(a, w, (int)(wave / w.parts), (int)(wave % w.parts), threadIdx.x & 63);
-}
-
-/* A trajectory with a part of more than SYN_CAP listed segments: detected
- * whole, by one wave, after the merge pass.                               */
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)))
-void k_synodic_whole(SynArgs a, SynWs w) {
-    const int i = (int)((blockIdx.x * (unsigned)BLOCK + threadIdx.x) >> 6);
-    if (i >= a.n) return;
-    const int* nl = w.nlisted + (size_t)i * w.parts;
-    bool over = false;
-    for (int p = 0; p < w.parts; ++p) over = over || nl[p] > SYN_CAP;
-    if (over) syn_detect_wave(a, i, threadIdx.x & 63);
-}
-
-/* Pass 2: one wave per trajectory.  Lane l rebuilds listed segment l of a
- * round of 64 from its samples k-1 .. k+2 (syn_segment, as pass 1 did);
- * the candidates are then refined by their lane and deduplicated in
- * segment order against the last kept hit (backend.py:369-440), as in
- * syn_detect_wave.  A trajectory with a part of more than SYN_CAP listed
- * segments is left to k_synodic_whole.                                     */
-__device__ __forceinline__ void syn_merge_traj(SynArgs a, const SynWs& w, int i, int lane,
-                                               double* rec) {
-    const int* nl = w.nlisted + (size_t)i * w.parts;
-    bool over = false;
-    for (int p = 0; p < w.parts; ++p) over = over || nl[p] > SYN_CAP;
-    if (over) return;                 /* k_synodic_whole detects it */
-    SynTraj tr;
-    tr.a = &a;
-    tr.base = (size_t)i * a.m;
-    tr.tbase = a.per_traj_times ? (size_t)i * a.m : 0;
-    tr.N = a.lengths ? min(a.lengths[i], a.m) : a.m;
-    int kept = 0;
-    bool done = false;
-    double last_t = 0.0, last_u = 0.0, last_v = 0.0;
-    /* the listed segments of all parts, in order, 64 per round: entry e of
-     * the trajectory is entry e - (entries of the parts before) of its part */
-    int np = 0;
-    for (int p = 0; p < w.parts; ++p) np += nl[p];
-    const int2* listed_all = w.listed + (size_t)i * w.parts * SYN_CAP;
-    {
-        for (int r0 = 0; r0 < np && !done; r0 += 64) {
-            const int e = r0 + lane;
-            SynSeg g;
-            int cnt = 0;
-            if (e < np) {
-                int part = 0, idx = e;
-                while (idx >= nl[part]) idx -= nl[part++];
-                const int2 ent = listed_all[(size_t)part * SYN_CAP + idx];
-                const int k = ent.x;
-                cnt = ent.y;
-                const double gk = tr.G(k), tk = tr.T(k);
-                const double gk1 = tr.G(k + 1), tk1 = tr.T(k + 1);
-                const bool m1 = k >= 1, p2 = k + 2 < tr.N;
-                const double gkm1 = m1 ? tr.G(k - 1) : 0.0, tkm1 = m1 ? tr.T(k - 1) : 0.0;
-                const double gk2 = p2 ? tr.G(k + 2) : 0.0, tk2 = p2 ? tr.T(k + 2) : 0.0;
-                g = syn_segment(a, tr.N, k, gkm1, gk, gk1, gk2, tkm1, tk, tk1, tk2);
-            }
+        if (__ballot(cnt > 0)) {
+            /* places in segment order: an exclusive scan of the counts */
             int inc = cnt;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
                 const int v = __shfl_up(inc, o);
                 if (lane >= o) inc += v;
             }
-            const int exc = inc - cnt;
-            const int total = __shfl(inc, 63);
-            /* every lane refines its segment's candidates at once, into the
-             * wave's LDS records in segment order (a round with more than
-             * SYN_LDS_REC candidates refines them one at a time instead) */
-            const bool staged = total <= SYN_LDS_REC;
-            if (staged) {
-                for (int c = 0; c < cnt; ++c) {
-                    double th, x[6];
-                    syn_candidate(tr, g, c, th, x);
-                    double* r = rec + (size_t)(exc + c) * SYN_REC;
-                    r[0] = th;
+            const int first = found + inc - cnt;
+            for (int c = 0; c < cnt && first + c < SYN_CAP; ++c)
+                seg[first + c] = make_int4(unit, k, c, first + c);
+            found += __shfl(inc, 63);
+        }
+    }
+    if (lane == 0) w.ncand[unit] = found;
+    if (found == 0 || found > SYN_CAP) return;   /* nothing, or detected whole by the merge */
+    /* found <= SYN_CAP: every candidate is in seg[]; append them to list
+     * unit % SYN_LISTS                                                    */
+    __builtin_amdgcn_wave_barrier();
+    const int li = unit % SYN_LISTS;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(w.nlist + li * 32, found);
+    base = __shfl(base, 0);
+    if (lane < found) w.list[(size_t)li * w.cap_list + base + lane] = seg[lane];
+}
+
+/* Four waves per SIMD fit without spills; three, with a second chunk in
+ * flight per wave, measured slower (288 against 247 us on 4,096 x 4,713
+ * samples).                                                               */
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void k_synodic_scan(SynArgs a, SynWs w) {
+    __shared__ int4 seg[BLOCK / 64][SYN_CAP];
+    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    if (wave >= (long long)a.n * w.parts) return;
+    syn_scan_part(a, w, (int)(wave / w.parts), (int)(wave % w.parts), threadIdx.x & 63,
+                  seg[threadIdx.x >> 6]);
+}
+
+/* Pass 2: every listed candidate in a lane of its own (a grid-stride loop
+ * over the lists in order), refined into its place in its part's records. */
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_synodic_refine(SynArgs a, SynWs w) {
+    const int stride = (int)(gridDim.x * BLOCK);
+    int skip = 0;         /* entries of the lists before list li */
+    for (int li = 0; li < SYN_LISTS; ++li) {
+        const int cnt_li = w.nlist[li * 32];
+        const int e0 = (int)(blockIdx.x * BLOCK + threadIdx.x) - (skip % stride);
+        skip += cnt_li;
+        for (int e = e0 < 0 ? e0 + stride : e0; e < cnt_li; e += stride) {
+            const int4 ent = w.list[(size_t)li * w.cap_list + e];
+            const int i = ent.x / w.parts;
+            SynTraj tr;
+            tr.a = &a;
+            tr.base = (size_t)i * a.m;
+            tr.tbase = a.per_traj_times ? (size_t)i * a.m : 0;
+            tr.N = a.lengths ? min(a.lengths[i], a.m) : a.m;
+            const int k = ent.y;
+            const bool m1 = k >= 1, p2 = k + 2 < tr.N;
+            /* the four samples around the segment, loaded together up front */
+            double x0[6], x1[6], t0, t1;
+            double xm1[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, x2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            double tm1 = 0.0, t2 = 0.0;
+            tr.load(k, x0, t0);
+            tr.load(k + 1, x1, t1);
+            if (m1) tr.load(k - 1, xm1, tm1);
+            if (p2) tr.load(k + 2, x2, t2);
+            const double gk = tr.G(x0), gk1 = tr.G(x1);
+            const double gkm1 = m1 ? tr.G(xm1) : 0.0, gk2 = p2 ? tr.G(x2) : 0.0;
+            const SynSeg g = syn_segment(a, tr.N, k, gkm1, gk, gk1, gk2, tm1, t0, t1, t2);
+            double th, x[6];
+            syn_candidate(a, g, tr.N, ent.z, xm1, x0, x1, x2, tm1, t2, th, x);
+            double* rec = w.rec + (size_t)ent.x * (SYN_CAP * SYN_REC) + ent.w;
+            rec[0] = th;
 #pragma unroll
-                    for (int d = 0; d < 6; ++d) r[1 + d] = x[d];
-                }
+            for (int d = 0; d < 6; ++d) rec[(1 + d) * SYN_CAP] = x[d];
+        }
+    }
+}
+
+/* Pass 3: one wave per trajectory.  Lane l loads record r0 + l of the
+ * trajectory's records (all parts, in order); the records are then
+ * deduplicated in that order against the last kept hit (backend.py:369-440)
+ * as in syn_detect_wave.  A trajectory with a part of more than SYN_CAP
+ * candidates (none recorded for that part) is detected whole by the wave,
+ * as syn_detect_wave does.                                                */
+__device__ __forceinline__ void syn_merge_traj(SynArgs a, const SynWs& w, int i, int lane) {
+    const int* nc = w.ncand + (size_t)i * w.parts;
+    int np = 0;
+    bool over = false;
+    for (int p = 0; p < w.parts; ++p) {
+        const int c = nc[p];
+        over = over || c > SYN_CAP;
+        np += c;
+    }
+    if (over) {
+        syn_detect_wave(a, i, lane);
+        return;
+    }
+    int kept = 0;
+    bool done = false;
+    double last_t = 0.0, last_u = 0.0, last_v = 0.0;
+    const double* rec_all = w.rec + (size_t)i * w.parts * (SYN_CAP * SYN_REC);
+    for (int r0 = 0; r0 < np && !done; r0 += 64) {
+        const int e = r0 + lane;
+        double th = 0.0, x[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (e < np) {
+            int part = 0, idx = e;
+            while (idx >= nc[part]) idx -= nc[part++];
+            const double* r = rec_all + (size_t)part * (SYN_CAP * SYN_REC) + idx;
+            th = r[0];
+#pragma unroll
+            for (int d = 0; d < 6; ++d) x[d] = r[(1 + d) * SYN_CAP];
+        }
+        const int total = min(64, np - r0);
+        for (int j = 0; j < total && !done; ++j) {
+            const double tj = rlane(th, j);
+            double xj[6];
+#pragma unroll
+            for (int d = 0; d < 6; ++d) xj[d] = rlane(x[d], j);
+            double u = xj[0], v = xj[0];
+#pragma unroll
+            for (int d = 1; d < 6; ++d) {
+                u = (a.pi == d) ? xj[d] : u;
+                v = (a.pj == d) ? xj[d] : v;
             }
-            __builtin_amdgcn_wave_barrier();
-            for (int j = 0; j < total && !done; ++j) {
-                double th = 0.0, x[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-                if (staged) {
-                    const double* r = rec + (size_t)j * SYN_REC;
-                    th = r[0];
-#pragma unroll
-                    for (int d = 0; d < 6; ++d) x[d] = r[1 + d];
-                } else {
-                    const unsigned long long own = __ballot(exc <= j && j < inc);
-                    const int src = __ffsll((long long)own) - 1;
-                    if (lane == src) syn_candidate(tr, g, j - exc, th, x);
-                    th = __shfl(th, src);
-#pragma unroll
-                    for (int d = 0; d < 6; ++d) x[d] = __shfl(x[d], src);
-                }
-                double u = x[0], v = x[0];
-#pragma unroll
-                for (int d = 1; d < 6; ++d) {
-                    u = (a.pi == d) ? x[d] : u;
-                    v = (a.pj == d) ? x[d] : v;
-                }
-                if (kept > 0) {
-                    if (__builtin_fabs(th - last_t) <= a.dedup_t) continue;
-                    const double du = u - last_u, dv = v - last_v;
-                    if ((du * du + dv * dv) <= (a.dedup_p * a.dedup_p)) continue;
-                }
-                if (lane == 0 && kept < a.cap) {
-                    const size_t o = (size_t)i * a.cap + kept;
-                    a.hit_t[o] = th;
-#pragma unroll
-                    for (int d = 0; d < 6; ++d) a.hit_x[o * 6 + d] = x[d];
-                    a.hit_p[o * 2] = u;
-                    a.hit_p[o * 2 + 1] = v;
-                }
-                ++kept;
-                last_t = th;
-                last_u = u;
-                last_v = v;
-                if (a.max_hits > 0 && kept >= a.max_hits) done = true;
+            if (kept > 0) {
+                if (__builtin_fabs(tj - last_t) <= a.dedup_t) continue;
+                const double du = u - last_u, dv = v - last_v;
+                if ((du * du + dv * dv) <= (a.dedup_p * a.dedup_p)) continue;
             }
-            /* the records are read by every lane before the next round writes */
-            __builtin_amdgcn_wave_barrier();
+            if (lane == 0 && kept < a.cap) {
+                const size_t o = (size_t)i * a.cap + kept;
+                a.hit_t[o] = tj;
+#pragma unroll
+                for (int d = 0; d < 6; ++d) a.hit_x[o * 6 + d] = xj[d];
+                a.hit_p[o * 2] = u;
+                a.hit_p[o * 2 + 1] = v;
+            }
+            ++kept;
+            last_t = tj;
+            last_u = u;
+            last_v = v;
+            if (a.max_hits > 0 && kept >= a.max_hits) done = true;
         }
     }
     if (lane == 0) a.count[i] = kept;
 }
 
-/* Three waves per SIMD (167 VGPRs, no spills); four spill and measured the
- * same.                                                                    */
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3)))
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void k_synodic_merge(SynArgs a, SynWs w) {
     const int i = (int)((blockIdx.x * (unsigned)BLOCK + threadIdx.x) >> 6);
     if (i >= a.n) return;
-    __shared__ double lds[BLOCK / 64][SYN_LDS_REC * SYN_REC];
-    syn_merge_traj(a, w, i, threadIdx.x & 63, lds[threadIdx.x >> 6]);
+    syn_merge_traj(a, w, i, threadIdx.x & 63);
 }
 
 }  // namespace hh

@@ -274,12 +274,13 @@ int hh_synodic_detect(int n, int m, const double* times, int per_traj_times,
                       double dedup_point_tol, int max_hits, int newton_max_iter, int cap,
                       int* count, double* hit_t, double* hit_x, double* hit_p, void* stream);
 
-/* hh_synodic_detect in two passes: every part of 512 segments of every
+/* hh_synodic_detect in passes: every part of 512 segments of every
  * trajectory is scanned by a wave of its own, which lists the segments
- * holding candidates in the workspace; then one wave per trajectory refines
- * and deduplicates their candidates in segment order -- the same hits, bit
- * for bit.  A trajectory with more than 32 such segments in a part is
- * detected whole by a third launch, as hh_synodic_detect does.  workspace: device memory of hh_synodic_workspace(n, m)
+ * holding candidates; the listed segments are refined one per lane; then
+ * one wave per trajectory deduplicates the refined candidates in segment
+ * order -- the same hits, bit for bit.  A trajectory with more than 32
+ * candidates in a part is detected whole by its wave of the last pass, as
+ * hh_synodic_detect does.  workspace: device memory of hh_synodic_workspace(n, m)
  * bytes, 256-byte aligned, owned by the caller (no memory is allocated).  */
 long long hh_synodic_workspace(int n, int m);
 int hh_synodic_detect_ws(int n, int m, const double* times, int per_traj_times,
