@@ -267,8 +267,8 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
         kw = dict(normal=synodic.axis_normal("y"), plane_coords=("y", "vy"),
                   interp_kind="cubic", segment_refine=20, newton_max_iter=25)
         res = synodic.detect_arrays(times, states, **kw)
-        # a pass takes about 0.3 ms: time at least 50 of them
-        steps = max(steps, 50)
+        # a pass takes about 0.3 ms: time at least 200 of them
+        steps = max(steps, 200)
         out["steps"] = steps
         wall, ev = _timed(res.relaunch, steps, warmup)
         hits = int(res.count.to(torch.int64).sum().item())
