@@ -39,8 +39,11 @@
  * reciprocal square root refined by the coupled iteration g ~ sqrt(x),
  * h ~ 1/(2 sqrt(x)) and two residual corrections d = x - g*g (fma), so the
  * result is the same bits as __builtin_sqrt; other x take __builtin_sqrt.  */
-static __device__ __forceinline__ double hh_dev_sqrt(double x) {
-    if (!(x >= 0x1p-767) || !(x <= 1.7976931348623157e308)) return __builtin_sqrt(x);
+static __device__ __forceinline__ bool hh_dev_sqrt_fast(double x) {
+    return x >= 0x1p-767 && x <= 1.7976931348623157e308;
+}
+/* the expansion for hh_dev_sqrt_fast(x) only */
+static __device__ __forceinline__ double hh_dev_sqrt_core(double x) {
     const double y0 = __builtin_amdgcn_rsq(x);
     double g = x * y0;
     double h = y0 * 0.5;
@@ -52,8 +55,45 @@ static __device__ __forceinline__ double hh_dev_sqrt(double x) {
     d = __builtin_fma(-g, g, x);
     return __builtin_fma(d, h, g);
 }
+static __device__ __forceinline__ double hh_dev_sqrt(double x) {
+    if (!hh_dev_sqrt_fast(x)) return __builtin_sqrt(x);
+    return hh_dev_sqrt_core(x);
+}
+/* hh_dev_sqrt with one wave-uniform test in place of a per-lane branch:
+ * when every active lane's argument is in the fast range (the case of every
+ * CR3BP distance away from a collision) the wave runs the expansion alone,
+ * else every lane takes hh_dev_sqrt.  Same bits either way.              */
+static __device__ __forceinline__ double hh_dev_sqrt_wave(double x) {
+    if (__ballot(!hh_dev_sqrt_fast(x)) == 0) return hh_dev_sqrt_core(x);
+    return hh_dev_sqrt(x);
+}
 #define HH_SQRT(x) hh_dev_sqrt(x)
 #include "hh_pow.h"
+
+/* hh_pow15_25 with one wave-uniform test in place of its per-lane branches
+ * (special arguments, and the square root's range): when every active
+ * lane's argument is in [2^-767, DBL_MAX] the wave runs hh_pow15_25's
+ * expressions on the fast square root directly, else every lane takes
+ * hh_pow15_25.  Same bits either way.                                     */
+static __device__ __forceinline__ void hh_dev_pow15_25(double x, double* p15, double* p25) {
+    if (__ballot(!hh_dev_sqrt_fast(x)) != 0) {
+        hh_pow15_25(x, p15, p25);
+        return;
+    }
+    const double s = hh_dev_sqrt_core(x);
+    const double e = __builtin_fma(-s, s, x);
+    const double he = 0.5 * e;
+    double p, pe;
+    HH_TWO_PROD(x, s, p, pe);
+    const double r15 = p + (pe + s * he);
+    *p15 = (p > 1.7976931348623157e308) ? p : r15;
+    double q, qe;
+    HH_TWO_PROD(x, x, q, qe);
+    double p2, pe2;
+    HH_TWO_PROD(q, s, p2, pe2);
+    const double r25 = p2 + (pe2 + (p * he + qe * s));
+    *p25 = (p2 > 1.7976931348623157e308) ? p2 : r25;
+}
 
 namespace hh {
 
@@ -202,8 +242,8 @@ struct Lane42 {
         const double r2 = (xm * xm + y * y) + z * z;
         const double R2 = (xr * xr + y * y) + z * z;
         double r3, r5, R3, R5;
-        hh_pow15_25(r2, &r3, &r5);
-        hh_pow15_25(R2, &R3, &R5);
+        hh_dev_pow15_25(r2, &r3, &r5);
+        hh_dev_pow15_25(R2, &R3, &R5);
         const double q1 = (st ? xm : mu2) / (st ? r3 : r5);
         const double q2 = (st ? xr : mu) / (st ? R3 : R5);
         const double q3 = (st ? y : mu2) / r3;

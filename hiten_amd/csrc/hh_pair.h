@@ -81,7 +81,7 @@ struct Pair42 {
             const double xo = x + w.cl;
             const double P = (xo * xo + y * y) + z * z;
             double P3, P5;
-            hh_pow15_25(P, &P3, &P5);
+            hh_dev_pow15_25(P, &P3, &P5);
             const double m = w.odd ? mu : mu2;
             const int kind = w.kind;
             double num = xo;

@@ -96,7 +96,7 @@ struct Wave42 {
         const double xo = x + cl;
         const double P = (xo * xo + y * y) + z * z;
         double P3, P5;
-        hh_pow15_25(P, &P3, &P5);
+        hh_dev_pow15_25(P, &P3, &P5);
         const double m = odd ? mu : mu2;
         double num = xo;
         num = (kind == 1) ? y : num;
