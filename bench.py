@@ -210,7 +210,9 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
         fl = per * 1167
         res = {}
         for variant, kern in (("plain", "k_rk_fixed6"), ("wide", "k_rk_fixed6_group")):
-            fn = lambda: batch.rk8_fixed(mu_d, y0, t0_d, tf_d, w["steps"], variant=variant)
+            # the reference driver's outputs: every grid state and its derivative
+            fn = lambda: batch.rk8_fixed(mu_d, y0, t0_d, tf_d, w["steps"], variant=variant,
+                                         return_derivatives=True)
             wall, ev = _timed(fn, steps, warmup)
             res[kern] = dict(value=per * steps / wall, wall=wall, launch_s=sum(ev) / len(ev))
         best = max(res, key=lambda k: res[k]["value"])
@@ -218,7 +220,8 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
         out.update(metric="FP64 trajectory-steps/s, fixed-step RK8 6-d", value=res[best]["value"],
                    ms_per_step=res[best]["wall"] / steps * 1e3,
                    config={"workload": "rk8_grid: 1024 x 6-d, RK8, 1999 steps of a 2000-point "
-                                       "grid to 2 pi, final state out, mu=0.012150585609624", "kernel": best,
+                                       "grid to 2 pi, every grid state and its derivative out, "
+                                       "mu=0.012150585609624", "kernel": best,
                            "by_kernel": {k: v["value"] for k, v in res.items()}},
                    roofline={"bound": "fp64_valu", "achieved": fl / ls / 1e12,
                              "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -228,11 +231,13 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
             from oracle import oracle as O
             k = 256
             t = time.perf_counter()
-            O.batch_rk8_final(w["mu"], w["y0"][:k], w["t0"], w["tf"], w["steps"], n_threads=threads)
+            O.batch_rk8_final(w["mu"], w["y0"][:k], w["t0"], w["tf"], w["steps"], n_threads=threads,
+                              with_derivs=True)
             dt = time.perf_counter() - t
             out["cpu_baseline"] = {"value": k * (w["steps"] - 1) / dt, "unit": "trajectory-steps/s",
                                    "cores": threads, "kind": "port",
-                                   "sample": f"first {k} of {n} trajectories"}
+                                   "sample": f"first {k} of {n} trajectories, grid states and "
+                                             "derivatives formed"}
     elif name == "stm_uneven":
         w = W.stm_uneven()
         res = {}

@@ -26,7 +26,7 @@ HH_VARIANT_WIDE = 2
 HH_VARIANT_PAIR = 3
 HH_VARIANT_PERSISTENT_PAIR = 4
 HH_STATUS_OK, HH_STATUS_MAXSTEPS, HH_STATUS_NONFINITE, HH_STATUS_DROPPED = 0, 1, 2, 4
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 class HipUnavailable(RuntimeError):
@@ -59,7 +59,7 @@ _SIGNATURES = {
     "hh_set_tuning": ([C.POINTER(Tuning)], _I),
     "hh_get_tuning": ([C.POINTER(Tuning)], _I),
     "hh_rk8_fixed": ([_I, _I, _P, _P, _P, _I, _P, _I, _ULL, _P, _P, _P, _I, _I, _P], _I),
-    "hh_rk_fixed": ([_I, _I, _I, _P, _P, _P, _I, _P, _I, _ULL, _P, _P, _P, _I, _I, _P], _I),
+    "hh_rk_fixed": ([_I, _I, _I, _P, _P, _P, _I, _P, _I, _ULL, _P, _P, _P, _P, _I, _I, _P], _I),
     "hh_rk_fixed_event": ([_I, _I, _P, _P, _P, _I, _P, _I, _ULL, _I, _D, _I, _D, _D, _P, _P, _P,
                            _P, _P, _I, _P], _I),
     "hh_dop853_final": ([_I, _I, _P, _P, _P, _D, _D, _D, _D, _LL, _I, _ULL, _P, _P, _P, _P, _P,

@@ -31,13 +31,15 @@ from dataclasses import asdict, dataclass, field, replace
 from typing import Callable, Dict, List, Optional, Sequence
 
 # knobs in search order, with their candidate values (defaults first)
+# (the pair-tail form's own knobs come before the variant: measured first,
+# they are not skipped when the other form wins its first comparison by noise)
 SPACE: Dict[str, Sequence] = {
-    "variant": ("persistent_pair", "persistent"),
     "heavy_projected": (2000, 1500, 2500, 3000),
     "min_age": (128, 64, 256),
-    "waves_per_simd": (2, 1),
+    "pair_projected": (0, 2000, 6000),
     "refill_min": (1, 2, 4),
-    "pair_projected": (0, 6000),
+    "waves_per_simd": (2, 1),
+    "variant": ("persistent_pair", "persistent"),
     "exclusive_projected": (0, 3000, 2000),
 }
 

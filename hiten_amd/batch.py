@@ -122,13 +122,16 @@ def rk8_fixed(mu, y0, t0, tf, steps: int, **kw):
 
 def rk_fixed(mu, y0, t0, tf, steps: int, *, order: int = 8, dim: int = 6, t_vals=None,
              forward: int = 1, flip_indices=None, return_trajectory: bool = False,
-             variant: str = "auto", block: int = 0, stream=None):
+             return_derivatives: bool = False, variant: str = "auto", block: int = 0,
+             stream=None):
     """Fixed-step RK of the given order (4, 6 or 8: the tableaux of
     rk.py:755-783) over numpy.linspace(t0, tf, steps), or over the explicit
     grid ``t_vals`` shared by all trajectories (rk.py:531-588).
 
     Returns the final states (n, dim); with ``return_trajectory`` also the
-    grid states as an (n, steps, dim) view.  For dim 6, ``variant`` "wide"
+    grid states as an (n, steps, dim) view, and with ``return_derivatives``
+    (which implies the grid) also the derivative at every grid state, as
+    _integrate_fixed_rk returns them.  For dim 6, ``variant`` "wide"
     integrates one trajectory per 8-lane group (one component per lane, the
     latency form), "plain" one per lane; "auto" takes "wide" up to 65,536
     trajectories.  Both give the same bits.
@@ -147,15 +150,20 @@ def rk_fixed(mu, y0, t0, tf, steps: int, *, order: int = 8, dim: int = 6, t_vals
     else:
         t0_t, tf_t = _vec(t0, n, "t0"), _vec(tf, n, "tf")
     out = torch.empty_like(y)
-    traj = torch.empty((steps, dim, n), dtype=torch.float64, device=y.device) if return_trajectory else None
+    grid = return_trajectory or return_derivatives
+    traj = torch.empty((steps, dim, n), dtype=torch.float64, device=y.device) if grid else None
+    der = torch.empty((steps, dim, n), dtype=torch.float64, device=y.device) \
+        if return_derivatives else None
     if variant == "auto":
         variant = "wide" if n <= 65536 else "plain"
     rc = lib.hh_rk_fixed(int(order), n, dim, _ptr(mu_t), _ptr(t0_t), _ptr(tf_t), int(steps),
                          _ptr(tv), int(forward), flip_mask(dim, flip_indices), _ptr(y), _ptr(out),
-                         _ptr(traj), {"plain": _lib.HH_VARIANT_PLAIN,
-                                      "wide": _lib.HH_VARIANT_WIDE}[variant],
+                         _ptr(traj), _ptr(der), {"plain": _lib.HH_VARIANT_PLAIN,
+                                                 "wide": _lib.HH_VARIANT_WIDE}[variant],
                          int(block), _lib.stream_handle(stream))
     _lib.check(rc, "hh_rk_fixed")
+    if return_derivatives:
+        return out.t(), traj.permute(2, 0, 1), der.permute(2, 0, 1)
     if return_trajectory:
         return out.t(), traj.permute(2, 0, 1)
     return out.t()

@@ -561,10 +561,15 @@ struct RK8 {
     static __device__ __forceinline__ double b(int j) { return HH_RK8_B[j]; }
 };
 
+/* rk_step (rk.py:154-215); k0 receives the first stage, f(t, y) at the
+ * step's start -- the derivative _integrate_fixed_rk stores for that state
+ * (rk.py:579-587)                                                         */
 template <class M, class P>
-__device__ __forceinline__ void rk_step(const P& p, double (&y)[6], double h) {
+__device__ __forceinline__ void rk_step(const P& p, double (&y)[6], double h, double (&k0)[6]) {
     double k[M::S][6];
     p.rhs(y, k[0]);
+#pragma unroll
+    for (int d = 0; d < 6; ++d) k0[d] = k[0][d];
 #pragma unroll
     for (int i = 1; i < M::S; ++i) {
         double ys[6];
@@ -590,6 +595,12 @@ __device__ __forceinline__ void rk_step(const P& p, double (&y)[6], double h) {
             for (int d = 0; d < 6; ++d) y[d] = y[d] + hb * k[j][d];
         }
     }
+}
+
+template <class M, class P>
+__device__ __forceinline__ void rk_step(const P& p, double (&y)[6], double h) {
+    double k0[6];
+    rk_step<M>(p, y, h, k0);
 }
 
 }  // namespace hh

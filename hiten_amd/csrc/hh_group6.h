@@ -213,11 +213,13 @@ struct Group6 {
     }
 };
 
-/* rk_step (rk.py:154-215) for this lane's component */
+/* rk_step (rk.py:154-215) for this lane's component; k0 receives the first
+ * stage (the derivative at the step's start)                              */
 template <class M>
-__device__ __forceinline__ void rk_step_group(const Group6& p, double& y, double h) {
+__device__ __forceinline__ void rk_step_group(const Group6& p, double& y, double h, double& k0) {
     double k[M::S];
     k[0] = p.rhs(y);
+    k0 = k[0];
 #pragma unroll
     for (int i = 1; i < M::S; ++i) {
         double ys = y;

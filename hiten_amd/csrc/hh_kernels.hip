@@ -244,29 +244,43 @@ __device__ __forceinline__ double grid_time(const double* t_vals, int s, int ste
 template <class M, class P>
 __device__ __forceinline__ void rk_run(const P& p, double (&y)[6], const double* t_vals,
                                         int steps, double t0, double tf, double* traj_base,
-                                        size_t traj_stride, int blk) {
+                                        double* deriv_base, size_t traj_stride, int blk) {
     const double delta = tf - t0;
     const double step = delta / (double)(steps - 1);
     const bool zstep = (step == 0.0);
     const bool constant = t_vals ? isclose(t_vals[0], t_vals[steps - 1]) : isclose(t0, tf);
-    auto store = [&](int s_idx) {
-        if (!traj_base) return;
+    auto store = [&](double* base, int s_idx, const double (&v)[6]) {
+        if (!base) return;
         if (P::N == 6) {
 #pragma unroll
-            for (int d = 0; d < 6; ++d) traj_base[((size_t)s_idx * 6 + d) * traj_stride] = y[d];
+            for (int d = 0; d < 6; ++d) base[((size_t)s_idx * 6 + d) * traj_stride] = v[d];
         } else if (blk < 7) {
 #pragma unroll
             for (int d = 0; d < 6; ++d)
-                traj_base[((size_t)s_idx * 42 + comp42(blk, d)) * traj_stride] = y[d];
+                base[((size_t)s_idx * 42 + comp42(blk, d)) * traj_stride] = v[d];
         }
     };
-    store(0);
+    store(traj_base, 0, y);
     double t_prev = grid_time(t_vals, 0, steps, t0, tf, step, delta, zstep);
     for (int s = 1; s < steps; ++s) {
         const double t_s = grid_time(t_vals, s, steps, t0, tf, step, delta, zstep);
-        if (!constant) rk_step<M>(p, y, t_s - t_prev);
+        if (!constant) {
+            double k0[6];
+            rk_step<M>(p, y, t_s - t_prev, k0);
+            store(deriv_base, s - 1, k0);   /* f at state s - 1 (rk.py:577, 587) */
+        }
         t_prev = t_s;
-        store(s);
+        store(traj_base, s, y);
+    }
+    if (deriv_base) {
+        /* the last state's derivative; on a constant grid every state is y0 */
+        double f[6];
+        p.rhs(y, f);
+        if (constant) {
+            for (int s = 0; s < steps; ++s) store(deriv_base, s, f);
+        } else {
+            store(deriv_base, steps - 1, f);
+        }
     }
 }
 
@@ -279,7 +293,8 @@ __global__ __launch_bounds__(256) void k_rk_fixed6(int n, const double* __restri
                                                    unsigned long long flip,
                                                    const double* __restrict__ y0,
                                                    double* __restrict__ yout,
-                                                   double* __restrict__ traj) {
+                                                   double* __restrict__ traj,
+                                                   double* __restrict__ deriv) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Lane6 p;
@@ -289,7 +304,8 @@ __global__ __launch_bounds__(256) void k_rk_fixed6(int n, const double* __restri
     double y[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) y[d] = y0[(size_t)d * n + i];
-    rk_run<M>(p, y, t_vals, steps, t0a[i], tfa[i], traj ? traj + i : nullptr, (size_t)n, 0);
+    rk_run<M>(p, y, t_vals, steps, t0a[i], tfa[i], traj ? traj + i : nullptr,
+              deriv ? deriv + i : nullptr, (size_t)n, 0);
 #pragma unroll
     for (int d = 0; d < 6; ++d) yout[(size_t)d * n + i] = y[d];
 }
@@ -300,7 +316,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_rk_fixed42(int n, const double* __restrict__ mu, const double* __restrict__ t0a,
                    const double* __restrict__ tfa, int steps, const double* __restrict__ t_vals,
                    int fwd, unsigned long long flip, const double* __restrict__ y0,
-                   double* __restrict__ yout, double* __restrict__ traj) {
+                   double* __restrict__ yout, double* __restrict__ traj,
+                   double* __restrict__ deriv) {
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gtid >> 3;
     const int b = gtid & 7;
@@ -314,7 +331,8 @@ void k_rk_fixed42(int n, const double* __restrict__ mu, const double* __restrict
     double y[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) y[d] = (b < 7) ? y0[(size_t)comp42(b, d) * n + i] : 0.0;
-    rk_run<M>(p, y, t_vals, steps, t0a[i], tfa[i], traj ? traj + i : nullptr, (size_t)n, b);
+    rk_run<M>(p, y, t_vals, steps, t0a[i], tfa[i], traj ? traj + i : nullptr,
+              deriv ? deriv + i : nullptr, (size_t)n, b);
     if (b < 7) {
 #pragma unroll
         for (int d = 0; d < 6; ++d) yout[(size_t)comp42(b, d) * n + i] = y[d];
@@ -436,7 +454,7 @@ __global__ __launch_bounds__(BLOCK) void k_rk_fixed6_group(
     int n, const double* __restrict__ mu, const double* __restrict__ t0a,
     const double* __restrict__ tfa, int steps, const double* __restrict__ t_vals, int fwd,
     unsigned long long flip, const double* __restrict__ y0, double* __restrict__ yout,
-    double* __restrict__ traj) {
+    double* __restrict__ traj, double* __restrict__ deriv) {
     const long long gtid = (long long)blockIdx.x * BLOCK + threadIdx.x;
     const int i = (int)(gtid >> 3);
     if (i >= n) return;
@@ -453,9 +471,25 @@ __global__ __launch_bounds__(BLOCK) void k_rk_fixed6_group(
     double t_prev = grid_time(t_vals, 0, steps, t0, tf, step, delta, zstep);
     for (int s_idx = 1; s_idx < steps; ++s_idx) {
         const double t_s = grid_time(t_vals, s_idx, steps, t0, tf, step, delta, zstep);
-        if (!constant) rk_step_group<M>(p, y, t_s - t_prev);
+        if (!constant) {
+            double k0;
+            rk_step_group<M>(p, y, t_s - t_prev, k0);
+            if (deriv && p.act) deriv[((size_t)(s_idx - 1) * 6 + d) * n + i] = k0;
+        }
         t_prev = t_s;
         if (traj && p.act) traj[((size_t)s_idx * 6 + d) * n + i] = y;
+    }
+    if (deriv) {
+        /* the last state's derivative; on a constant grid every state is y0 */
+        const double f = p.rhs(y);
+        if (p.act) {
+            if (constant) {
+                for (int s_idx = 0; s_idx < steps; ++s_idx)
+                    deriv[((size_t)s_idx * 6 + d) * n + i] = f;
+            } else {
+                deriv[((size_t)(steps - 1) * 6 + d) * n + i] = f;
+            }
+        }
     }
     if (p.act) yout[(size_t)d * n + i] = y;
 }
@@ -1797,22 +1831,24 @@ template <class M>
 void launch_rk_fixed(int n, int dim, const double* mu, const double* t0p,
                             const double* tfp, int steps, const double* t_vals, int fwd,
                             unsigned long long flip_mask, const double* y0, double* y_out,
-                            double* traj_out, int variant, int block, hipStream_t st) {
+                            double* traj_out, double* deriv_out, int variant, int block,
+                            hipStream_t st) {
     if (dim == 6 && variant == HH_VARIANT_WIDE) {
         const int g = blocks_for((long long)n * 8, block);
         switch (block) {
-            case 64: hipLaunchKernelGGL((k_rk_fixed6_group<M, 64>), dim3(g), dim3(64), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out); break;
-            case 128: hipLaunchKernelGGL((k_rk_fixed6_group<M, 128>), dim3(g), dim3(128), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out); break;
-            case 192: hipLaunchKernelGGL((k_rk_fixed6_group<M, 192>), dim3(g), dim3(192), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out); break;
-            default: hipLaunchKernelGGL((k_rk_fixed6_group<M, 256>), dim3(g), dim3(256), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out); break;
+            case 64: hipLaunchKernelGGL((k_rk_fixed6_group<M, 64>), dim3(g), dim3(64), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out, deriv_out); break;
+            case 128: hipLaunchKernelGGL((k_rk_fixed6_group<M, 128>), dim3(g), dim3(128), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out, deriv_out); break;
+            case 192: hipLaunchKernelGGL((k_rk_fixed6_group<M, 192>), dim3(g), dim3(192), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out, deriv_out); break;
+            default: hipLaunchKernelGGL((k_rk_fixed6_group<M, 256>), dim3(g), dim3(256), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out, deriv_out); break;
         }
     } else if (dim == 6) {
         hipLaunchKernelGGL(k_rk_fixed6<M>, dim3(blocks_for(n, block)), dim3(block), 0, st, n, mu,
-                           t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out);
+                           t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out, traj_out,
+                           deriv_out);
     } else {
         hipLaunchKernelGGL(k_rk_fixed42<M>, dim3(blocks_for((long long)n * 8, block)),
                            dim3(block), 0, st, n, mu, t0p, tfp, steps, t_vals, fwd, flip_mask,
-                           y0, y_out, traj_out);
+                           y0, y_out, traj_out, deriv_out);
     }
 }
 
@@ -1862,8 +1898,8 @@ int hh_get_tuning(hh_tuning_t* t) {
 
 int hh_rk_fixed(int order, int n, int dim, const double* mu, const double* t0, const double* tf,
                 int steps, const double* t_vals, int fwd, unsigned long long flip_mask,
-                const double* y0, double* y_out, double* traj_out, int variant, int block,
-                void* stream) {
+                const double* y0, double* y_out, double* traj_out, double* deriv_out,
+                int variant, int block, void* stream) {
     if (order != 4 && order != 6 && order != 8)
         return fail(HH_ERR_ARGS, "hh_rk_fixed: order must be 4, 6 or 8");
     if (n < 0) return fail(HH_ERR_ARGS, "hh_rk_fixed: n < 0");
@@ -1882,13 +1918,13 @@ int hh_rk_fixed(int order, int n, int dim, const double* mu, const double* t0, c
     hipStream_t st = (hipStream_t)stream;
     if (order == 4)
         launch_rk_fixed<RK4>(n, dim, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out,
-                             traj_out, variant, block, st);
+                             traj_out, deriv_out, variant, block, st);
     else if (order == 6)
         launch_rk_fixed<RK6>(n, dim, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out,
-                             traj_out, variant, block, st);
+                             traj_out, deriv_out, variant, block, st);
     else
         launch_rk_fixed<RK8>(n, dim, mu, t0p, tfp, steps, t_vals, fwd, flip_mask, y0, y_out,
-                             traj_out, variant, block, st);
+                             traj_out, deriv_out, variant, block, st);
     return check_launch("hh_rk_fixed launch");
 }
 
@@ -1933,7 +1969,7 @@ int hh_rk8_fixed(int n, int dim, const double* mu, const double* t0, const doubl
                  const double* t_vals, int fwd, unsigned long long flip_mask, const double* y0,
                  double* y_out, double* traj_out, int variant, int block, void* stream) {
     return hh_rk_fixed(8, n, dim, mu, t0, tf, steps, t_vals, fwd, flip_mask, y0, y_out,
-                       traj_out, variant, block, stream);
+                       traj_out, nullptr, variant, block, stream);
 }
 
 /* Resident workgroups of a kernel on the current device (CU count times the

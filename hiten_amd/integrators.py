@@ -220,12 +220,11 @@ class _FixedStepRK(_Integrator):
             t_end = float(r.t_hit[0]) if bool(r.hit[0]) else float(t_vals[-1])
             return _Solution(times=np.array([t_vals[0], t_end]),
                              states=np.vstack([y0, r.y_hit[0].cpu().numpy()]))
-        fin, traj = batch.rk_fixed(mu, y0[None, :], None, None, 0, order=self._p, dim=dim,
-                                   t_vals=t_vals, forward=fwd, flip_indices=flip,
-                                   return_trajectory=True)
-        states = traj[0].cpu().numpy()
-        derivs = _derivatives(system, mu, dim, fwd, flip, states)
-        return _Solution(times=t_vals.copy(), states=states, derivatives=derivs)
+        fin, traj, der = batch.rk_fixed(mu, y0[None, :], None, None, 0, order=self._p, dim=dim,
+                                        t_vals=t_vals, forward=fwd, flip_indices=flip,
+                                        return_derivatives=True)
+        return _Solution(times=t_vals.copy(), states=traj[0].cpu().numpy(),
+                         derivatives=der[0].cpu().numpy())
 
 
 class _RK4(_FixedStepRK):

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define HH_ABI_VERSION 2
+#define HH_ABI_VERSION 3
 
 #define HH_OK 0
 #define HH_ERR_ARGS 1
@@ -110,12 +110,16 @@ int hh_rk8_fixed(int n, int dim, const double* mu, const double* t0, const doubl
  * Replaces: _RK4 / _RK6 / _RK8 (integrators/rk.py:755-783, tableaux
  * integrators/coefficients/rk4.py, rk6.py, rk8.py) through
  * _FixedStepRK._integrate_fixed_rk (rk.py:531-588), i.e.
- * _propagate_dynsys(..., method="fixed", order=4|6|8).  Other arguments as
- * hh_rk8_fixed.                                                            */
+ * _propagate_dynsys(..., method="fixed", order=4|6|8).
+ *   deriv_out  : [steps][dim][n] the derivative f(t, y) at every grid state
+ *                (the `derivs` _integrate_fixed_rk returns beside the
+ *                states, rk.py:575-588: each step's first stage, and one
+ *                more evaluation at the last state), or NULL
+ * Other arguments as hh_rk8_fixed.                                         */
 int hh_rk_fixed(int order, int n, int dim, const double* mu, const double* t0, const double* tf,
                 int steps, const double* t_vals, int fwd, unsigned long long flip_mask,
-                const double* y0, double* y_out, double* traj_out, int variant, int block,
-                void* stream);
+                const double* y0, double* y_out, double* traj_out, double* deriv_out,
+                int variant, int block, void* stream);
 
 /* Fixed-step RK of the given order (4, 6 or 8) on the 6-d field until the
  * plane event g = y[ev_index] - ev_offset crosses zero in `direction`

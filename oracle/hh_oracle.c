@@ -823,11 +823,13 @@ static void set_threads(int n_threads) {
 
 int or_batch_rk8_final(int n, const double *mu, const double *y0, double t0, double tf, int steps,
                        double *y_out, int n_threads) {
-    return or_batch_rk_final(8, n, mu, y0, t0, tf, steps, y_out, n_threads);
+    return or_batch_rk_final(8, n, mu, y0, t0, tf, steps, y_out, n_threads, 0);
 }
 
+/* with_derivs: also form every grid state's derivative, as the reference's
+ * fixed driver does (rk.py:575-588); only the final states are returned */
 int or_batch_rk_final(int order, int n, const double *mu, const double *y0, double t0, double tf,
-                      int steps, double *y_out, int n_threads) {
+                      int steps, double *y_out, int n_threads, int with_derivs) {
     if (!rk_tableau(order)) return OR_ERR_ARGS;
     set_threads(n_threads);
     double *tv = malloc(sizeof(double) * steps);
@@ -837,18 +839,20 @@ int or_batch_rk_final(int order, int n, const double *mu, const double *y0, doub
 #pragma omp parallel
     {
         double *states = malloc(sizeof(double) * 6 * (size_t)steps);
-        if (!states) {
+        double *derivs = with_derivs ? malloc(sizeof(double) * 6 * (size_t)steps) : NULL;
+        if (!states || (with_derivs && !derivs)) {
 #pragma omp atomic write
             err = OR_ERR_NOMEM;
         } else {
 #pragma omp for schedule(dynamic, 4)
             for (int i = 0; i < n; ++i) {
                 or_sys s = {OR_KIND_CR3BP6, 6, mu[i], 1, 0, 0, 0, 0, 0, 0, 0, 0};
-                or_rk_fixed_order(order, &s, y0 + (size_t)i * 6, tv, steps, states, NULL);
+                or_rk_fixed_order(order, &s, y0 + (size_t)i * 6, tv, steps, states, derivs);
                 memcpy(y_out + (size_t)i * 6, states + (size_t)(steps - 1) * 6, sizeof(double) * 6);
             }
-            free(states);
         }
+        free(states);
+        free(derivs);
     }
     free(tv);
     return err;

@@ -115,7 +115,7 @@ double or_dop853_single_step(or_sys *s, double t, const double *y, double h, dou
 int or_batch_rk8_final(int n, const double *mu, const double *y0, double t0, double tf,
                        int steps, double *y_out, int n_threads);
 int or_batch_rk_final(int order, int n, const double *mu, const double *y0, double t0, double tf,
-                      int steps, double *y_out, int n_threads);
+                      int steps, double *y_out, int n_threads, int with_derivs);
 int or_batch_dop853_final(int n, int dim, const double *mu, const double *y0,
                           const double *t0, const double *tf, double rtol, double atol,
                           double max_step, long long max_steps, double *y_out,

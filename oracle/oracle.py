@@ -115,7 +115,7 @@ def _declare(L):
     L.or_dop853_single_step.restype = C.c_double
     L.or_batch_rk8_final.argtypes = [C.c_int, _D, _D, C.c_double, C.c_double, C.c_int, _D, C.c_int]
     L.or_batch_rk_final.argtypes = [C.c_int, C.c_int, _D, _D, C.c_double, C.c_double, C.c_int,
-                                    _D, C.c_int]
+                                    _D, C.c_int, C.c_int]
     L.or_rk_fixed_order.argtypes = [C.c_int, C.POINTER(OrSys), _D, _D, C.c_int, _D, _D]
     L.or_batch_dop853_final.argtypes = [C.c_int, C.c_int, _D, _D, _D, _D, C.c_double, C.c_double,
                                         C.c_double, C.c_longlong, _D, _L, _L, _I, C.c_int]
@@ -407,13 +407,16 @@ def plane_crossing(mu, x0, forward=1, t_guess=None, t_window=None, index=1, offs
 # batched (CPU baseline)
 # ---------------------------------------------------------------------------
 
-def batch_rk8_final(mu, y0, t0, tf, steps, n_threads=0, order=8):
+def batch_rk8_final(mu, y0, t0, tf, steps, n_threads=0, order=8, with_derivs=False):
+    """Final states of the fixed-step driver over linspace(t0, tf, steps);
+    ``with_derivs`` also forms every grid state's derivative, as the
+    reference's driver does (the work it times), without returning them."""
     y0 = _f64(y0)
     n = y0.shape[0]
     mu = _f64(np.broadcast_to(mu, (n,)))
     out = np.empty((n, 6))
     rc = lib().or_batch_rk_final(int(order), n, _p(mu), _p(y0), float(t0), float(tf), int(steps),
-                                 _p(out), int(n_threads))
+                                 _p(out), int(n_threads), int(bool(with_derivs)))
     if rc != 0:
         raise ValueError(f"or_batch_rk_final: order {order} (error {rc})")
     return out

@@ -71,7 +71,8 @@ def test_no_gpu_raises():
 def test_rk_fixed_order_checked_without_gpu():
     from hiten_amd import _lib
     lib = _lib.load(require_gpu=False)
-    rc = lib.hh_rk_fixed(5, 4, 6, None, None, None, 10, None, 1, 0, None, None, None, 0, 0, None)
+    rc = lib.hh_rk_fixed(5, 4, 6, None, None, None, 10, None, 1, 0, None, None, None, None, 0, 0,
+                         None)
     assert rc == 1 and b"order must be 4, 6 or 8" in lib.hh_last_error()
 
 

@@ -132,6 +132,45 @@ def test_rk8_fixed_matches_oracle(gpu, oracle, variant):
 
 
 @pytest.mark.parametrize("variant", ["plain", "wide"])
+@pytest.mark.parametrize("order", [4, 6, 8])
+def test_rk_fixed_derivatives_match_oracle(gpu, oracle, variant, order):
+    """_integrate_fixed_rk returns the derivative at every grid state beside
+    the states (rk.py:575-588): each step's first stage and one more
+    evaluation at the last state; forward and backward time (flipped
+    components), and a constant grid (every state and derivative y0's)."""
+    say(f"hh_rk_fixed order {order} {variant}: states and derivatives on the grid")
+    w = W.rk8_grid(n=24)
+    for fwd, t1, steps in ((1, 2.0 * np.pi, 400), (-1, 1.5, 150), (1, 0.0, 5)):
+        fin, traj, der = batch.rk_fixed(w["mu"], w["y0"], 0.0, t1, steps, order=order,
+                                        forward=fwd, return_derivatives=True, variant=variant)
+        traj, der = traj.cpu().numpy(), der.cpu().numpy()
+        tv = np.linspace(0.0, t1, steps)
+        for i in (0, 7, 23):
+            sysm = oracle.System("cr3bp6", mu=w["mu"], fwd=fwd)
+            if t1 == 0.0:
+                f0 = sysm.rhs(w["y0"][i])
+                st, dv = np.repeat(w["y0"][i][None], steps, 0), np.repeat(f0[None], steps, 0)
+            else:
+                st, dv = oracle.rk_fixed(sysm, w["y0"][i], tv, derivs=True, order=order)
+            assert_bitwise(traj[i], st, f"rk{order} grid states traj {i} fwd {fwd}")
+            assert_bitwise(der[i], dv, f"rk{order} grid derivatives traj {i} fwd {fwd}")
+
+
+def test_rk_fixed_derivatives_42d(gpu, oracle):
+    say("hh_rk_fixed order 8, dim 42: states and derivatives on the grid")
+    w = W.stm_uneven(n=16)
+    fin, traj, der = batch.rk_fixed(w["mu"], w["y0"], 0.0, 1.0, 120, order=8, dim=42,
+                                    return_derivatives=True)
+    traj, der = traj.cpu().numpy(), der.cpu().numpy()
+    tv = np.linspace(0.0, 1.0, 120)
+    for i in (0, 9, 15):
+        st, dv = oracle.rk_fixed(oracle.System("cr3bp42", mu=w["mu"]), w["y0"][i], tv,
+                                 derivs=True)
+        assert_bitwise(traj[i], st, f"rk8 42-d grid states traj {i}")
+        assert_bitwise(der[i], dv, f"rk8 42-d grid derivatives traj {i}")
+
+
+@pytest.mark.parametrize("variant", ["plain", "wide"])
 def test_rk8_backward_flip(gpu, oracle, variant):
     say(f"hh_rk8_fixed {variant}: backward time with all components flipped")
     w = W.rk8_grid(n=64)
