@@ -396,12 +396,14 @@ __device__ __forceinline__ double corr_stm_start(const Wave42& W, const hh::Newt
  * event time, which the first group always does.                         */
 __global__ __launch_bounds__(192) void k_correct(CorrArgs a) {
     __shared__ CorrLds L;
+    __shared__ PowTab powt;
     const int i = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int grp = tid >> 3;
     const int wid = __builtin_amdgcn_readfirstlane((int)(tid >> 6));
     if (i >= a.n) return;
+    pow_tab_fill(&powt);
     if (tid == 0) {
         const hh_correction_t& pb = a.prob[i];
         L.fail = 1;
@@ -425,10 +427,10 @@ __global__ __launch_bounds__(192) void k_correct(CorrArgs a) {
     if (!L.fail) {
         const double mu = L.row.mu;
         Group6 ga, ge;
-        ga.setup(lane, mu, L.row.fwd, a.x.flip_align);
-        ge.setup(lane, mu, 1, 0ULL);
+        ga.setup(lane, mu, L.row.fwd, a.x.flip_align, &powt);
+        ge.setup(lane, mu, 1, 0ULL, &powt);
         Wave42 W;
-        W.setup(lane, mu, 1, a.flip_stm);
+        W.setup(lane, mu, 1, a.flip_stm, &powt);
         for (;;) {
             /* the residual, or the line-search trials of the step; the STM
              * wave meanwhile integrates state 0's STM                      */
@@ -572,13 +574,16 @@ __global__ __launch_bounds__(BLOCK) void k_plane_crossing(int n, const double* _
                                                           double* __restrict__ t_hit,
                                                           double* __restrict__ y_hit,
                                                           int* __restrict__ n_steps) {
+    __shared__ PowTab powt;
+    pow_tab_fill(&powt);
+    __syncthreads();
     const long long gtid = (long long)blockIdx.x * BLOCK + threadIdx.x;
     const int i = (int)(gtid >> 3);
     if (i >= n) return;
     const int lane = threadIdx.x & 63;
     Group6 ga, ge;
-    ga.setup(lane, mu[i], fwd, x.flip_align);
-    ge.setup(lane, mu[i], 1, 0ULL);
+    ga.setup(lane, mu[i], fwd, x.flip_align, &powt);
+    ge.setup(lane, mu[i], 1, 0ULL, &powt);
     const int d = ge.d;
     const double y = ge.act ? y0[(size_t)d * n + i] : 0.0;
     int okv, steps;

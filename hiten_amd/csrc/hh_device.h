@@ -97,6 +97,33 @@ static __device__ __forceinline__ void hh_dev_pow15_25(double x, double* p15, do
 
 namespace hh {
 
+/* hh_pow's five tables in one block.  The latency layouts (one trajectory
+ * per wave, pair or 8-lane group) read them from an LDS copy that the
+ * kernel fills at its start: their step-size factors sit on the
+ * trajectory's dependent chain, and two rounds of table loads from L2
+ * under a full GPU cost more than the rest of the pow.                   */
+struct PowTab {
+    double invc[128], log_hi[128], log_lo[128], exp_hi[128], exp_lo[128];
+};
+
+/* copy the tables into t with the workgroup's threads; a barrier must
+ * follow before any thread reads t                                       */
+__device__ __forceinline__ void pow_tab_fill(PowTab* t) {
+    const int nt = (int)(blockDim.x * blockDim.y * blockDim.z);
+    for (int i = (int)threadIdx.x; i < 128; i += nt) {
+        t->invc[i] = HH_POW_INVC[i];
+        t->log_hi[i] = HH_POW_LOG_HI[i];
+        t->log_lo[i] = HH_POW_LOG_LO[i];
+        t->exp_hi[i] = HH_POW_EXP_HI[i];
+        t->exp_lo[i] = HH_POW_EXP_LO[i];
+    }
+}
+
+/* hh_pow on the tables at t (same bits as hh_pow) */
+__device__ __forceinline__ double pow_tab(const PowTab* t, double x, double y) {
+    return hh_pow_t(x, y, t->invc, t->log_hi, t->log_lo, t->exp_hi, t->exp_lo);
+}
+
 constexpr int DOP_S = 12;     /* stages of one DOP853 step                 */
 constexpr int DOP_K = 13;     /* stored stage derivatives (k[12] = f(y_high)) */
 constexpr int DOP_SX = 16;    /* stages of the dense-output extension       */
@@ -334,6 +361,37 @@ __device__ __forceinline__ double pi_reject_factor(double err) {
     if (f < 0.2) f = 0.2;
     if (f > 10.0) f = 10.0;
     return f;
+}
+
+/* pi_accept_factor(err, err_prev) and pi_reject_factor(err) of an aligned
+ * 8-lane group holding one trajectory (err, err_prev group-uniform; every
+ * lane of the group active): entry b of the group forms one of the three
+ * powers (b = 0: err^-beta, b = 1: err_prev^alpha, b >= 2: err^(-1/8)), so
+ * a wave forms one pow per lane whatever its groups decide, where the two
+ * factors formed in their branches take up to three pows one after the
+ * other.  Same expressions on the same powers: same bits.                 */
+__device__ __forceinline__ void group_step_factors(int b, double err, double err_prev,
+                                                   double& f_acc, double& f_rej) {
+    const double beta = 1.0 / (8.0 + 1.0);
+    const double alpha = 0.4 * beta;
+    const double err_exp = 1.0 / 8.0;
+    const double px = (b == 1) ? err_prev : err;
+    const double pe = (b == 0) ? -beta : ((b == 1) ? alpha : -err_exp);
+    const double pw = hh_pow(px, pe);
+    const double pb = group_get<0>(pw), pa = group_get<1>(pw), pr = group_get<2>(pw);
+    double f;
+    if (err == 0.0) f = 10.0;
+    else if (err_prev < 0.0) f = 0.9 * pb;
+    else f = (0.9 * pb) * pa;
+    if (!(f == f)) f = 10.0;
+    if (f < 0.2) f = 0.2;
+    if (f > 10.0) f = 10.0;
+    f_acc = f;
+    double g = (err <= 0.0) ? 0.2 : 0.9 * pr;
+    if (!(g == g)) g = 0.2;
+    if (g < 0.2) g = 0.2;
+    if (g > 10.0) g = 10.0;
+    f_rej = g;
 }
 
 /* rk.py:2443-2448 */

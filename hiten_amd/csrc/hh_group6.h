@@ -37,10 +37,12 @@ struct Group6 {
     double c1;                /* one_mu, or -one_mu on lane 5                 */
     double c2;                /* 2.0 on lane 3, -2.0 otherwise                */
     unsigned long long sign;  /* bit 63: this lane's derivative is negated    */
+    const PowTab* pt;         /* hh_pow's tables (an LDS copy)                */
 
     __device__ __forceinline__ void setup(int lane_, double mu_, int fwd,
-                                          unsigned long long flip) {
+                                          unsigned long long flip, const PowTab* pt_) {
         lane = lane_;
+        pt = pt_;
         d = lane & 7;
         act = d < 6;
         odd = (d & 1) != 0;
@@ -164,7 +166,7 @@ struct Group6 {
         const double err_exp = 1.0 / 8.0;
         const double px = (d == 1) ? err_prev : err;
         const double pe = (d == 0) ? -beta : ((d == 1) ? alpha : -err_exp);
-        const double pw = hh_pow(px, pe);
+        const double pw = pow_tab(pt, px, pe);
         const double pb = group_get<0>(pw), pa = group_get<1>(pw), pr = group_get<2>(pw);
         double f;
         if (err == 0.0) f = 10.0;

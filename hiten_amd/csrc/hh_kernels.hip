@@ -206,6 +206,10 @@ __device__ __forceinline__ void traj_iterate(const P& p, Traj<P>& s, const StepP
         o.stop(s.y);
         return;
     }
+    /* Lane42: both factors from one pow per lane of the group, formed
+     * before the group-uniform branch; Lane6: the factor its branch needs */
+    double f_acc = 0.0, f_rej = 0.0;
+    if constexpr (P::N == 42) group_step_factors(p.blk, err, s.err_prev, f_acc, f_rej);
     if (err <= 1.0) {
         const double t_new = s.t + h;
         const bool last = !((t_new - s.tf) * 1.0 < 0.0);
@@ -217,11 +221,13 @@ __device__ __forceinline__ void traj_iterate(const P& p, Traj<P>& s, const StepP
             s.y[d] = yh[d];
             s.k[0][d] = s.k[DOP_S][d];
         }
-        s.h = h * pi_accept_factor(err, s.err_prev);
+        if constexpr (P::N != 42) f_acc = pi_accept_factor(err, s.err_prev);
+        s.h = h * f_acc;
         s.err_prev = err;
         s.nacc += 1;
     } else {
-        s.h = clamp_step(h * pi_reject_factor(err), sp.max_step, sp.min_step);
+        if constexpr (P::N != 42) f_rej = pi_reject_factor(err);
+        s.h = clamp_step(h * f_rej, sp.max_step, sp.min_step);
         s.nrej += 1;
     }
 }
@@ -459,7 +465,7 @@ __global__ __launch_bounds__(BLOCK) void k_rk_fixed6_group(
     const int i = (int)(gtid >> 3);
     if (i >= n) return;
     Group6 p;
-    p.setup(threadIdx.x & 63, mu[i], fwd, flip);
+    p.setup(threadIdx.x & 63, mu[i], fwd, flip, nullptr);   /* no step control */
     const int d = p.d;
     double y = p.act ? y0[(size_t)d * n + i] : 0.0;
     const double t0 = t0a[i], tf = tfa[i];
@@ -665,11 +671,14 @@ __global__ __launch_bounds__(BLOCK) void k_dop853_run6_group(
     const double* __restrict__ y0, double* __restrict__ yout, int m,
     const double* __restrict__ teval, int* __restrict__ status, int* __restrict__ nacc,
     int* __restrict__ nrej) {
+    __shared__ PowTab powt;
+    pow_tab_fill(&powt);
+    __syncthreads();
     const long long gtid = (long long)blockIdx.x * BLOCK + threadIdx.x;
     const int i = (int)(gtid >> 3);
     if (i >= n) return;
     Group6 p;
-    p.setup(threadIdx.x & 63, mu[i], fwd, flip);
+    p.setup(threadIdx.x & 63, mu[i], fwd, flip, &powt);
     const int d = p.d;
     const double y = p.act ? y0[(size_t)d * n + i] : 0.0;
     double yfin;
@@ -881,11 +890,14 @@ __global__ __launch_bounds__(BLOCK) void k_dop853_event6_group(
     unsigned long long flip, const double* __restrict__ y0, int* __restrict__ hit,
     double* __restrict__ t_hit, double* __restrict__ y_hit, double* __restrict__ y_last,
     int* __restrict__ status, int* __restrict__ nacc, int* __restrict__ nrej) {
+    __shared__ PowTab powt;
+    pow_tab_fill(&powt);
+    __syncthreads();
     const long long gtid = (long long)blockIdx.x * BLOCK + threadIdx.x;
     const int i = (int)(gtid >> 3);
     if (i >= n) return;
     Group6 p;
-    p.setup(threadIdx.x & 63, mu[i], fwd, flip);
+    p.setup(threadIdx.x & 63, mu[i], fwd, flip, &powt);
     const int d = p.d;
     const double y = p.act ? y0[(size_t)d * n + i] : 0.0;
     EventOut6 o;
@@ -1079,11 +1091,11 @@ __device__ __forceinline__ void wide_finish(const Final42Args& a, const WL& W, i
  * Raises the wave's issue priority meanwhile: such a trajectory is the
  * longest-running work on its SIMD.                                         */
 template <class Hook = NoHook>
-__device__ __forceinline__ void run_wide(const Final42Args& a, int lane, int src, int traj_lane,
-                                         const Lane42& p, const Traj<Lane42>& s,
+__device__ __forceinline__ void run_wide(const Final42Args& a, const PowTab* pt, int lane, int src,
+                                         int traj_lane, const Lane42& p, const Traj<Lane42>& s,
                                          Hook hook = Hook()) {
     Wave42 W;
-    W.setup(lane, rlane(p.mu, src), a.fwd, a.flip);
+    W.setup(lane, rlane(p.mu, src), a.fwd, a.flip, pt);
     const int from = src + (lane >> 3);
     double y = 0.0;
     double k[DOP_K];
@@ -1110,11 +1122,14 @@ __device__ __forceinline__ void run_wide(const Final42Args& a, int lane, int src
  * traj_init (rk.py:2441-2448) in the wave-wide layout, then wide_finish.    */
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_dop853_final42_wide(Final42Args a) {
+    __shared__ PowTab powt;
+    pow_tab_fill(&powt);
+    __syncthreads();
     const int traj = (int)((blockIdx.x * (unsigned)BLOCK + threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
     if (traj >= a.n) return;
     Wave42 W;
-    W.setup(lane, a.mu[traj], a.fwd, a.flip);
+    W.setup(lane, a.mu[traj], a.fwd, a.flip, &powt);
     const double y0 = W.act ? a.y0[(size_t)comp42(lane >> 3, W.d) * a.n + traj] : 0.0;
     const double t0 = a.t0[traj], tf = a.tf[traj];
     if (isclose(t0, tf) || !((t0 - tf) * 1.0 < 0.0)) {
@@ -1178,6 +1193,9 @@ __device__ __forceinline__ void pair_final(const Pair42& P, const StepParams& sp
 
 __global__ __launch_bounds__(128) void k_dop853_final42_pair(Final42Args a) {
     __shared__ PairLds lds;
+    __shared__ PowTab powt;
+    pow_tab_fill(&powt);
+    __syncthreads();
     const int traj = blockIdx.x;
     const int lane = threadIdx.x & 63;
     /* wave-uniform in a scalar register, so that every branch on the role is
@@ -1185,7 +1203,7 @@ __global__ __launch_bounds__(128) void k_dop853_final42_pair(Final42Args a) {
     const int role = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (traj >= a.n) return;
     Pair42 P;
-    P.setup(lane, role, a.mu[traj], a.fwd, a.flip, &lds);
+    P.setup(lane, role, a.mu[traj], a.fwd, a.flip, &lds, &powt);
     const double y0 = P.holds() ? a.y0[(size_t)P.comp() * a.n + traj] : 0.0;
     double out;
     int status, nacc, nrej;
@@ -1207,6 +1225,9 @@ __global__ __launch_bounds__(128) void k_dop853_final42_pair(Final42Args a) {
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_dop853_final42_persist(Final42Args a) {
+    __shared__ PowTab powt;
+    pow_tab_fill(&powt);
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int b = lane & 7;
     const int grp = lane >> 3;
@@ -1263,7 +1284,7 @@ void k_dop853_final42_persist(Final42Args a) {
         const unsigned long long live = __ballot(traj >= 0 && b == 0);
         if (__popcll(live) == 1 && (hold || __any(exhausted))) {
             /* one trajectory left in the wave: finish it one component per lane */
-            run_wide(a, lane, __ffsll((long long)live) - 1, traj, p, s);
+            run_wide(a, &powt, lane, __ffsll((long long)live) - 1, traj, p, s);
             traj = -1;
             continue;
         }
@@ -1437,10 +1458,10 @@ __device__ __forceinline__ void resume42(const Final42Args& a, const ResumeArgs&
 
 /* Finish the handed-over trajectory with both waves of the workgroup */
 __device__ __forceinline__ void run_pair(const Final42Args& a, const ResumeArgs& r,
-                                         const PairHandoff& ho, PairLds* plds, int wid,
-                                         int lane, int key) {
+                                         const PairHandoff& ho, PairLds* plds, const PowTab* pt,
+                                         int wid, int lane, int key) {
     Pair42 P;
-    P.setup(lane, wid, ho.mu, a.fwd, a.flip, plds);
+    P.setup(lane, wid, ho.mu, a.fwd, a.flip, plds, pt);
     const int bb = lane >> 3, dd = lane & 7;
     const double y = P.holds() ? ho.y[bb][dd] : 0.0;
     double k[DOP_K];
@@ -1477,11 +1498,13 @@ void k_dop853_final42_resume(Final42Args a, ResumeArgs r) {
     __shared__ PairLds plds;
     __shared__ PairHandoff ho;
     __shared__ int pair_state;
+    __shared__ PowTab powt;
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int b = lane & 7;
     const int grp = lane >> 3;
     if (threadIdx.x == 0) pair_state = -1;
+    pow_tab_fill(&powt);
     __syncthreads();
     const int key = simd_key();
     Lane42 p;
@@ -1550,7 +1573,7 @@ void k_dop853_final42_resume(Final42Args a, ResumeArgs r) {
             }
             traj = -1;
             __syncthreads();
-            run_pair(a, r, ho, &plds, wid, lane, key);
+            run_pair(a, r, ho, &plds, &powt, wid, lane, key);
             __syncthreads();
             if (threadIdx.x == 0) pair_state = -1;
             __syncthreads();
@@ -1566,7 +1589,7 @@ void k_dop853_final42_resume(Final42Args a, ResumeArgs r) {
             if (!moved) traj = -1;
             const int tr = __builtin_amdgcn_readlane(traj, src);
             const double t0 = a.t0[tr];
-            run_wide(a, lane, src, traj, p, s,
+            run_wide(a, &powt, lane, src, traj, p, s,
                      SimdReserve{r.simd_tab, key, r.exclusive, lane, t0, a.tf[tr] - t0, false});
             if (lane == 0) atomicSub(&r.ctr[3], 1);
             traj = -1;
@@ -2102,8 +2125,10 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
                 128, 2 * wpe, &g);
             if (rc != HH_OK) return rc;
         }
-        const long long max_grid = ((long long)n * 8 + 127) / 128;
-        if (g > max_grid) g = (int)max_grid;
+        /* at most one workgroup per trajectory: each one may end up taking a
+         * workgroup's pair of waves, and with fewer workgroups than heavy
+         * trajectories those would take the pairs one after the other      */
+        if (g > n) g = n;
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, r.ctr, 0, RQ_CTR);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, r.ctr + 3, n, 1);
         hipLaunchKernelGGL(k_fill_int, dim3((r.cap + 255) / 256), dim3(256), 0, st, r.rq, -1, r.cap);
@@ -2122,7 +2147,8 @@ int hh_dop853_final(int n, int dim, const double* mu, const double* t0, const do
         const int rc = resident_grid(fn, block, 0, &grid);
         if (rc != HH_OK) return rc;
     }
-    const long long max_grid = ((long long)n * 8 + block - 1) / block;
+    /* at most one wave per trajectory (each one may end up taking a wave) */
+    const long long max_grid = ((long long)n * 64 + block - 1) / block;
     if (grid > max_grid) grid = (int)max_grid;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, a.counter, 0, 1);
     switch (block) {

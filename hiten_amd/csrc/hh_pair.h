@@ -45,8 +45,9 @@ struct Pair42 {
     mutable int seq;        /* right-hand sides evaluated so far            */
 
     __device__ __forceinline__ void setup(int lane, int role_, double mu, int fwd,
-                                          unsigned long long flip, PairLds* l) {
-        w.setup(lane, mu, fwd, flip);
+                                          unsigned long long flip, PairLds* l,
+                                          const PowTab* pt) {
+        w.setup(lane, mu, fwd, flip, pt);
         role = role_;
         lds = l;
         seq = 0;

@@ -19,6 +19,8 @@
  *                             before the final rounding), i.e. correctly
  *                             rounded except in rare near-midpoint cases, the
  *                             same accuracy class as glibc's pow.
+ *   hh_pow_t(x, y, tables)    hh_pow on tables passed by pointer (a kernel's
+ *                             LDS copy of them); same bits.
  *
  * Contract: translation units including this header must be compiled with
  * -ffp-contract=off (no implicit fma), and fma() must be a true fused
@@ -130,8 +132,11 @@ HH_FN void hh_pow15_25(double x, double* p15, double* p25) {
 }
 
 /* x^y for finite y.  Special cases follow C99 pow for the cases a step-size
- * controller can produce (x = 0, inf, NaN, 1, negative).                    */
-HH_FN double hh_pow(double x, double y) {
+ * controller can produce (x = 0, inf, NaN, 1, negative).  The five tables
+ * come in as pointers, so that a device caller can read copies of them from
+ * LDS; hh_pow below reads them where they are defined.                      */
+HH_FN double hh_pow_t(double x, double y, const double* invc, const double* log_hi,
+                      const double* log_lo, const double* exp_hi, const double* exp_lo) {
     if (y == 0.0) return 1.0;
     if (x != x || y != y) return x + y;               /* NaN                */
     if (x == 1.0) return 1.0;
@@ -154,7 +159,7 @@ HH_FN double hh_pow(double x, double y) {
     int i = (int)((ux >> 45) & 0x7f);
     double c = 1.0 + (double)i * 0.0078125;
     double dlt = m - c;                               /* exact (Sterbenz)   */
-    double ic = HH_POW_INVC[i];                       /* RN(1 / c_i)        */
+    double ic = invc[i];                       /* RN(1 / c_i)        */
     double zh = dlt * ic;
     /* c_i has 8 significant bits, so dlt - zh*c_i is exact in one fma      */
     double zl = __builtin_fma(-zh, c, dlt) * ic;
@@ -172,9 +177,9 @@ HH_FN double hh_pow(double x, double y) {
     /* L = k ln2 + log(c_i) + log1p(z) */
     double kd = (double)k;
     double s1, e1, s2, e2;
-    HH_TWO_SUM(kd * HH_POW_LN2_HI, HH_POW_LOG_HI[i], s1, e1);
+    HH_TWO_SUM(kd * HH_POW_LN2_HI, log_hi[i], s1, e1);
     HH_TWO_SUM(s1, a_hi, s2, e2);
-    double l_lo = ((e1 + e2) + (kd * HH_POW_LN2_LO + HH_POW_LOG_LO[i])) + (a_lo + small);
+    double l_lo = ((e1 + e2) + (kd * HH_POW_LN2_LO + log_lo[i])) + (a_lo + small);
     double l_hi, l_lo2;
     HH_TWO_SUM(s2, l_lo, l_hi, l_lo2);
 
@@ -200,8 +205,8 @@ HH_FN double hh_pow(double x, double y) {
     HH_TWO_SUM(1.0, r, e_hi, e_lo);
     e_lo = e_lo + q;
     double p_hi, p_lo;
-    HH_TWO_PROD(HH_POW_EXP_HI[j], e_hi, p_hi, p_lo);
-    p_lo = p_lo + (HH_POW_EXP_HI[j] * e_lo + HH_POW_EXP_LO[j] * e_hi);
+    HH_TWO_PROD(exp_hi[j], e_hi, p_hi, p_lo);
+    p_lo = p_lo + (exp_hi[j] * e_lo + exp_lo[j] * e_hi);
     double res = p_hi + p_lo;
     /* scale by 2^e in two exact steps (keeps each factor a normal number) */
     int e1s = e / 2;
@@ -209,6 +214,10 @@ HH_FN double hh_pow(double x, double y) {
     res = res * hh_from_bits((HH_U64)(e1s + 1023) << 52);
     res = res * hh_from_bits((HH_U64)(e2s + 1023) << 52);
     return res;
+}
+
+HH_FN double hh_pow(double x, double y) {
+    return hh_pow_t(x, y, HH_POW_INVC, HH_POW_LOG_HI, HH_POW_LOG_LO, HH_POW_EXP_HI, HH_POW_EXP_LO);
 }
 
 #endif /* HH_POW_H */

@@ -46,10 +46,12 @@ struct Wave42 {
     double cl;                /* mu (even lanes) or -mu2 (odd lanes): x + cl is
                                  x + mu or x - mu2 exactly                       */
     unsigned long long sign;  /* bit 63: this lane's derivative is negated       */
+    const PowTab* pt;         /* hh_pow's tables (an LDS copy)                   */
 
     __device__ __forceinline__ void setup(int lane_, double mu_, int fwd,
-                                          unsigned long long flip) {
+                                          unsigned long long flip, const PowTab* pt_) {
         lane = lane_;
+        pt = pt_;
         const int b = lane >> 3;
         d = lane & 7;
         st = (b == 0);
@@ -203,7 +205,7 @@ struct Wave42 {
         const double err_exp = 1.0 / 8.0;
         const double px = (lane == 1) ? err_prev : err;
         const double pe = (lane == 0) ? -beta : ((lane == 1) ? alpha : -err_exp);
-        const double pw = hh_pow(px, pe);
+        const double pw = pow_tab(pt, px, pe);
         const double pb = rlane(pw, 0), pa = rlane(pw, 1), pr = rlane(pw, 2);
         double f;
         if (err == 0.0) f = 10.0;
