@@ -58,6 +58,37 @@ def test_cond2_matches_numpy_and_its_threshold_decisions():
     assert L2.cond2([1.0, 2.0, 2.0, 4.0]) == np.inf
 
 
+# Exactly singular steps: J + 1e-12 I rounds back to J (diagonal entries of
+# at least 2^14, whose ulp exceeds 2e-12) and getf2's Schur complement is
+# exactly 0 (a power-of-two pivot), so the reference's solve raises and it
+# takes numpy.linalg.lstsq(rcond=None), LAPACK dgelsd
+SINGULAR_J = [2.0**20 * np.array([[4.0, 2.0], [2.0, 1.0]]),
+              2.0**20 * np.array([[1.0, 1.0], [1.0, 1.0]]),
+              2.0**30 * np.array([[2.0, -4.0], [-1.0, 2.0]]),
+              2.0**16 * np.array([[8.0, -2.0], [-4.0, 1.0]]),
+              2.0**24 * np.array([[-1.0, 3.0], [0.5, -1.5]])]
+
+
+def test_singular_step_is_the_minimum_norm_solution():
+    """The lstsq branch of _solve_delta_dense (backends/base.py:227-230) on
+    rank-1 systems, pinned by the closed form of the minimum-norm solution:
+    for A = s u v^T, A^+ = A^T / |A|_F^2, so delta = -A^T r / |A|_F^2."""
+    rng = np.random.default_rng(11)
+    for J in SINGULAR_J:
+        J_reg = J + np.eye(2) * 1e-12
+        assert np.array_equal(J_reg, J)
+        assert np.linalg.cond(J) > 1e8
+        with pytest.raises(np.linalg.LinAlgError):
+            np.linalg.solve(J_reg, np.ones(2))
+        for _ in range(200):
+            r = rng.standard_normal(2)
+            got = Cr.solve_delta_dense(J, r)
+            want = -(J.T @ r) / np.sum(J * J)
+            # to rounding on the solution's scale |r| / |A|_F = |r| |A^+|
+            scale = np.linalg.norm(r) / np.sqrt(np.sum(J * J))
+            assert np.max(np.abs(got - want)) <= 4e-15 * scale
+
+
 def test_record_layouts_match_the_library():
     lib = _lib.load(require_gpu=False)
     a, b = C.c_int(), C.c_int()

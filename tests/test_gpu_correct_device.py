@@ -87,6 +87,34 @@ def test_step_solve_is_numpys_bit_for_bit(gpu):
     assert ((c > 1e8) == (cw > 1e8)).all()
 
 
+def test_step_solve_singular_is_lstsq(gpu):
+    """Exactly singular regularised systems take the least-squares branch:
+    the device's closed-form minimum-norm solution (one Jacobi rotation)
+    against numpy.linalg.lstsq (LAPACK dgelsd), which forms it differently:
+    equal to rounding, not bit for bit."""
+    from test_correct_device_cpu import SINGULAR_J
+    say(f"hh_eval_solve2: {len(SINGULAR_J)} exactly singular systems x 20 residuals vs lstsq")
+    rng = np.random.default_rng(12)
+    J = np.stack([j for j in SINGULAR_J for _ in range(20)])
+    r = rng.standard_normal((len(J), 2)) * np.exp(rng.uniform(-6, 6, (len(J), 1)))
+    n = len(J)
+    dev = torch.device("cuda:0")
+    Jd = torch.from_numpy(J.reshape(n, 4).copy()).to(dev)
+    rd = torch.from_numpy(r.copy()).to(dev)
+    dd = torch.empty((n, 2), dtype=torch.float64, device=dev)
+    cd = torch.empty(n, dtype=torch.float64, device=dev)
+    lib = _lib.load()
+    _lib.check(lib.hh_eval_solve2(n, Jd.data_ptr(), rd.data_ptr(), dd.data_ptr(), cd.data_ptr(),
+                                  None), "hh_eval_solve2")
+    d, c = dd.cpu().numpy(), cd.cpu().numpy()
+    assert np.isinf(c).all()
+    for i in range(n):
+        want = Cr.solve_delta_dense(J[i], r[i])
+        # to rounding on the solution's scale |r| / |A|_F = |r| |A^+|
+        scale = np.linalg.norm(r[i]) / np.sqrt(np.sum(J[i] * J[i]))
+        assert np.max(np.abs(d[i] - want)) <= 2e-14 * scale, i
+
+
 def test_device_corrector_matches_the_bank_and_the_reference(gpu, oracle):
     ps = _problems()
     say(f"hh_correct: {len(ps)} problems vs correct_batch and the sequential reference loop")
