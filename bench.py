@@ -419,13 +419,13 @@ def main():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--heavy", type=int, default=2000)
-    ap.add_argument("--heavy-age", type=int, default=128)
-    ap.add_argument("--pair-heavy", type=int, default=0)
+    ap.add_argument("--heavy-age", type=int, default=256)
+    ap.add_argument("--pair-heavy", type=int, default=2000)
     ap.add_argument("--variant", default="auto",
                     choices=["auto", "persistent", "persistent_pair"],
                     help="auto: autotune variant and knobs on the batch before timing")
     ap.add_argument("--tune-trials", type=int, default=12)
-    ap.add_argument("--refill-min", type=int, default=1)
+    ap.add_argument("--refill-min", type=int, default=4)
     ap.add_argument("--waves-per-simd", type=int, default=2)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--order", default="span", choices=["span", "interleave", "index"])
@@ -461,7 +461,7 @@ def main():
     # or as given on the command line
     tune_info = None
     if args.variant == "auto":
-        res = autotune.tune_final(b, reps=3, max_trials=args.tune_trials, use_cache=False,
+        res = autotune.tune_final(b, reps=5, max_trials=args.tune_trials, use_cache=False,
                                   **step_kw)
         cfg = res.best
         tune_info = {"trials": len(res.trials),

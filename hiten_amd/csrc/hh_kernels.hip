@@ -39,7 +39,7 @@ namespace {
 
 thread_local char g_err[512] = "";
 /* persistent 42-d kernels: see hh_set_tuning */
-hh_tuning_t g_tuning = {2000, 128, 0, 1, 2, 0};
+hh_tuning_t g_tuning = {2000, 256, 2000, 4, 2, 0};
 
 int fail(int code, const char* msg) {
     snprintf(g_err, sizeof g_err, "%s", msg);

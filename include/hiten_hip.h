@@ -62,12 +62,12 @@ int hh_abi_version(void);
  *                     there; HH_VARIANT_PERSISTENT_PAIR moves the wave's
  *                     other trajectories to its resume queue and switches at
  *                     once.  0 = only when no work is left.  Default 2000.
- *   min_age         : default 128
+ *   min_age         : default 256
  *   pair_projected  : HH_VARIANT_PERSISTENT_PAIR: from this projected total
  *                     both waves of the workgroup take the trajectory (pair
- *                     layout); 0 = never (default)
+ *                     layout); 0 = never.  Default 2000.
  *   refill_min      : HH_VARIANT_PERSISTENT_PAIR: idle groups (1..8) a wave
- *                     with live groups gathers before it refills; default 1
+ *                     with live groups gathers before it refills; default 4
  *   waves_per_simd  : HH_VARIANT_PERSISTENT_PAIR: 1 or 2 (default): the
  *                     register budget per lane (512 or 256 VGPRs) and the
  *                     resident grid

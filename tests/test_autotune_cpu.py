@@ -36,10 +36,11 @@ def test_knobs_the_variant_does_not_read_are_skipped():
     res = A.coordinate_descent(lambda c: 1.0 if c.variant == "persistent" else 2.0,
                                max_trials=100)
     assert res.best.variant == "persistent"
+    d = A.Config()
     for t in res.trials:
         if t.config.variant == "persistent":
             assert (t.config.refill_min, t.config.waves_per_simd, t.config.pair_projected) == \
-                (1, 2, 0)
+                (d.refill_min, d.waves_per_simd, d.pair_projected)
 
 
 def test_config_maps_to_tuning_and_block():
