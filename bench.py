@@ -410,8 +410,8 @@ def other_workload(name: str, steps: int, warmup: int, no_cpu: bool) -> dict:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--max-steps", type=int, default=200_000)
