@@ -646,16 +646,23 @@ void k_synodic_scan(SynArgs a, SynWs w) {
 }
 
 /* Pass 2: every listed candidate in a lane of its own (a grid-stride loop
- * over the lists in order), refined into its place in its part's records. */
+ * over the lists laid end to end), refined into its place in its part's
+ * records.  The list lengths are loaded together up front.               */
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_synodic_refine(SynArgs a, SynWs w) {
     const int stride = (int)(gridDim.x * BLOCK);
-    int skip = 0;         /* entries of the lists before list li */
-    for (int li = 0; li < SYN_LISTS; ++li) {
-        const int cnt_li = w.nlist[li * 32];
-        const int e0 = (int)(blockIdx.x * BLOCK + threadIdx.x) - (skip % stride);
-        skip += cnt_li;
-        for (int e = e0 < 0 ? e0 + stride : e0; e < cnt_li; e += stride) {
+    int start[SYN_LISTS + 1];   /* entries of the lists before list li */
+#pragma unroll
+    for (int li = 0; li < SYN_LISTS; ++li) start[li + 1] = w.nlist[li * 32];
+    start[0] = 0;
+#pragma unroll
+    for (int li = 0; li < SYN_LISTS; ++li) start[li + 1] += start[li];
+    for (int q = (int)(blockIdx.x * BLOCK + threadIdx.x); q < start[SYN_LISTS]; q += stride) {
+        int li = 0;
+#pragma unroll
+        for (int j = 1; j < SYN_LISTS; ++j) li = (q >= start[j]) ? j : li;
+        const int e = q - start[li];
+        {
             const int4 ent = w.list[(size_t)li * w.cap_list + e];
             const int i = ent.x / w.parts;
             SynTraj tr;
@@ -694,12 +701,22 @@ __global__ __launch_bounds__(BLOCK) void k_synodic_refine(SynArgs a, SynWs w) {
  * as syn_detect_wave does.                                                */
 __device__ __forceinline__ void syn_merge_traj(SynArgs a, const SynWs& w, int i, int lane) {
     const int* nc = w.ncand + (size_t)i * w.parts;
+    /* the parts' candidate counts, 64 at a time one per lane; with at most
+     * 64 parts lane p keeps the number of records before part p           */
     int np = 0;
     bool over = false;
-    for (int p = 0; p < w.parts; ++p) {
-        const int c = nc[p];
-        over = over || c > SYN_CAP;
-        np += c;
+    int pre = 0;
+    for (int p0 = 0; p0 < w.parts; p0 += 64) {
+        const int c = (p0 + lane < w.parts) ? nc[p0 + lane] : 0;
+        over = over || __any(c > SYN_CAP);
+        int inc = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(inc, o);
+            if (lane >= o) inc += v;
+        }
+        if (p0 == 0) pre = inc - c;
+        np += __shfl(inc, 63);
     }
     if (over) {
         syn_detect_wave(a, i, lane);
@@ -714,7 +731,21 @@ __device__ __forceinline__ void syn_merge_traj(SynArgs a, const SynWs& w, int i,
         double th = 0.0, x[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (e < np) {
             int part = 0, idx = e;
-            while (idx >= nc[part]) idx -= nc[part++];
+            if (w.parts <= 64) {
+                /* the last part whose first record is at or before e (a
+                 * part with no records shares its start with the next)    */
+                int first = 0;
+                for (int q = 1; q < w.parts; ++q) {
+                    const int pq = __builtin_amdgcn_readlane(pre, q);
+                    if (e >= pq) {
+                        part = q;
+                        first = pq;
+                    }
+                }
+                idx = e - first;
+            } else {
+                while (idx >= nc[part]) idx -= nc[part++];
+            }
             const double* r = rec_all + (size_t)part * (SYN_CAP * SYN_REC) + idx;
             th = r[0];
 #pragma unroll
