@@ -1010,9 +1010,6 @@ void k_dop853_final42_plain(Final42Args a) {
     }
 }
 
-/* traj_iterate + FinalOut for one trajectory in the wave-wide layout, from
- * the current (y, k[0], t, h, err_prev, counters) to its end; stores the
- * results as store42 does.                                                  */
 /* Per-attempt hook of wide_finish: scheduling only, never touches results */
 struct NoHook {
     __device__ __forceinline__ void attempt(int, double) {}
