@@ -117,7 +117,7 @@ def ulp_spread(run, x0, ref):
     """Largest change of ``run``'s output when one of the six state components
     of ``x0`` (the last six entries) moves up by one ulp: the rounding
     sensitivity of the case, against which agreement in another arithmetic
-    is judged."""
+    is measured."""
     x0 = np.asarray(x0, float)
     worst = 0.0
     for c in range(x0.size - 6, x0.size):

@@ -21,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PATH = os.path.join(HERE, "golden", "integrator_golden.npz")
 MU_EM = 0.012150585609624
 RTOL = ATOL = 1e-12
-# Agreement in another arithmetic is judged against the case's own rounding
+# Agreement in another arithmetic is measured against the case's own rounding
 # sensitivity: the largest change a one-ulp move of one initial state
 # component makes in the reference's output (stored per case), or the
 # tolerance-proportional global error N_accepted * rtol * max(1, |y|),
